@@ -1,0 +1,235 @@
+/*
+ * lumenwright-hip C ABI: the HQ voxel path tracer on AMD Instinct MI355X (gfx950).
+ *
+ * This is the drop-in boundary for the reference's HQ backend contract:
+ *
+ *   trait Backend { fn name(&self) -> String;
+ *                   fn render_quick(&mut self, &QuickFrame) -> Result<Vec<u32>>;
+ *                   fn render_hq(&mut self, &HqFrame, progress) -> Result<Vec<[f32; 3]>>; }
+ *     crates/lumenwright/src/render/mod.rs:92-103
+ *   struct HqFrame { scene: &HqScene, params: Params, spp: u32, sky: Vec<f32> }
+ *     crates/lumenwright/src/render/mod.rs:80-89
+ *
+ * and for the kernel crate's device ABI (the buffers `pt_render` takes,
+ * crates/lumenwright-ptx/src/lib.rs:37-56). Structs are byte-identical to the
+ * reference's #[repr(C)] types (crates/lumenwright-kernel/src/scene.rs:44-228;
+ * size asserts at crates/lumenwright-kernel/src/trace.rs:327-333).
+ *
+ * Conventions (mirroring anyhow::Result in the reference):
+ *   - every int-returning call returns 0 on success, a negative LW_E* code on failure;
+ *     the message is then available from lw_hip_last_error (per handle) or
+ *     lw_last_error (for calls without a handle);
+ *   - inputs are borrowed for the duration of the call; outputs are caller-owned;
+ *   - one thread at a time per handle (the reference takes &mut self); progress
+ *     callbacks run on the caller's thread.
+ */
+#ifndef LW_ABI_H
+#define LW_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants: crates/lumenwright-kernel/src/scene.rs:16-37,146-155 ---- */
+#define LW_EMPTY 0xFFFFFFFFu
+#define LW_UNIFORM 0x80000000u
+#define LW_BRICK_WORDS 2048u
+#define LW_OV_CELL_WORDS 8192u
+#define LW_ALPHA_OPAQUE 0u
+#define LW_ALPHA_CUTOUT 1u
+#define LW_ALPHA_TRANSLUCENT 2u
+#define LW_FLAG_ALPHA_MASK 3u
+#define LW_FLAG_WATER (1u << 2)
+#define LW_FLAG_SHADE (1u << 3)
+#define LW_FLAG_FRONT_ONLY (1u << 4)
+#define LW_EMITTER_SAMPLING_NONE 0u
+#define LW_EMITTER_SAMPLING_ONE 1u
+#define LW_EMITTER_SAMPLING_ONE_BLOCK 2u
+#define LW_EMITTER_SAMPLING_ALL 3u
+#define LW_EMITTER_GRID_HEADER 8u
+
+/* ---- error codes ---- */
+#define LW_OK 0
+#define LW_EINVAL (-1)  /* malformed scene or argument (the reference would panic on it) */
+#define LW_ENODEV (-2)  /* no HIP device / runtime (cuda.rs:29-41 maps this to an error) */
+#define LW_EHIP (-3)    /* a HIP runtime call failed */
+#define LW_ENOMEM (-4)  /* device allocation failed */
+#define LW_EUNSUPPORTED (-5)
+
+/* GpuQuad, 80 bytes (scene.rs:44-57): p0.w = flags, eu.w = tint rgba8, ev.w = emittance. */
+typedef struct lw_quad {
+  float p0[4];
+  float eu[4];
+  float ev[4];
+  float uv[4];
+  float uvt[4];
+} lw_quad;
+
+/* GpuModel, 16 bytes (scene.rs:101-112). */
+typedef struct lw_model {
+  uint32_t quad_offset;
+  uint32_t quad_count;
+  uint32_t flags;
+  uint32_t pad;
+} lw_model;
+
+/* GpuEmitter, 32 bytes (scene.rs:117-124): pos.w = first quad (bit cast). */
+typedef struct lw_emitter {
+  float pos[4];
+  uint32_t faces[4];
+} lw_emitter;
+
+/* Params, 432 bytes = 27 vec4 (scene.rs:157-228). */
+typedef struct lw_params {
+  float cam_pos[4];
+  float cam_fwd[4];
+  float cam_right[4];
+  float cam_up[4];
+  float sun_dir[4];
+  float sun_color[4];
+  float sky_sun[4];
+  float sky_zenith[4];
+  float perez_a[4];
+  float perez_b[4];
+  float perez_c[4];
+  float perez_d[4];
+  float perez_e[4];
+  float water_color[4];
+  float misc_f[4];
+  uint32_t grid_dims[4];
+  uint32_t off[4];
+  uint32_t off2[4];
+  uint32_t frame[4];
+  uint32_t misc_u[4];
+  uint32_t see_through[4];
+  uint32_t emitter[4];
+  float sun_su[4];
+  float sun_sv[4];
+  float sun_sprite[4];
+  float sky_map[4];
+  float water_plane[4];
+} lw_params;
+
+/* SceneView plus the slice lengths (scene.rs:231-259; the argument list of
+ * pt_render, lumenwright-ptx/src/lib.rs:37-56). */
+typedef struct lw_scene_view {
+  const lw_params *params;
+  const uint32_t *grid;
+  size_t grid_len;
+  const lw_model *models;
+  size_t models_len;
+  const lw_quad *quads;
+  size_t quads_len;
+  const uint32_t *atlas;
+  size_t atlas_len;
+  const lw_emitter *emitters;
+  size_t emitters_len;
+  const uint32_t *emitter_grid;
+  size_t emitter_grid_len;
+  const float *sky;
+  size_t sky_len;
+} lw_scene_view;
+
+/* trace's Hit (trace.rs:11-22) plus the cell the traversal stopped in and whether
+ * anything was hit (trace returns Option<Hit>). */
+typedef struct lw_hit {
+  float t;
+  uint32_t quad;
+  uint32_t texel;
+  float s;
+  float tt;
+  int32_t cell[3];
+  uint32_t hit; /* 1 = Some(hit), 0 = None */
+  uint32_t pad;
+} lw_hit;
+
+/* progress(done, total): the reference's `&(dyn Fn(u32, u32) + Sync)`. */
+typedef void (*lw_progress_fn)(uint32_t done, uint32_t total, void *user);
+
+typedef struct lw_hip_backend lw_hip_backend;
+typedef struct lw_hip_scene lw_hip_scene;
+
+/* Kernel-clock timing of the last render on a handle (the reference logs the same
+ * CUDA-event sum, render/cuda.rs:128-131,154-159,170-173). */
+typedef struct lw_timing {
+  double trace_ms;       /* sum of HIP-event durations of the path-tracing kernels */
+  double accumulate_ms;  /* sum of HIP-event durations of the f64 accumulation kernels */
+  double wall_ms;        /* host wall clock of the whole call */
+  uint64_t samples;      /* width * height * spp rendered */
+  uint32_t launches;     /* path-tracing kernel launches */
+  uint32_t pad;
+} lw_timing;
+
+/* ---- backend: crates/lumenwright/src/render/cuda.rs ---- */
+
+/* Describes device 0 (name and ISA) or says why HIP is unavailable; ~ cuda::probe
+ * (render/cuda.rs:44-51). Writes a NUL-terminated string into desc. */
+int lw_hip_probe(char *desc, size_t cap);
+
+/* Opens a HIP device; ~ CudaBackend::new (render/cuda.rs:55-62). */
+int lw_hip_open(int device, lw_hip_backend **out);
+
+/* Human-readable name, e.g. "amd instinct mi355x (hip)"; ~ Backend::name
+ * (render/mod.rs:94, render/cuda.rs:66-68). Owned by the handle. */
+const char *lw_hip_name(const lw_hip_backend *b);
+
+/* Backend::render_hq (render/mod.rs:98-102; render/cuda.rs:77-184): renders
+ * params.frame[0] x params.frame[1] pixels at max(spp, 1) samples each and writes the
+ * mean linear radiance per pixel, row-major, top row first, to out_rgb (w*h*3 floats).
+ * Non-finite samples are dropped from the sum and still count in the mean
+ * (render/cpu/mod.rs:66-79). Uploads the view's buffers for the call, as the reference
+ * does (render/cuda.rs:96-113). */
+int lw_hip_render_hq(lw_hip_backend *b, const lw_scene_view *view, uint32_t spp,
+                     lw_progress_fn progress, void *user, float *out_rgb);
+
+/* Backend::render_quick (render/mod.rs:96): quick mode has no HIP path; the reference's
+ * CUDA backend falls back to the CPU rasterizer (render/cuda.rs:70-75). Returns
+ * LW_EUNSUPPORTED so the caller can take its own rasterizer. */
+int lw_hip_render_quick(lw_hip_backend *b);
+
+/* The message of the last failure on this handle ("" if none). */
+const char *lw_hip_last_error(const lw_hip_backend *b);
+
+/* The message of the last failure of a call made without a handle. */
+const char *lw_last_error(void);
+
+void lw_hip_close(lw_hip_backend *b);
+
+/* ---- the kernel crate's public functions, on the device (parity surface) ---- */
+
+/* trace (crates/lumenwright-kernel/src/trace.rs:137-290) for n rays: o, d are n*3
+ * floats, tmin/tmax n floats each; out[i].hit says whether ray i met a surface. */
+int lw_hip_trace(lw_hip_backend *b, const lw_scene_view *view, size_t n, const float *o,
+                 const float *d, const float *tmin, const float *tmax, lw_hit *out);
+
+/* render_sample (crates/lumenwright-kernel/src/integrator.rs:539-709) for n
+ * (px, py, sample) triples; out_rgb gets n*3 floats of linear radiance. */
+int lw_hip_render_samples(lw_hip_backend *b, const lw_scene_view *view, size_t n,
+                          const uint32_t *px, const uint32_t *py, const uint32_t *sample,
+                          float *out_rgb);
+
+/* ---- resident scenes: upload once, render many frames (the reference re-uploads on
+ * every call, render/cuda.rs:96-113) ---- */
+
+/* Validates and uploads the scene buffers of a view (everything but params), so that
+ * later renders read them from HBM. The view's params are only used for validation. */
+int lw_hip_scene_upload(lw_hip_backend *b, const lw_scene_view *view, lw_hip_scene **out);
+
+/* render_hq over a resident scene with the given params (validated against it). */
+int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene,
+                           const lw_params *params, uint32_t spp, lw_progress_fn progress,
+                           void *user, float *out_rgb);
+
+void lw_hip_scene_free(lw_hip_backend *b, lw_hip_scene *scene);
+
+/* Kernel-clock timing of the last render_hq / render_hq_scene on this handle. */
+int lw_hip_last_timing(const lw_hip_backend *b, lw_timing *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LW_ABI_H */

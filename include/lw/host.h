@@ -1,0 +1,156 @@
+/*
+ * lumenwright-hip host-side frame preparation (C ABI).
+ *
+ * The reference prepares each HQ frame on the host before the backend call: the sun and
+ * the Preetham sky (crates/lumenwright/src/render/sky.rs), the camera basis
+ * (crates/lumenwright/src/scene/camera.rs), the kernel's Params (request.rs:1153-1225), the
+ * sky cache (sky.rs:593-611 over integrator.rs:133-148) and the scene buffers
+ * (scene/voxel.rs). These are the C++ counterparts, so a caller without the reference's
+ * Rust crate (this repository's tests and bench) can build a frame. f64 code follows the
+ * reference with the same libm calls; f32 code follows its evaluation order with no FMA
+ * contraction.
+ *
+ * The synthetic worlds stand in for Minecraft saves, which need the game's client jar: a
+ * value-noise terrain laid out exactly as build_hq lays out a loaded world
+ * (scene/voxel.rs:111-491, emitter_list :542-607).
+ */
+#ifndef LW_HOST_H
+#define LW_HOST_H
+
+#include "lw/abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* sky.rs:28-32: vanilla celestial angle (0 = noon) for a time of day in ticks. */
+float lw_celestial_angle(int64_t ticks);
+/* sky.rs:35-38: unit vector towards the sun for a celestial angle. */
+void lw_sun_direction_from_angle(float angle, float out[3]);
+/* sky.rs:41-44: sun direction from altitude/azimuth in degrees. */
+void lw_sun_direction(float altitude_deg, float azimuth_deg, float out[3]);
+/* sky.rs:625-629 */
+float lw_twilight(float altitude);
+
+/* HqSky (sky.rs:549-557), built by hq_sky (sky.rs:636-644). */
+typedef struct lw_hq_sky {
+  float sun_dir[3];
+  float sun_color[3];
+  float sky_light;
+} lw_hq_sky;
+void lw_hq_sky_new(const float sun_dir[3], float sun_intensity, float sky_intensity,
+                   lw_hq_sky *out);
+
+/* Preetham (sky.rs:656-664), built by Preetham::new (sky.rs:666-768) at the given
+ * turbidity (the reference fixes it at 2.5, sky.rs:667). */
+typedef struct lw_preetham {
+  float sun[3];
+  float zenith[3];
+  float perez[5][3];
+} lw_preetham;
+void lw_preetham_new(const float sun_dir[3], double turbidity, lw_preetham *out);
+
+/* sky.rs:574-584 */
+void lw_sun_texture_axes(const float sw[3], float su[3], float sv[3]);
+
+/* sky.rs:593-611: the sky cache for params (resolution params.misc_f[2]); writes
+ * (n + 1)^2 * 3 floats. Returns the count written, or a negative error if cap is short. */
+long lw_sky_cache(const lw_params *params, float *out, size_t cap);
+
+/* The 256 gamma-decoded byte values: gamma_to_linear(k / 255) (math.rs:213-216,
+ * scene.rs:299-306), with libm's expf. */
+void lw_gamma_lut(float out[256]);
+
+/* Camera::basis (camera.rs:20-37) for Minecraft yaw/pitch in degrees. */
+void lw_camera_basis(float yaw_deg, float pitch_deg, float fwd[3], float right[3], float up[3]);
+
+/* HqOptions (request.rs:180-210) as hq_params reads them. */
+typedef struct lw_hq_options {
+  uint32_t spp;
+  uint32_t max_bounces;
+  uint32_t seed;
+  uint32_t emitter_sampling; /* LW_EMITTER_SAMPLING_* */
+  uint32_t prevent_normal_emitter;
+} lw_hq_options;
+
+/* The HqScene facts hq_params reads (voxel.rs:19-49). */
+typedef struct lw_scene_info {
+  uint32_t dims[3];
+  uint32_t offsets[5];
+  uint32_t atlas_size[2];
+  uint32_t quads_len;
+  uint32_t models_len;
+  uint32_t emitter_count;
+  uint32_t see_through[2];
+  uint32_t has_sun_sprite;
+  uint32_t sun_sprite[4];
+} lw_scene_info;
+
+/* CameraRays (render/mod.rs:172-183). */
+typedef struct lw_camera_rays {
+  float forward[3];
+  float right[3];
+  float up[3];
+  float tan_half_fov;
+  float aspect;
+} lw_camera_rays;
+
+/* The camera rays of a frame (request.rs:643-652): Camera::basis, tan(fov_y / 2) with
+ * fov_y in degrees, and width / height. */
+void lw_camera_rays_new(float yaw_deg, float pitch_deg, float fov_y_deg, uint32_t width,
+                        uint32_t height, lw_camera_rays *out);
+
+/* hq_params (request.rs:1153-1225), with the Preetham sky at `turbidity`. */
+void lw_hq_params(const lw_hq_options *o, float emitter_intensity, uint32_t width, uint32_t height,
+                  const lw_scene_info *scene, const float eye[3], const lw_camera_rays *rays,
+                  const lw_hq_sky *sky, double turbidity, lw_params *out);
+
+/* ---- synthetic worlds ---- */
+
+typedef struct lw_world_spec {
+  uint32_t chunks_x, chunks_z; /* footprint in 16-block chunks */
+  uint32_t sections;           /* height in 16-block sections */
+  uint64_t seed;               /* terrain, block types, emitters, plants */
+  uint64_t texture_seed;       /* atlas texels and emittances */
+  uint32_t cube_types;         /* opaque full-cube block types */
+  uint32_t cutout_types;       /* alpha-tested cross plants */
+  int32_t water_level;         /* non-solid cells below this height hold water; < 0: none */
+  float emitter_fraction;      /* of solid cells with a non-solid neighbour */
+  float cutout_fraction;       /* of columns whose surface is above the water */
+  uint32_t pad;
+} lw_world_spec;
+
+/* The HQ scene of a synthetic world (an owned HqScene). */
+typedef struct lw_world {
+  uint32_t *grid;
+  size_t grid_len;
+  lw_model *models;
+  size_t models_len;
+  lw_quad *quads;
+  size_t quads_len;
+  uint32_t *atlas;
+  size_t atlas_len;
+  lw_emitter *emitters;
+  size_t emitters_len;
+  uint32_t *emitter_grid;
+  size_t emitter_grid_len;
+  lw_scene_info info;
+  /* what the builder made (voxel.rs HqStats) */
+  uint64_t bricks, uniform_bricks, buried_bricks, solid_cells, water_cells, cutout_cells;
+} lw_world;
+
+/* The terrain height h(x, z) = floor(64 + 24 V(x, z)) of a spec, V the 4-octave value noise. */
+int32_t lw_world_height(const lw_world_spec *spec, int32_t x, int32_t z);
+int lw_world_build(const lw_world_spec *spec, lw_world **out);
+void lw_world_free(lw_world *w);
+
+/* A procedural dusk sky as the game look's two equirectangular maps (the lens's, then the
+ * light's; layout of integrator.rs:190-243): writes lw*lh*3 + gw*gh*3 floats. */
+long lw_dusk_sky_maps(const float sun_dir[3], uint32_t lw, uint32_t lh, uint32_t gw, uint32_t gh,
+                      float *out, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LW_HOST_H */

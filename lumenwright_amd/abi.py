@@ -1,0 +1,165 @@
+"""Python mirror of include/lw/abi.h: numpy dtypes of the scene ABI and the ctypes view.
+
+The record layouts are byte-identical to the reference's ``#[repr(C)]`` types
+(crates/lumenwright-kernel/src/scene.rs:44-228; sizes asserted at
+crates/lumenwright-kernel/src/trace.rs:327-333).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+# scene.rs:16-37,146-155
+EMPTY = 0xFFFFFFFF
+UNIFORM = 0x80000000
+BRICK_WORDS = 2048
+OV_CELL_WORDS = 8192
+ALPHA_OPAQUE = 0
+ALPHA_CUTOUT = 1
+ALPHA_TRANSLUCENT = 2
+FLAG_ALPHA_MASK = 3
+FLAG_WATER = 1 << 2
+FLAG_SHADE = 1 << 3
+FLAG_FRONT_ONLY = 1 << 4
+EMITTER_SAMPLING_NONE = 0
+EMITTER_SAMPLING_ONE = 1
+EMITTER_SAMPLING_ONE_BLOCK = 2
+EMITTER_SAMPLING_ALL = 3
+EMITTER_GRID_HEADER = 8
+
+# GpuQuad (scene.rs:44-57), 80 bytes.
+QUAD = np.dtype([("p0", "<f4", 4), ("eu", "<f4", 4), ("ev", "<f4", 4), ("uv", "<f4", 4),
+                 ("uvt", "<f4", 4)])
+# GpuModel (scene.rs:101-112), 16 bytes.
+MODEL = np.dtype([("quad_offset", "<u4"), ("quad_count", "<u4"), ("flags", "<u4"), ("pad", "<u4")])
+# GpuEmitter (scene.rs:117-124), 32 bytes.
+EMITTER = np.dtype([("pos", "<f4", 4), ("faces", "<u4", 4)])
+
+_PARAM_FIELDS = [
+    ("cam_pos", "<f4"), ("cam_fwd", "<f4"), ("cam_right", "<f4"), ("cam_up", "<f4"),
+    ("sun_dir", "<f4"), ("sun_color", "<f4"), ("sky_sun", "<f4"), ("sky_zenith", "<f4"),
+    ("perez_a", "<f4"), ("perez_b", "<f4"), ("perez_c", "<f4"), ("perez_d", "<f4"),
+    ("perez_e", "<f4"), ("water_color", "<f4"), ("misc_f", "<f4"), ("grid_dims", "<u4"),
+    ("off", "<u4"), ("off2", "<u4"), ("frame", "<u4"), ("misc_u", "<u4"),
+    ("see_through", "<u4"), ("emitter", "<u4"), ("sun_su", "<f4"), ("sun_sv", "<f4"),
+    ("sun_sprite", "<f4"), ("sky_map", "<f4"), ("water_plane", "<f4"),
+]
+# Params (scene.rs:157-228), 432 bytes = 27 vec4.
+PARAMS = np.dtype([(name, t, 4) for name, t in _PARAM_FIELDS])
+# lw_hit: trace's Hit (trace.rs:11-22) + final cell + hit flag.
+HIT = np.dtype([("t", "<f4"), ("quad", "<u4"), ("texel", "<u4"), ("s", "<f4"), ("tt", "<f4"),
+                ("cell", "<i4", 3), ("hit", "<u4"), ("pad", "<u4")])
+
+assert QUAD.itemsize == 80 and MODEL.itemsize == 16 and EMITTER.itemsize == 32
+assert PARAMS.itemsize == 432 and HIT.itemsize == 40
+
+
+def f32_bits(u: int) -> np.float32:
+    """A u32 bit pattern as the f32 that holds it (Rust ``f32::from_bits``)."""
+    return np.array([u], dtype="<u4").view("<f4")[0]
+
+
+def bits_f32(f) -> int:
+    """The bit pattern of an f32 (Rust ``f32::to_bits``)."""
+    return int(np.array([f], dtype="<f4").view("<u4")[0])
+
+
+def new_params() -> np.ndarray:
+    """A zeroed Params record (``Params::default()``)."""
+    return np.zeros((), dtype=PARAMS)
+
+
+class SceneViewC(ctypes.Structure):
+    """lw_scene_view: SceneView plus slice lengths (scene.rs:231-259)."""
+
+    _fields_ = [
+        ("params", ctypes.c_void_p),
+        ("grid", ctypes.c_void_p), ("grid_len", ctypes.c_size_t),
+        ("models", ctypes.c_void_p), ("models_len", ctypes.c_size_t),
+        ("quads", ctypes.c_void_p), ("quads_len", ctypes.c_size_t),
+        ("atlas", ctypes.c_void_p), ("atlas_len", ctypes.c_size_t),
+        ("emitters", ctypes.c_void_p), ("emitters_len", ctypes.c_size_t),
+        ("emitter_grid", ctypes.c_void_p), ("emitter_grid_len", ctypes.c_size_t),
+        ("sky", ctypes.c_void_p), ("sky_len", ctypes.c_size_t),
+    ]
+
+
+class TimingC(ctypes.Structure):
+    """lw_timing."""
+
+    _fields_ = [("trace_ms", ctypes.c_double), ("accumulate_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double), ("samples", ctypes.c_uint64),
+                ("launches", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    if a is None or a.size == 0:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+@dataclass
+class HqScene:
+    """The HQ scene buffers (crates/lumenwright/src/scene/voxel.rs:17-49)."""
+
+    grid: np.ndarray
+    models: np.ndarray
+    quads: np.ndarray
+    atlas: np.ndarray
+    atlas_size: tuple
+    dims: tuple
+    offsets: tuple
+    see_through: tuple = (0, 0)
+    emitters: np.ndarray = field(default_factory=lambda: np.zeros(1, dtype=EMITTER))
+    emitter_count: int = 0
+    emitter_grid: np.ndarray = field(default_factory=lambda: np.zeros(8, dtype="<u4"))
+    sun_sprite: Optional[tuple] = None
+
+    def memory_bytes(self) -> int:
+        """voxel.rs:91-99"""
+        return (self.grid.nbytes + self.models.nbytes + self.quads.nbytes + self.atlas.nbytes
+                + self.emitters.nbytes + self.emitter_grid.nbytes)
+
+
+@dataclass
+class HqFrame:
+    """HqFrame (crates/lumenwright/src/render/mod.rs:80-89)."""
+
+    scene: HqScene
+    params: np.ndarray
+    spp: int
+    sky: np.ndarray
+
+    def view(self) -> "BoundView":
+        return BoundView(self.params, self.scene.grid, self.scene.models, self.scene.quads,
+                         self.scene.atlas, self.scene.emitters, self.scene.emitter_grid,
+                         self.sky)
+
+
+class BoundView:
+    """A ctypes lw_scene_view over numpy buffers, keeping the buffers alive."""
+
+    def __init__(self, params, grid, models, quads, atlas, emitters, emitter_grid, sky):
+        self.params = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        self.grid = np.ascontiguousarray(grid, dtype="<u4")
+        self.models = np.ascontiguousarray(models, dtype=MODEL)
+        self.quads = np.ascontiguousarray(quads, dtype=QUAD)
+        self.atlas = np.ascontiguousarray(atlas, dtype="<u4")
+        self.emitters = np.ascontiguousarray(emitters, dtype=EMITTER)
+        self.emitter_grid = np.ascontiguousarray(emitter_grid, dtype="<u4")
+        self.sky = np.ascontiguousarray(sky if sky is not None else np.zeros(0), dtype="<f4")
+        self.c = SceneViewC(
+            self.params.ctypes.data,
+            _ptr(self.grid), self.grid.size, _ptr(self.models), self.models.size,
+            _ptr(self.quads), self.quads.size, _ptr(self.atlas), self.atlas.size,
+            _ptr(self.emitters), self.emitters.size, _ptr(self.emitter_grid),
+            self.emitter_grid.size, _ptr(self.sky), self.sky.size)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.c)

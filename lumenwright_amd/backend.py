@@ -1,0 +1,159 @@
+"""The HIP backend behind the reference's Backend contract.
+
+Mirrors ``trait Backend { name, render_quick, render_hq }``
+(crates/lumenwright/src/render/mod.rs:92-103) and CudaBackend's construction and errors
+(crates/lumenwright/src/render/cuda.rs:29-75). Every call goes through the C ABI of
+include/lw/abi.h; there is no CPU path here.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _native
+from .abi import HIT, PARAMS, BoundView, HqFrame, TimingC
+
+LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+
+
+class BackendError(RuntimeError):
+    """A failed backend call (the reference's anyhow::Error); `code` is the LW_E* value."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def probe() -> str:
+    """Describes device 0 or says why HIP is unavailable (render/cuda.rs:44-51)."""
+    buf = ctypes.create_string_buffer(512)
+    rc = _native.lib().lw_hip_probe(buf, 512)
+    if rc != LW_OK:
+        raise BackendError(rc, buf.value.decode())
+    return buf.value.decode()
+
+
+def _addr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class HipScene:
+    """A scene validated and resident in device memory (lw_hip_scene)."""
+
+    def __init__(self, backend: "HipBackend", handle: int, view: BoundView):
+        self.backend = backend
+        self.handle = handle
+        self.params = view.params.copy()
+
+    def free(self):
+        if self.handle:
+            _native.lib().lw_hip_scene_free(self.backend.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HipBackend:
+    """BackendKind::Hip: the HQ path tracer on an AMD GPU."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    @classmethod
+    def open(cls, device: int = 0) -> "HipBackend":
+        """CudaBackend::new's counterpart (render/cuda.rs:55-62)."""
+        L = _native.lib()
+        h = ctypes.c_void_p()
+        rc = L.lw_hip_open(device, ctypes.byref(h))
+        if rc != LW_OK:
+            raise BackendError(rc, "HIP: " + L.lw_last_error().decode())
+        return cls(h.value)
+
+    def _check(self, rc: int):
+        if rc != LW_OK:
+            raise BackendError(rc, _native.lib().lw_hip_last_error(self.handle).decode())
+
+    def name(self) -> str:
+        """Backend::name (render/mod.rs:94)."""
+        return _native.lib().lw_hip_name(self.handle).decode()
+
+    def render_quick(self, frame=None):
+        """Backend::render_quick: there is no HIP rasterizer; the caller takes the CPU
+        rasterizer, as the reference's CUDA backend does (render/cuda.rs:70-75)."""
+        self._check(_native.lib().lw_hip_render_quick(self.handle))
+
+    def render_hq(self, frame: HqFrame,
+                  progress: Optional[Callable[[int, int], None]] = None) -> np.ndarray:
+        """Backend::render_hq (render/mod.rs:98-102): the mean linear radiance per pixel,
+        row-major, top row first, as (w*h, 3) float32."""
+        view = frame.view()
+        w, h = int(view.params["frame"][0]), int(view.params["frame"][1])
+        out = np.zeros((w * h, 3), dtype="<f4")
+        cb = _native.PROGRESS_FN((lambda d, t, u: progress(d, t)) if progress else 0)
+        self._check(_native.lib().lw_hip_render_hq(self.handle, view.ref, frame.spp, cb, None,
+                                                   _addr(out)))
+        return out
+
+    def upload(self, frame: HqFrame) -> HipScene:
+        """Validates and uploads the frame's scene buffers once (lw_hip_scene_upload)."""
+        view = frame.view()
+        h = ctypes.c_void_p()
+        self._check(_native.lib().lw_hip_scene_upload(self.handle, view.ref, ctypes.byref(h)))
+        return HipScene(self, h.value, view)
+
+    def render_hq_scene(self, scene: HipScene, params: np.ndarray, spp: int,
+                        progress: Optional[Callable[[int, int], None]] = None) -> np.ndarray:
+        """render_hq over a resident scene."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        w, h = int(p["frame"][0]), int(p["frame"][1])
+        out = np.zeros((w * h, 3), dtype="<f4")
+        cb = _native.PROGRESS_FN((lambda d, t, u: progress(d, t)) if progress else 0)
+        self._check(_native.lib().lw_hip_render_hq_scene(self.handle, scene.handle, _addr(p), spp,
+                                                         cb, None, _addr(out)))
+        return out
+
+    def last_timing(self) -> dict:
+        t = TimingC()
+        self._check(_native.lib().lw_hip_last_timing(self.handle, ctypes.byref(t)))
+        return {"trace_ms": t.trace_ms, "accumulate_ms": t.accumulate_ms, "wall_ms": t.wall_ms,
+                "samples": int(t.samples), "launches": int(t.launches)}
+
+    def trace(self, view: BoundView, o, d, tmin=None, tmax=None) -> np.ndarray:
+        """trace (trace.rs:137-290) on the device for rays (n, 3); HIT records."""
+        o = np.ascontiguousarray(o, dtype="<f4").reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype="<f4").reshape(-1, 3)
+        n = o.shape[0]
+        tmin = np.ascontiguousarray(np.broadcast_to(0.0 if tmin is None else tmin, (n,)), "<f4")
+        tmax = np.ascontiguousarray(np.broadcast_to(np.inf if tmax is None else tmax, (n,)), "<f4")
+        out = np.zeros(n, dtype=HIT)
+        self._check(_native.lib().lw_hip_trace(self.handle, view.ref, n, _addr(o), _addr(d),
+                                               _addr(tmin), _addr(tmax), _addr(out)))
+        return out
+
+    def render_samples(self, view: BoundView, px, py, sample) -> np.ndarray:
+        """render_sample (integrator.rs:539-709) on the device; (n, 3) float32."""
+        px = np.ascontiguousarray(px, dtype="<u4")
+        py = np.ascontiguousarray(py, dtype="<u4")
+        sample = np.ascontiguousarray(sample, dtype="<u4")
+        out = np.zeros((px.size, 3), dtype="<f4")
+        self._check(_native.lib().lw_hip_render_samples(self.handle, view.ref, px.size, _addr(px),
+                                                        _addr(py), _addr(sample), _addr(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            _native.lib().lw_hip_close(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,663 @@
+// HIP backend of the HQ path tracer: the gfx950 kernels and the C ABI of include/lw/abi.h.
+//
+// Frame structure (one render_hq call), mirroring CudaBackend::render_hq
+// (crates/lumenwright/src/render/cuda.rs:77-184) with the per-pixel sum in f64:
+//   for each launch of `batch` samples, in sample order:
+//     k_render:     one thread per (pixel, sample); a wave is an 8 x 8 pixel tile of one
+//                   sample index, so primary rays of a wave are coherent. Each thread
+//                   writes the sample's f32 radiance to a staging buffer.
+//     k_accumulate: one thread per pixel adds the batch's finite samples, in sample order,
+//                   to the pixel's f64 sums.
+//   k_resolve:      mean = sum / spp, rounded to f32 (render/cpu/mod.rs:75-79).
+// Because every pixel's f64 additions happen in the same order as the CPU backend's
+// (sample 0, 1, 2, ...), the image equals the CPU backend's wherever the samples do.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lw/abi.h"
+#include "lw_device.hpp"
+#include "lw_host.hpp"
+
+using namespace lw;
+
+// ------------------------------------------------------------------------------------
+// kernels
+
+namespace {
+
+constexpr int kBlock = 256;  // 4 waves of 64: a 16 x 16 pixel tile
+
+__device__ __forceinline__ void load_lut(float *lut_s, const float *lut) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) lut_s[i] = lut[i];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) k_render(const lw_params p, const DevScene s,
+                                                   uint32_t sample_start, float *__restrict__ rad) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint32_t tiles_x = (w + 15u) / 16u;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  if (px >= w || py >= h) return;
+  const uint32_t sl = blockIdx.y;
+  const V3 c = render_sample(s, p, lut_s, px, py, sample_start + sl);
+  float *o = rad + ((size_t)sl * h * w + (size_t)py * w + px) * 3;
+  o[0] = c.x;
+  o[1] = c.y;
+  o[2] = c.z;
+}
+
+__global__ void __launch_bounds__(kBlock) k_accumulate(uint32_t npix, uint32_t samples,
+                                                       const float *__restrict__ rad,
+                                                       double *__restrict__ acc) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  double a0 = acc[3 * (size_t)i], a1 = acc[3 * (size_t)i + 1], a2 = acc[3 * (size_t)i + 2];
+  for (uint32_t sl = 0; sl < samples; sl++) {
+    const float *c = rad + ((size_t)sl * npix + i) * 3;
+    const float r = c[0], g = c[1], b = c[2];
+    if (isfinite(r) && isfinite(g) && isfinite(b)) {
+      a0 += (double)r;
+      a1 += (double)g;
+      a2 += (double)b;
+    }
+  }
+  acc[3 * (size_t)i] = a0;
+  acc[3 * (size_t)i + 1] = a1;
+  acc[3 * (size_t)i + 2] = a2;
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero(double *__restrict__ acc, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) acc[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_resolve(uint32_t npix, uint32_t spp,
+                                                    const double *__restrict__ acc,
+                                                    float *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3u * npix) return;
+  out[i] = (float)(acc[i] / (double)spp);
+}
+
+__global__ void __launch_bounds__(kBlock) k_trace(const lw_params p, const DevScene s, uint32_t n,
+                                                  const float *__restrict__ o,
+                                                  const float *__restrict__ d,
+                                                  const float *__restrict__ tmin,
+                                                  const float *__restrict__ tmax,
+                                                  lw_hit *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Hit h;
+  int32_t cell[3];
+  const bool hit = trace(s, p, V3{o[3 * i], o[3 * i + 1], o[3 * i + 2]},
+                         V3{d[3 * i], d[3 * i + 1], d[3 * i + 2]}, tmin[i], tmax[i], h, cell);
+  lw_hit r;
+  r.t = hit ? h.t : INFINITY;
+  r.quad = hit ? h.quad : UINT32_MAX;
+  r.texel = hit ? h.texel : 0u;
+  r.s = hit ? h.s : 0.0f;
+  r.tt = hit ? h.tt : 0.0f;
+  r.cell[0] = cell[0];
+  r.cell[1] = cell[1];
+  r.cell[2] = cell[2];
+  r.hit = hit ? 1u : 0u;
+  r.pad = 0u;
+  out[i] = r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_render_samples(const lw_params p, const DevScene s,
+                                                           uint32_t n,
+                                                           const uint32_t *__restrict__ px,
+                                                           const uint32_t *__restrict__ py,
+                                                           const uint32_t *__restrict__ sample,
+                                                           float *__restrict__ out) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const V3 c = render_sample(s, p, lut_s, px[i], py[i], sample[i]);
+  out[3 * i] = c.x;
+  out[3 * i + 1] = c.y;
+  out[3 * i + 2] = c.z;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// host side
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError {
+  hipError_t code;
+  std::string what;
+};
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) throw HipError{e_, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+struct InvalidScene {
+  std::string what;
+};
+
+template <typename T>
+struct DevBuf {
+  T *ptr = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    n = count;
+    if (count) {
+      hipError_t e = hipMalloc((void **)&ptr, count * sizeof(T));
+      if (e != hipSuccess) {
+        ptr = nullptr;
+        n = 0;
+        throw HipError{e, "hipMalloc(" + std::to_string(count * sizeof(T)) + " bytes)"};
+      }
+    }
+  }
+  void upload(const T *src, size_t count, hipStream_t st) {
+    alloc(count ? count : 1);
+    if (count) HIP_TRY(hipMemcpyAsync(ptr, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void free() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+// The params fields that decide which buffer words a kernel reads; a resident scene is
+// validated once for these, so a render's params must repeat them.
+struct LayoutKey {
+  uint32_t grid_dims[4], off[4], off2[4], see_through[4], strategy_on;
+  float sky_res, sky_map[4];
+  bool operator==(const LayoutKey &o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+LayoutKey layout_key(const lw_params &p) {
+  LayoutKey k;
+  std::memset(&k, 0, sizeof(k));
+  std::memcpy(k.grid_dims, p.grid_dims, sizeof(k.grid_dims));
+  std::memcpy(k.off, p.off, sizeof(k.off));
+  std::memcpy(k.off2, p.off2, sizeof(k.off2));
+  std::memcpy(k.see_through, p.see_through, sizeof(k.see_through));
+  k.strategy_on = (p.misc_u[3] > 0 && p.emitter[0] != LW_EMITTER_SAMPLING_NONE) ? 1u : 0u;
+  k.sky_res = p.misc_f[2];
+  std::memcpy(k.sky_map, p.sky_map, sizeof(k.sky_map));
+  return k;
+}
+
+}  // namespace
+
+struct lw_hip_scene {
+  DevBuf<uint32_t> grid;
+  DevBuf<lw_model> models;
+  DevBuf<lw_quad> quads;
+  DevBuf<uint32_t> qinfo;
+  DevBuf<uint32_t> atlas;
+  DevBuf<lw_emitter> emitters;
+  DevBuf<uint32_t> egrid;
+  DevBuf<float> sky;
+  DevBuf<float> lut;
+  LayoutKey key;
+  DevScene dev() const {
+    return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
+                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr};
+  }
+};
+
+struct lw_hip_backend {
+  int device = 0;
+  std::string name;
+  std::string last_error;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf<float> rad;
+  DevBuf<double> acc;
+  DevBuf<float> out;
+  lw_timing timing{};
+};
+
+namespace {
+
+void set_err(lw_hip_backend *b, const std::string &msg) {
+  if (b) b->last_error = msg;
+  g_last_error = msg;
+}
+
+// Every index the kernels compute from the scene buffers stays in bounds: the checks the
+// reference's slice indexing makes at run time (a panic there), made once here.
+void validate(const lw_scene_view &v) {
+  auto fail = [](const std::string &m) { throw InvalidScene{m}; };
+  if (!v.params) fail("params is null");
+  const lw_params &p = *v.params;
+  const uint64_t glen = v.grid_len;
+  if (!v.grid || !v.models || !v.quads || !v.atlas || !v.emitters || !v.emitter_grid)
+    fail("a scene buffer is null");
+  if (v.models_len == 0 || v.quads_len == 0 || v.emitters_len == 0 || v.atlas_len == 0)
+    fail("models, quads, emitters and atlas must not be empty");
+  const uint64_t dx = p.grid_dims[0], dy = p.grid_dims[1], dz = p.grid_dims[2];
+  if (dx == 0 || dy == 0 || dz == 0 || dx * 16 > 0x7FFFFFF || dy * 16 > 0x7FFFFFF ||
+      dz * 16 > 0x7FFFFFF)
+    fail("grid_dims out of range");
+  const uint64_t nb = dx * dy * dz;
+  if (nb > 0xFFFFFFFFull) fail("too many bricks");
+  if ((uint64_t)p.off[0] + nb > glen || (uint64_t)p.off[2] + nb > glen)
+    fail("top-level or overlay entries out of the grid buffer");
+  // models: every quad of every model is in the quad buffer.
+  for (size_t m = 0; m < v.models_len; m++)
+    if ((uint64_t)v.models[m].quad_offset + v.models[m].quad_count > v.quads_len)
+      fail("model " + std::to_string(m) + " names quads past the quad buffer");
+  for (uint64_t b = 0; b < nb; b++) {
+    const uint32_t e = v.grid[p.off[0] + b];
+    if (e != LW_EMPTY) {
+      if (e & LW_UNIFORM) {
+        if ((e & ~LW_UNIFORM) >= v.models_len) fail("uniform brick with an unknown model");
+      } else {
+        const uint64_t base = (uint64_t)p.off[1] + (uint64_t)e * LW_BRICK_WORDS;
+        if (base + LW_BRICK_WORDS > glen || base + LW_BRICK_WORDS > 0xFFFFFFFFull)
+          fail("brick index past the grid buffer");
+        for (uint32_t k = 0; k < LW_BRICK_WORDS; k++) {
+          const uint32_t w = v.grid[base + k];
+          if ((w & 0xFFFFu) >= v.models_len || (w >> 16) >= v.models_len)
+            fail("voxel with an unknown model id");
+        }
+      }
+    }
+    const uint32_t o = v.grid[p.off[2] + b];
+    if (o != LW_EMPTY) {
+      const uint64_t base = (uint64_t)p.off[3] + (uint64_t)o * LW_OV_CELL_WORDS;
+      if (base + LW_OV_CELL_WORDS > glen || base + LW_OV_CELL_WORDS > 0xFFFFFFFFull)
+        fail("overlay brick past the grid buffer");
+      for (uint32_t c = 0; c < 4096; c++) {
+        const uint64_t off = v.grid[base + 2 * c], cnt = v.grid[base + 2 * c + 1];
+        const uint64_t first = (uint64_t)p.off2[0] + off;
+        if (first + cnt > glen || first + cnt > 0xFFFFFFFFull) fail("overlay cell past the grid buffer");
+        for (uint64_t i = 0; i < cnt; i++)
+          if (v.grid[first + i] >= v.quads_len) fail("overlay quad index past the quad buffer");
+      }
+    }
+  }
+  const int64_t aw = (int32_t)p.off2[1], ah = (int32_t)p.off2[2];
+  if (aw < 1 || ah < 1 || (uint64_t)(aw * ah) > v.atlas_len) fail("atlas size does not fit the atlas");
+  if ((uint64_t)p.see_through[0] + p.see_through[1] > v.quads_len) fail("see-through range past the quads");
+  for (size_t q = 0; q < v.quads_len; q++) {
+    const lw_quad &Q = v.quads[q];
+    (void)Q;
+  }
+  // emitters: the grid's cells, lists and emitters stay in their buffers.
+  if (p.misc_u[3] > 0 && p.emitter[0] != LW_EMITTER_SAMPLING_NONE) {
+    const uint32_t *g = v.emitter_grid;
+    if (v.emitter_grid_len < LW_EMITTER_GRID_HEADER) fail("emitter grid shorter than its header");
+    const uint64_t cells = (uint64_t)g[1] * g[2] * g[3];
+    const uint64_t list = LW_EMITTER_GRID_HEADER + cells * 2;
+    if (list > v.emitter_grid_len) fail("emitter grid cells past the buffer");
+    if ((int32_t)g[1] < 0 || (int32_t)g[2] < 0 || (int32_t)g[3] < 0) fail("emitter grid size");
+    for (uint64_t c = 0; c < cells; c++) {
+      const uint64_t start = g[LW_EMITTER_GRID_HEADER + 2 * c], cnt = g[LW_EMITTER_GRID_HEADER + 2 * c + 1];
+      if (list + start + cnt > v.emitter_grid_len) fail("emitter cell list past the buffer");
+      for (uint64_t k = 0; k < cnt; k++)
+        if (g[list + start + k] >= v.emitters_len) fail("emitter index past the emitter buffer");
+    }
+    for (size_t e = 0; e < v.emitters_len; e++) {
+      uint32_t first;
+      std::memcpy(&first, &v.emitters[e].pos[3], 4);
+      if ((uint64_t)first + v.emitters[e].faces[0] > v.quads_len) fail("emitter faces past the quads");
+    }
+  }
+  // sky: the cache or the two maps the integrator reads.
+  if (p.sky_map[0] >= 1.0f) {
+    for (int k = 1; k < 4; k++)
+      if (!(p.sky_map[k] >= 1.0f)) fail("sky map sizes must be at least 1");
+    const uint64_t w0 = as_u32(p.sky_map[0]), h0 = as_u32(p.sky_map[1]);
+    const uint64_t w1 = as_u32(p.sky_map[2]), h1 = as_u32(p.sky_map[3]);
+    const uint64_t base = (uint64_t)as_u32(p.sky_map[0] * p.sky_map[1]) * 3;
+    if (w0 * h0 * 3 > v.sky_len || base + w1 * h1 * 3 > v.sky_len || !v.sky) fail("sky maps past the sky buffer");
+  } else if (p.misc_f[2] >= 1.0f) {
+    const uint64_t m = (uint64_t)as_u32(p.misc_f[2]) + 1;
+    if (m * m * 3 > v.sky_len || !v.sky) fail("sky cache past the sky buffer");
+  }
+}
+
+// The axis code of a quad (QI_* in lw_device.hpp).
+uint32_t quad_info(const lw_quad &q) {
+  for (int c = 0; c < 4; c++)
+    for (const float *a : {q.p0, q.eu, q.ev})
+      if (!std::isfinite(a[c]) && c < 3) return 0u;
+  auto axis_of = [](const float *e, int &axis, bool &neg) {
+    int nz = 0;
+    for (int c = 0; c < 3; c++) {
+      if (e[c] == 1.0f || e[c] == -1.0f) {
+        axis = c;
+        neg = e[c] < 0.0f;
+        nz++;
+      } else if (e[c] != 0.0f) {
+        return false;
+      }
+    }
+    return nz == 1;
+  };
+  int i, j;
+  bool nu, nv;
+  if (!axis_of(q.eu, i, nu) || !axis_of(q.ev, j, nv) || i == j) return 0u;
+  const int k = 3 - i - j;
+  // eu x ev = su sv (e_i x e_j) = su sv eps_ijk e_k
+  const bool cyclic = (j == (i + 1) % 3);
+  const bool nn = (nu != nv) != !cyclic;
+  return QI_AXIS | (uint32_t)k | ((uint32_t)i << 2) | ((uint32_t)j << 4) | (nu ? QI_NEG_U : 0u) |
+         (nv ? QI_NEG_V : 0u) | (nn ? QI_NEG_N : 0u);
+}
+
+void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
+  validate(v);
+  HIP_TRY(hipSetDevice(b->device));
+  sc.grid.upload(v.grid, v.grid_len, b->stream);
+  sc.models.upload(v.models, v.models_len, b->stream);
+  sc.quads.upload(v.quads, v.quads_len, b->stream);
+  std::vector<uint32_t> info(v.quads_len);
+  for (size_t q = 0; q < v.quads_len; q++) info[q] = quad_info(v.quads[q]);
+  sc.qinfo.upload(info.data(), info.size(), b->stream);
+  sc.atlas.upload(v.atlas, v.atlas_len, b->stream);
+  sc.emitters.upload(v.emitters, v.emitters_len, b->stream);
+  sc.egrid.upload(v.emitter_grid, v.emitter_grid_len, b->stream);
+  sc.sky.upload(v.sky, v.sky ? v.sky_len : 0, b->stream);
+  float lut[256];
+  gamma_lut(lut);
+  sc.lut.upload(lut, 256, b->stream);
+  HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors above die with this scope
+  sc.key = layout_key(*v.params);
+}
+
+constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
+constexpr uint32_t kMaxBatch = 64;         // render/cuda.rs:165
+
+void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
+                  uint32_t spp_in, lw_progress_fn progress, void *user, float *out_rgb) {
+  auto t_wall = std::chrono::steady_clock::now();
+  if (!(layout_key(params_in) == sc.key))
+    throw InvalidScene{"params do not match the layout the scene was validated with"};
+  lw_params p = params_in;
+  p.frame[2] = 0;
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  if (w == 0 || h == 0) throw InvalidScene{"empty frame"};
+  const uint64_t npix = (uint64_t)w * h;
+  if (npix > (1ull << 31) / 3) throw InvalidScene{"frame too large"};
+  const uint32_t spp = std::max(spp_in, 1u);
+  HIP_TRY(hipSetDevice(b->device));
+  const DevScene ds = sc.dev();
+
+  // Staging for up to kMaxBatch samples per launch, capped at 2 GiB.
+  uint32_t max_batch = kMaxBatch;
+  while (max_batch > 1 && npix * 3 * sizeof(float) * max_batch > (2ull << 30)) max_batch /= 2;
+  b->rad.ensure(npix * 3 * max_batch);
+  b->acc.ensure(npix * 3);
+  b->out.ensure(npix * 3);
+  {
+    const size_t n = npix * 3;
+    k_zero<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(b->acc.ptr, n);
+    HIP_TRY(hipGetLastError());
+  }
+
+  const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+  lw_timing tm{};
+  uint32_t done = 0, batch = 1;
+  while (done < spp) {
+    const uint32_t nb = std::min(batch, std::min(spp - done, max_batch));
+    const auto start = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    k_render<<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(p, ds, done, b->rad.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    k_accumulate<<<dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        (uint32_t)npix, nb, b->rad.ptr, b->acc.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev[2]));
+    float ms_r = 0.0f, ms_a = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms_r, b->ev[0], b->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_a, b->ev[1], b->ev[2]));
+    tm.trace_ms += ms_r;
+    tm.accumulate_ms += ms_a;
+    tm.launches += 1;
+    done += nb;
+    if (progress) progress(done, spp, user);
+    const double took =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+    if (took < kTargetLaunchMs / 2)
+      batch = std::min(batch * 2, kMaxBatch);
+    else if (took > kTargetLaunchMs * 2 && batch > 1)
+      batch /= 2;
+  }
+  {
+    const size_t n = npix * 3;
+    k_resolve<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        (uint32_t)npix, spp, b->acc.ptr, b->out.ptr);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(out_rgb, b->out.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                         b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  tm.samples = npix * spp;
+  tm.wall_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+  b->timing = tm;
+}
+
+template <typename F>
+int guarded(lw_hip_backend *b, F &&f) {
+  try {
+    f();
+    if (b) b->last_error.clear();
+    return LW_OK;
+  } catch (const InvalidScene &e) {
+    set_err(b, "invalid scene: " + e.what);
+    return LW_EINVAL;
+  } catch (const HipError &e) {
+    set_err(b, "HIP: " + e.what);
+    return e.code == hipErrorOutOfMemory ? LW_ENOMEM : LW_EHIP;
+  } catch (const std::exception &e) {
+    set_err(b, e.what());
+    return LW_EHIP;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lw_last_error(void) { return g_last_error.c_str(); }
+
+int lw_hip_probe(char *desc, size_t cap) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  std::string s;
+  int rc = LW_OK;
+  if (e != hipSuccess || n == 0) {
+    s = std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "device count 0");
+    rc = LW_ENODEV;
+    g_last_error = s;
+  } else {
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, 0);
+    if (e != hipSuccess) {
+      s = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+      rc = LW_EHIP;
+      g_last_error = s;
+    } else {
+      s = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+          std::to_string(prop.multiProcessorCount) + " CUs)";
+    }
+  }
+  if (desc && cap) {
+    std::strncpy(desc, s.c_str(), cap - 1);
+    desc[cap - 1] = '\0';
+  }
+  return rc;
+}
+
+int lw_hip_open(int device, lw_hip_backend **out) {
+  if (!out) return LW_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    g_last_error = std::string("no HIP device: ") +
+                   (e != hipSuccess ? hipGetErrorString(e) : "device count 0");
+    return LW_ENODEV;
+  }
+  if (device < 0 || device >= n) {
+    g_last_error = "device index out of range";
+    return LW_EINVAL;
+  }
+  auto *b = new lw_hip_backend();
+  b->device = device;
+  int rc = guarded(b, [&] {
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::string nm = std::string(prop.name) + " (hip)";
+    for (auto &c : nm) c = (char)std::tolower((unsigned char)c);
+    b->name = nm;
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto &ev : b->ev) HIP_TRY(hipEventCreate(&ev));
+  });
+  if (rc != LW_OK) {
+    lw_hip_close(b);
+    return rc;
+  }
+  *out = b;
+  return LW_OK;
+}
+
+const char *lw_hip_name(const lw_hip_backend *b) { return b ? b->name.c_str() : ""; }
+
+const char *lw_hip_last_error(const lw_hip_backend *b) { return b ? b->last_error.c_str() : ""; }
+
+int lw_hip_render_quick(lw_hip_backend *b) {
+  set_err(b, "quick mode has no HIP path; use the CPU rasterizer (render/cuda.rs:70-75)");
+  return LW_EUNSUPPORTED;
+}
+
+void lw_hip_close(lw_hip_backend *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  b->rad.free();
+  b->acc.free();
+  b->out.free();
+  for (auto &ev : b->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int lw_hip_scene_upload(lw_hip_backend *b, const lw_scene_view *view, lw_hip_scene **out) {
+  if (!b || !view || !out) return LW_EINVAL;
+  *out = nullptr;
+  auto *sc = new lw_hip_scene();
+  int rc = guarded(b, [&] { upload_scene(b, *view, *sc); });
+  if (rc != LW_OK) {
+    delete sc;
+    return rc;
+  }
+  *out = sc;
+  return LW_OK;
+}
+
+void lw_hip_scene_free(lw_hip_backend *b, lw_hip_scene *scene) {
+  if (b) (void)hipSetDevice(b->device);
+  delete scene;
+}
+
+int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                           uint32_t spp, lw_progress_fn progress, void *user, float *out_rgb) {
+  if (!b || !scene || !params || !out_rgb) return LW_EINVAL;
+  return guarded(b, [&] { render_frame(b, *scene, *params, spp, progress, user, out_rgb); });
+}
+
+int lw_hip_render_hq(lw_hip_backend *b, const lw_scene_view *view, uint32_t spp,
+                     lw_progress_fn progress, void *user, float *out_rgb) {
+  if (!b || !view || !out_rgb) return LW_EINVAL;
+  return guarded(b, [&] {
+    lw_hip_scene sc;
+    upload_scene(b, *view, sc);
+    render_frame(b, sc, *view->params, spp, progress, user, out_rgb);
+  });
+}
+
+int lw_hip_last_timing(const lw_hip_backend *b, lw_timing *out) {
+  if (!b || !out) return LW_EINVAL;
+  *out = b->timing;
+  return LW_OK;
+}
+
+int lw_hip_trace(lw_hip_backend *b, const lw_scene_view *view, size_t n, const float *o,
+                 const float *d, const float *tmin, const float *tmax, lw_hit *out) {
+  if (!b || !view || (n && (!o || !d || !tmin || !tmax || !out))) return LW_EINVAL;
+  return guarded(b, [&] {
+    if (n > 0xFFFFFFFFull) throw InvalidScene{"too many rays"};
+    if (n == 0) return;
+    lw_hip_scene sc;
+    upload_scene(b, *view, sc);
+    DevBuf<float> d_o, d_d, d_tmin, d_tmax;
+    DevBuf<lw_hit> d_out;
+    d_o.upload(o, 3 * n, b->stream);
+    d_d.upload(d, 3 * n, b->stream);
+    d_tmin.upload(tmin, n, b->stream);
+    d_tmax.upload(tmax, n, b->stream);
+    d_out.alloc(n);
+    k_trace<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        *view->params, sc.dev(), (uint32_t)n, d_o.ptr, d_d.ptr, d_tmin.ptr, d_tmax.ptr, d_out.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d_out.ptr, n * sizeof(lw_hit), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  });
+}
+
+int lw_hip_render_samples(lw_hip_backend *b, const lw_scene_view *view, size_t n,
+                          const uint32_t *px, const uint32_t *py, const uint32_t *sample,
+                          float *out_rgb) {
+  if (!b || !view || (n && (!px || !py || !sample || !out_rgb))) return LW_EINVAL;
+  return guarded(b, [&] {
+    if (n > 0xFFFFFFFFull) throw InvalidScene{"too many samples"};
+    if (n == 0) return;
+    lw_hip_scene sc;
+    upload_scene(b, *view, sc);
+    DevBuf<uint32_t> d_px, d_py, d_s;
+    DevBuf<float> d_out;
+    d_px.upload(px, n, b->stream);
+    d_py.upload(py, n, b->stream);
+    d_s.upload(sample, n, b->stream);
+    d_out.alloc(3 * n);
+    k_render_samples<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        *view->params, sc.dev(), (uint32_t)n, d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out_rgb, d_out.ptr, 3 * n * sizeof(float), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  });
+}
+
+}  // extern "C"

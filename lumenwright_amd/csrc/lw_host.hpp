@@ -1,0 +1,9 @@
+// Host-side frame preparation used inside the library (see include/lw/host.h).
+#pragma once
+
+#include "lw/host.h"
+
+namespace lw {
+// gamma_to_linear(k / 255) for k = 0..255 (math.rs:213-216 with scene.rs:299-306).
+void gamma_lut(float out[256]);
+}  // namespace lw

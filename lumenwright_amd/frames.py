@@ -1,0 +1,331 @@
+"""Host-side frame preparation (include/lw/host.h) and the synthetic benchmark frames.
+
+The reference builds a frame from a Minecraft save and the game's client jar
+(crates/lumenwright/src/request.rs:751-950). Without them, the frames here are built from
+seeded synthetic worlds with the same buffer layout (scene/voxel.rs) and the reference's
+own sun, sky, camera and Params logic, restated in C++ (lumenwright_amd/csrc/lw_host.cpp).
+
+Configurations (the workloads the benchmark and the parity tests run):
+  0  8 x 8 chunks x 384, 32 cube types, 320 x 180, 16 spp, 4 bounces, Chunky's Preetham day
+     sky at tick 4000 with the sun (sky cache), no emitters.
+  1  24 x 24 chunks x 384, 56 cube + 8 cutout types, water below y = 62, ~1% emitters
+     (intensity 137.9628, sampling ONE), 1280 x 720, 256 spp, 8 bounces, the game look's dusk
+     at tick 12800 (two sky maps, no sun light).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, replace
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from .abi import (EMITTER, EMITTER_SAMPLING_NONE, EMITTER_SAMPLING_ONE, MODEL, PARAMS, QUAD,
+                  HqFrame, HqScene)
+
+f3 = ctypes.c_float * 3
+
+
+class HqSkyC(ctypes.Structure):
+    _fields_ = [("sun_dir", ctypes.c_float * 3), ("sun_color", ctypes.c_float * 3),
+                ("sky_light", ctypes.c_float)]
+
+
+class PreethamC(ctypes.Structure):
+    _fields_ = [("sun", ctypes.c_float * 3), ("zenith", ctypes.c_float * 3),
+                ("perez", (ctypes.c_float * 3) * 5)]
+
+
+class HqOptionsC(ctypes.Structure):
+    _fields_ = [("spp", ctypes.c_uint32), ("max_bounces", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("emitter_sampling", ctypes.c_uint32),
+                ("prevent_normal_emitter", ctypes.c_uint32)]
+
+
+class SceneInfoC(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_uint32 * 3), ("offsets", ctypes.c_uint32 * 5),
+                ("atlas_size", ctypes.c_uint32 * 2), ("quads_len", ctypes.c_uint32),
+                ("models_len", ctypes.c_uint32), ("emitter_count", ctypes.c_uint32),
+                ("see_through", ctypes.c_uint32 * 2), ("has_sun_sprite", ctypes.c_uint32),
+                ("sun_sprite", ctypes.c_uint32 * 4)]
+
+
+class CameraRaysC(ctypes.Structure):
+    _fields_ = [("forward", ctypes.c_float * 3), ("right", ctypes.c_float * 3),
+                ("up", ctypes.c_float * 3), ("tan_half_fov", ctypes.c_float),
+                ("aspect", ctypes.c_float)]
+
+
+class WorldSpecC(ctypes.Structure):
+    _fields_ = [("chunks_x", ctypes.c_uint32), ("chunks_z", ctypes.c_uint32),
+                ("sections", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("texture_seed", ctypes.c_uint64), ("cube_types", ctypes.c_uint32),
+                ("cutout_types", ctypes.c_uint32), ("water_level", ctypes.c_int32),
+                ("emitter_fraction", ctypes.c_float), ("cutout_fraction", ctypes.c_float),
+                ("pad", ctypes.c_uint32)]
+
+
+class WorldC(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_void_p), ("grid_len", ctypes.c_size_t),
+                ("models", ctypes.c_void_p), ("models_len", ctypes.c_size_t),
+                ("quads", ctypes.c_void_p), ("quads_len", ctypes.c_size_t),
+                ("atlas", ctypes.c_void_p), ("atlas_len", ctypes.c_size_t),
+                ("emitters", ctypes.c_void_p), ("emitters_len", ctypes.c_size_t),
+                ("emitter_grid", ctypes.c_void_p), ("emitter_grid_len", ctypes.c_size_t),
+                ("info", SceneInfoC),
+                ("bricks", ctypes.c_uint64), ("uniform_bricks", ctypes.c_uint64),
+                ("buried_bricks", ctypes.c_uint64), ("solid_cells", ctypes.c_uint64),
+                ("water_cells", ctypes.c_uint64), ("cutout_cells", ctypes.c_uint64)]
+
+
+def _v3(a) -> "ctypes.Array":
+    return f3(*[float(x) for x in a])
+
+
+# ---- sky.rs / camera.rs / request.rs restated ----
+
+def celestial_angle(ticks: int) -> float:
+    return _native.lib().lw_celestial_angle(ticks)
+
+
+def sun_direction_from_angle(angle: float) -> np.ndarray:
+    out = f3()
+    _native.lib().lw_sun_direction_from_angle(angle, out)
+    return np.array(out, dtype="<f4")
+
+
+def sun_direction(altitude_deg: float, azimuth_deg: float) -> np.ndarray:
+    out = f3()
+    _native.lib().lw_sun_direction(altitude_deg, azimuth_deg, out)
+    return np.array(out, dtype="<f4")
+
+
+def twilight(altitude: float) -> float:
+    return _native.lib().lw_twilight(altitude)
+
+
+def hq_sky(sun_dir, sun_intensity: float = 1.25, sky_intensity: float = 1.0) -> HqSkyC:
+    out = HqSkyC()
+    _native.lib().lw_hq_sky_new(_v3(sun_dir), sun_intensity, sky_intensity, ctypes.byref(out))
+    return out
+
+
+def preetham_new(sun_dir, turbidity: float = 2.5) -> PreethamC:
+    out = PreethamC()
+    _native.lib().lw_preetham_new(_v3(sun_dir), turbidity, ctypes.byref(out))
+    return out
+
+
+def sun_texture_axes(sw):
+    su, sv = f3(), f3()
+    _native.lib().lw_sun_texture_axes(_v3(sw), su, sv)
+    return np.array(su, dtype="<f4"), np.array(sv, dtype="<f4")
+
+
+def sky_cache(params: np.ndarray) -> np.ndarray:
+    n = int(params["misc_f"][2])
+    if n == 0:
+        return np.zeros(0, dtype="<f4")
+    out = np.zeros((n + 1) * (n + 1) * 3, dtype="<f4")
+    p = np.ascontiguousarray(params).reshape(())
+    r = _native.lib().lw_sky_cache(ctypes.c_void_p(p.ctypes.data), ctypes.c_void_p(out.ctypes.data),
+                                   out.size)
+    assert r == out.size
+    return out
+
+
+def gamma_lut() -> np.ndarray:
+    out = np.zeros(256, dtype="<f4")
+    _native.lib().lw_gamma_lut(ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def camera_basis(yaw: float, pitch: float):
+    f, r, u = f3(), f3(), f3()
+    _native.lib().lw_camera_basis(yaw, pitch, f, r, u)
+    return (np.array(f, dtype="<f4"), np.array(r, dtype="<f4"), np.array(u, dtype="<f4"))
+
+
+def scene_info(scene: HqScene) -> SceneInfoC:
+    info = SceneInfoC()
+    info.dims[:] = list(scene.dims)
+    info.offsets[:] = list(scene.offsets)
+    info.atlas_size[:] = list(scene.atlas_size)
+    info.quads_len = len(scene.quads)
+    info.models_len = len(scene.models)
+    info.emitter_count = scene.emitter_count
+    info.see_through[:] = list(scene.see_through)
+    if scene.sun_sprite is not None:
+        info.has_sun_sprite = 1
+        info.sun_sprite[:] = list(scene.sun_sprite)
+    return info
+
+
+def hq_params(scene: HqScene, *, width: int, height: int, eye, yaw: float, pitch: float,
+              fov_y: float, sky: HqSkyC, max_bounces: int, seed: int = 0,
+              emitter_intensity: float = 0.0, emitter_sampling: int = EMITTER_SAMPLING_ONE,
+              prevent_normal_emitter: bool = False, turbidity: float = 2.5) -> np.ndarray:
+    """hq_params (request.rs:1153-1225) for a camera at `eye` (grid-relative) with Minecraft
+    yaw/pitch and a vertical fov in degrees (request.rs:643-652)."""
+    rays = CameraRaysC()
+    _native.lib().lw_camera_rays_new(yaw, pitch, fov_y, width, height, ctypes.byref(rays))
+    opts = HqOptionsC(0, max_bounces, seed, emitter_sampling, int(prevent_normal_emitter))
+    out = np.zeros((), dtype=PARAMS)
+    _native.lib().lw_hq_params(ctypes.byref(opts), emitter_intensity, width, height,
+                               ctypes.byref(scene_info(scene)), _v3(eye), ctypes.byref(rays),
+                               ctypes.byref(sky), turbidity, ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def dusk_sky_maps(sun_dir, lens=(2048, 1024), light=(360, 180)) -> np.ndarray:
+    n = (lens[0] * lens[1] + light[0] * light[1]) * 3
+    out = np.zeros(n, dtype="<f4")
+    r = _native.lib().lw_dusk_sky_maps(_v3(sun_dir), lens[0], lens[1], light[0], light[1],
+                                       ctypes.c_void_p(out.ctypes.data), n)
+    assert r == n
+    return out
+
+
+# ---- synthetic worlds ----
+
+@dataclass(frozen=True)
+class WorldSpec:
+    chunks_x: int = 8
+    chunks_z: int = 8
+    sections: int = 24
+    seed: int = 1
+    texture_seed: int = 2
+    cube_types: int = 32
+    cutout_types: int = 0
+    water_level: int = -1
+    emitter_fraction: float = 0.0
+    cutout_fraction: float = 0.0
+
+    def c(self) -> WorldSpecC:
+        return WorldSpecC(self.chunks_x, self.chunks_z, self.sections, self.seed,
+                          self.texture_seed, self.cube_types, self.cutout_types, self.water_level,
+                          self.emitter_fraction, self.cutout_fraction, 0)
+
+
+def world_height(spec: WorldSpec, x: int, z: int) -> int:
+    return _native.lib().lw_world_height(ctypes.byref(spec.c()), x, z)
+
+
+def _copy(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    nbytes = n * np.dtype(dtype).itemsize
+    buf = (ctypes.c_uint8 * nbytes).from_address(ptr)
+    return np.frombuffer(bytes(buf), dtype=dtype).copy()
+
+
+def build_world(spec: WorldSpec) -> HqScene:
+    L = _native.lib()
+    h = ctypes.c_void_p()
+    rc = L.lw_world_build(ctypes.byref(spec.c()), ctypes.byref(h))
+    if rc != 0:
+        raise RuntimeError("world build failed")
+    try:
+        w = WorldC.from_address(h.value)
+        info = w.info
+        scene = HqScene(
+            grid=_copy(w.grid, w.grid_len, "<u4"),
+            models=_copy(w.models, w.models_len, MODEL),
+            quads=_copy(w.quads, w.quads_len, QUAD),
+            atlas=_copy(w.atlas, w.atlas_len, "<u4"),
+            atlas_size=(info.atlas_size[0], info.atlas_size[1]),
+            dims=tuple(info.dims),
+            offsets=tuple(info.offsets),
+            see_through=tuple(info.see_through),
+            emitters=_copy(w.emitters, w.emitters_len, EMITTER),
+            emitter_count=info.emitter_count,
+            emitter_grid=_copy(w.emitter_grid, w.emitter_grid_len, "<u4"),
+        )
+        scene.stats = {k: int(getattr(w, k)) for k in ("bricks", "uniform_bricks", "buried_bricks",
+                                                        "solid_cells", "water_cells",
+                                                        "cutout_cells")}
+        return scene
+    finally:
+        L.lw_world_free(h)
+
+
+GAME_EMITTER_INTENSITY = 137.9628  # request.rs:138
+DEFAULT_EMITTER_INTENSITY = 13.0   # request.rs:126
+
+
+@dataclass(frozen=True)
+class FrameConfig:
+    """One benchmark workload: a world, a camera and the HQ options."""
+    name: str
+    world: WorldSpec
+    width: int
+    height: int
+    spp: int
+    max_bounces: int
+    sky: str                 # "chunky" (Preetham cache, sun light) or "game-dusk" (two maps)
+    ticks: int
+    emitter_intensity: float
+    fov_y: float = 70.0
+    pitch: float = 30.0
+    yaw: float = 0.0
+    camera_height: float = 20.0
+    seed: int = 0
+    turbidity: float = 2.5
+
+
+CONFIGS = {
+    0: FrameConfig("cfg0: 8x8 chunks, 320x180, 16 spp, 4 bounces, Preetham day sky",
+                   WorldSpec(8, 8, 24, cube_types=32), 320, 180, 16, 4, "chunky", 4000, 0.0),
+    1: FrameConfig("cfg1: 24x24 chunks with water and emitters, 1280x720, 256 spp, 8 bounces, "
+                   "game dusk sky",
+                   WorldSpec(24, 24, 24, cube_types=56, cutout_types=8, water_level=62,
+                             emitter_fraction=0.01, cutout_fraction=0.05),
+                   1280, 720, 256, 8, "game-dusk", 12800, GAME_EMITTER_INTENSITY),
+}
+
+
+def config(n: int, **overrides) -> FrameConfig:
+    """CONFIGS[n] with fields replaced; `world_*` keys replace fields of the world spec."""
+    c = CONFIGS[n]
+    wo = {k[6:]: v for k, v in overrides.items() if k.startswith("world_")}
+    fo = {k: v for k, v in overrides.items() if not k.startswith("world_")}
+    if wo:
+        fo["world"] = replace(c.world, **wo)
+    return replace(c, **fo)
+
+
+_world_cache = {}
+
+
+def make_frame(cfg: FrameConfig, scene: Optional[HqScene] = None) -> HqFrame:
+    """The HqFrame of a configuration: its world, camera, Params and sky buffer."""
+    if scene is None:
+        scene = _world_cache.get(cfg.world)
+        if scene is None:
+            scene = build_world(cfg.world)
+            _world_cache.clear()
+            _world_cache[cfg.world] = scene
+    gx, gz = cfg.world.chunks_x * 16, cfg.world.chunks_z * 16
+    cx, cz = gx // 2, gz // 2
+    eye = [cx + 0.5, world_height(cfg.world, cx, cz) + cfg.camera_height, cz + 0.5]
+    sun = sun_direction_from_angle(celestial_angle(cfg.ticks))
+    sky = hq_sky(sun, 1.25, 1.0)
+    params = hq_params(scene, width=cfg.width, height=cfg.height, eye=eye, yaw=cfg.yaw,
+                       pitch=cfg.pitch, fov_y=cfg.fov_y, sky=sky, max_bounces=cfg.max_bounces,
+                       seed=cfg.seed, emitter_intensity=cfg.emitter_intensity,
+                       turbidity=cfg.turbidity)
+    if cfg.sky == "chunky":
+        sky_buf = sky_cache(params)
+    elif cfg.sky == "game-dusk":
+        # The game look (request.rs:848-924 at day weight 0): the sun casts no light and is
+        # not drawn by Chunky's sprite, the maps replace the cache.
+        params["sun_color"][:3] = 0.0
+        params["sun_sprite"][:] = 0.0
+        params["sky_map"] = [2048.0, 1024.0, 360.0, 180.0]
+        params["misc_f"][2] = 0.0
+        sky_buf = dusk_sky_maps(sun)
+    else:
+        raise ValueError(cfg.sky)
+    return HqFrame(scene=scene, params=params, spp=cfg.spp, sky=sky_buf)

@@ -1,0 +1,303 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes binding of the C++ restatement of the reference kernel crate
+(oracle/lw_oracle_impl.hpp). Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker or the CPU baseline.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "_build" / "liblw_oracle.so"
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compiles the oracle with its Makefile (g++, no FMA contraction)."""
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return LIB_PATH
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("bytes", ctypes.c_uint64), ("dda_steps", ctypes.c_uint64),
+                ("quads_tested", ctypes.c_uint64), ("traces", ctypes.c_uint64),
+                ("samples", ctypes.c_uint64), ("sky_lookups", ctypes.c_uint64),
+                ("pad", ctypes.c_uint64 * 2)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, sz, u32, f32, i32 = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                                 ctypes.c_float, ctypes.c_int)
+        L.lwo_last_error.restype = ctypes.c_char_p
+        L.lwo_sizeof.argtypes = [i32]
+        L.lwo_sizeof.restype = u32
+        L.lwo_pcg_hash.argtypes = [u32]
+        L.lwo_pcg_hash.restype = u32
+        L.lwo_rng_new.argtypes = [u32, u32, u32]
+        L.lwo_rng_new.restype = u32
+        L.lwo_rng_draws.argtypes = [u32, u32, vp]
+        L.lwo_rng_draws.restype = u32
+        for name in ("lwo_acos", "lwo_ln", "lwo_gamma_to_linear", "lwo_expf"):
+            getattr(L, name).argtypes = [vp, sz, vp]
+            getattr(L, name).restype = None
+        L.lwo_sin_cos_turns.argtypes = [vp, sz, vp, vp]
+        L.lwo_sin_cos_turns.restype = None
+        L.lwo_atan2.argtypes = [vp, vp, sz, vp]
+        L.lwo_atan2.restype = None
+        L.lwo_onb.argtypes = [vp, vp, vp]
+        L.lwo_onb.restype = None
+        L.lwo_trace.argtypes = [vp, sz, vp, vp, vp, vp, vp]
+        L.lwo_trace.restype = i32
+        L.lwo_is_see_through.argtypes = [vp, u32]
+        L.lwo_is_see_through.restype = i32
+        L.lwo_see_through_hit.argtypes = [vp, vp, vp, f32, vp]
+        L.lwo_see_through_hit.restype = i32
+        L.lwo_render_samples.argtypes = [vp, sz, vp, vp, vp, vp, ctypes.c_uint]
+        L.lwo_render_samples.restype = i32
+        L.lwo_render_hq.argtypes = [vp, u32, ctypes.c_uint, vp]
+        L.lwo_render_hq.restype = i32
+        L.lwo_render_hq_counted.argtypes = [vp, u32, ctypes.c_uint, vp, vp]
+        L.lwo_render_hq_counted.restype = i32
+        L.lwo_preetham_radiance.argtypes = [vp, vp, sz, vp]
+        L.lwo_preetham_radiance.restype = i32
+        L.lwo_rgb_to_hsl.argtypes = [vp, sz, vp]
+        L.lwo_rgb_to_hsl.restype = None
+        L.lwo_rgb_from_hsl.argtypes = [vp, sz, vp]
+        L.lwo_rgb_from_hsl.restype = None
+        L.lwo_build_sky_cache.argtypes = [vp, u32, vp, sz]
+        L.lwo_build_sky_cache.restype = i32
+        L.lwo_sky.argtypes = [vp, i32, vp, sz, vp]
+        L.lwo_sky.restype = i32
+        L.lwo_emitter_cell.argtypes = [vp, vp, vp]
+        L.lwo_emitter_cell.restype = i32
+        L.lwo_sample_emitters.argtypes = [vp, vp, vp, vp, u32, sz, vp]
+        L.lwo_sample_emitters.restype = i32
+        L.lwo_water_interface.argtypes = [vp, vp, vp, vp, i32, i32, f32, u32, vp, vp, vp]
+        L.lwo_water_interface.restype = u32
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().lwo_last_error().decode())
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype="<f4")
+
+
+def _addr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _params_addr(view):
+    return ctypes.c_void_p(view.params.ctypes.data)
+
+
+# ---- math (crates/lumenwright-kernel/src/math.rs) ----
+
+def pcg_hash(v: int) -> int:
+    return lib().lwo_pcg_hash(v & 0xFFFFFFFF)
+
+
+def rng_new(pixel: int, sample: int, seed: int) -> int:
+    return lib().lwo_rng_new(pixel, sample, seed)
+
+
+def rng_draws(state: int, n: int):
+    out = np.zeros(n, dtype="<f4")
+    st = lib().lwo_rng_draws(state, n, _addr(out))
+    return out, st
+
+
+def _unary(name, x):
+    x = _f32(x)
+    out = np.empty_like(x)
+    getattr(lib(), name)(_addr(x), x.size, _addr(out))
+    return out
+
+
+def acos(x):
+    return _unary("lwo_acos", x)
+
+
+def ln(x):
+    return _unary("lwo_ln", x)
+
+
+def gamma_to_linear(x):
+    return _unary("lwo_gamma_to_linear", x)
+
+
+def expf(x):
+    return _unary("lwo_expf", x)
+
+
+def sin_cos_turns(u):
+    u = _f32(u)
+    s, c = np.empty_like(u), np.empty_like(u)
+    lib().lwo_sin_cos_turns(_addr(u), u.size, _addr(s), _addr(c))
+    return s, c
+
+
+def atan2(y, x):
+    y, x = _f32(y), _f32(x)
+    out = np.empty_like(y)
+    lib().lwo_atan2(_addr(y), _addr(x), y.size, _addr(out))
+    return out
+
+
+def onb(n):
+    n = _f32(n)
+    t, b = np.empty(3, "<f4"), np.empty(3, "<f4")
+    lib().lwo_onb(_addr(n), _addr(t), _addr(b))
+    return t, b
+
+
+# ---- trace.rs ----
+
+def trace(view, o, d, tmin=None, tmax=None):
+    """trace (trace.rs:137-290) for rays o, d of shape (n, 3); returns HIT records."""
+    from lumenwright_amd.abi import HIT
+    o = _f32(o).reshape(-1, 3)
+    d = _f32(d).reshape(-1, 3)
+    n = o.shape[0]
+    tmin = _f32(np.zeros(n) if tmin is None else np.broadcast_to(tmin, (n,)))
+    tmax = _f32(np.full(n, np.inf) if tmax is None else np.broadcast_to(tmax, (n,)))
+    out = np.zeros(n, dtype=HIT)
+    _check(lib().lwo_trace(view.ref, n, _addr(o), _addr(d), _addr(tmin), _addr(tmax), _addr(out)))
+    return out
+
+
+def is_see_through(view, qi):
+    return bool(lib().lwo_is_see_through(view.ref, qi))
+
+
+def see_through_hit(view, o, d, tmin):
+    from lumenwright_amd.abi import HIT
+    out = np.zeros((), dtype=HIT)
+    _check(lib().lwo_see_through_hit(view.ref, _addr(_f32(o)), _addr(_f32(d)), float(tmin),
+                                     _addr(out)))
+    return out if out["hit"] else None
+
+
+# ---- integrator.rs ----
+
+def render_samples(view, px, py, sample, threads: int = 0):
+    """render_sample (integrator.rs:539-709) for each (px, py, sample); (n, 3) f32."""
+    px = np.ascontiguousarray(px, dtype="<u4")
+    py = np.ascontiguousarray(py, dtype="<u4")
+    sample = np.ascontiguousarray(sample, dtype="<u4")
+    n = px.size
+    out = np.zeros((n, 3), dtype="<f4")
+    threads = threads or os.cpu_count() or 1
+    _check(lib().lwo_render_samples(view.ref, n, _addr(px), _addr(py), _addr(sample), _addr(out),
+                                    threads))
+    return out
+
+
+def render_hq(view, spp: int, threads: int = 0):
+    """CpuBackend::render_hq (render/cpu/mod.rs:39-87); (h, w, 3) f32 mean radiance."""
+    w, h = int(view.params["frame"][0]), int(view.params["frame"][1])
+    out = np.zeros((h, w, 3), dtype="<f4")
+    threads = threads or os.cpu_count() or 1
+    _check(lib().lwo_render_hq(view.ref, spp, threads, _addr(out)))
+    return out
+
+
+def render_hq_counted(view, spp: int, threads: int = 0):
+    """render_hq with the algorithmic-byte counters; returns (image, Counters)."""
+    w, h = int(view.params["frame"][0]), int(view.params["frame"][1])
+    out = np.zeros((h, w, 3), dtype="<f4")
+    c = Counters()
+    threads = threads or os.cpu_count() or 1
+    _check(lib().lwo_render_hq_counted(view.ref, spp, threads, _addr(out), ctypes.byref(c)))
+    return out, c
+
+
+def preetham_radiance(view, d):
+    d = _f32(d).reshape(-1, 3)
+    out = np.zeros_like(d)
+    _check(lib().lwo_preetham_radiance(view.ref, _addr(d), d.shape[0], _addr(out)))
+    return out
+
+
+def rgb_to_hsl(rgb):
+    rgb = _f32(rgb).reshape(-1, 3)
+    out = np.zeros_like(rgb)
+    lib().lwo_rgb_to_hsl(_addr(rgb), rgb.shape[0], _addr(out))
+    return out
+
+
+def rgb_from_hsl(hsl):
+    hsl = _f32(hsl).reshape(-1, 3)
+    out = np.zeros_like(hsl)
+    lib().lwo_rgb_from_hsl(_addr(hsl), hsl.shape[0], _addr(out))
+    return out
+
+
+def build_sky_cache(view, n: int):
+    out = np.zeros((n + 1) * (n + 1) * 3, dtype="<f4")
+    _check(lib().lwo_build_sky_cache(view.ref, n, _addr(out), out.size))
+    return out
+
+
+def _sky(view, which, d):
+    d = _f32(d).reshape(-1, 3)
+    out = np.zeros_like(d)
+    _check(lib().lwo_sky(view.ref, which, _addr(d), d.shape[0], _addr(out)))
+    return out
+
+
+def sky_radiance(view, d):
+    return _sky(view, 0, d)
+
+
+def sky_lens(view, d):
+    return _sky(view, 1, d)
+
+
+def sky_light(view, d):
+    return _sky(view, 2, d)
+
+
+def sun_texture(view, d):
+    return _sky(view, 3, d)
+
+
+def emitter_cell(view, p):
+    out = np.zeros(2, dtype="<u4")
+    _check(lib().lwo_emitter_cell(view.ref, _addr(_f32(p)), _addr(out)))
+    return int(out[0]), int(out[1])
+
+
+def sample_emitters(view, pos, origin, n, rng_state: int, draws: int):
+    out = np.zeros((draws, 3), dtype="<f4")
+    _check(lib().lwo_sample_emitters(view.ref, _addr(_f32(pos)), _addr(_f32(origin)),
+                                     _addr(_f32(n)), rng_state, draws, _addr(out)))
+    return out
+
+
+def water_interface(params, pos, d, n, front, in_water, eps, rng_state):
+    d_out, o_out = np.zeros(3, "<f4"), np.zeros(3, "<f4")
+    w = ctypes.c_int(0)
+    st = lib().lwo_water_interface(ctypes.c_void_p(params.ctypes.data), _addr(_f32(pos)),
+                                   _addr(_f32(d)), _addr(_f32(n)), int(front), int(in_water),
+                                   float(eps), rng_state, _addr(d_out), _addr(o_out),
+                                   ctypes.byref(w))
+    return d_out, o_out, bool(w.value), st

@@ -1,0 +1,253 @@
+// ORACLE -- test infrastructure only (see lw_oracle_impl.hpp). The extern "C" surface
+// that tests/ (through ctypes), __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// call. Each function names the reference function it restates.
+
+#define LWO_NS lwo
+#define LWO_COUNTING 0
+#include "lw_oracle_impl.hpp"
+
+#include <cstdio>
+
+using namespace lwo;
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char *lwo_last_error(void) { return g_err.c_str(); }
+
+uint32_t lwo_sizeof(int which) {
+  switch (which) {
+    case 0: return sizeof(Params);
+    case 1: return sizeof(Quad);
+    case 2: return sizeof(Model);
+    case 3: return sizeof(Emitter);
+    case 4: return sizeof(HitOut);
+    default: return 0;
+  }
+}
+
+// rng.rs:5-9
+uint32_t lwo_pcg_hash(uint32_t v) { return pcg_hash(v); }
+// rng.rs:23-27
+uint32_t lwo_rng_new(uint32_t pixel, uint32_t sample, uint32_t seed) {
+  return Rng::make(pixel, sample, seed).state;
+}
+// rng.rs:31-34: n draws from state; returns the new state.
+uint32_t lwo_rng_draws(uint32_t state, uint32_t n, float *out) {
+  Rng r{state};
+  for (uint32_t i = 0; i < n; i++) out[i] = r.next();
+  return r.state;
+}
+
+// math.rs elementwise over arrays (n values).
+void lwo_sin_cos_turns(const float *u, size_t n, float *s, float *c) {
+  for (size_t i = 0; i < n; i++) sin_cos_turns(u[i], s[i], c[i]);
+}
+void lwo_acos(const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = f_acos(x[i]);
+}
+void lwo_atan2(const float *y, const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = f_atan2(y[i], x[i]);
+}
+void lwo_ln(const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = f_ln(x[i]);
+}
+void lwo_gamma_to_linear(const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = gamma_to_linear(x[i]);
+}
+void lwo_expf(const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = f_exp(x[i]);
+}
+// math.rs:347-356
+void lwo_onb(const float *n3, float *t3, float *b3) {
+  V3 t, b;
+  onb(v3(n3[0], n3[1], n3[2]), t, b);
+  t3[0] = t.x; t3[1] = t.y; t3[2] = t.z;
+  b3[0] = b.x; b3[1] = b.y; b3[2] = b.z;
+}
+
+// trace.rs:137-290 for n rays.
+int lwo_trace(const View *view, size_t n, const float *o, const float *d, const float *tmin,
+              const float *tmax, HitOut *out) {
+  return guarded([&] {
+    for (size_t i = 0; i < n; i++) {
+      Hit h;
+      HitOut r{};
+      bool hit = trace(*view, v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                       v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), tmin[i], tmax[i], h, r.cell);
+      if (hit) {
+        r.t = h.t;
+        r.quad = h.quad;
+        r.texel = h.texel;
+        r.s = h.s;
+        r.tt = h.tt;
+        r.hit = 1;
+      } else {
+        r.t = INFINITY;
+        r.quad = UINT32_MAX;
+      }
+      out[i] = r;
+    }
+  });
+}
+
+// trace.rs:113-116
+int lwo_is_see_through(const View *view, uint32_t qi) { return is_see_through(*view, qi) ? 1 : 0; }
+
+// trace.rs:120-131
+int lwo_see_through_hit(const View *view, const float *o, const float *d, float tmin, HitOut *out) {
+  return guarded([&] {
+    Hit h;
+    HitOut r{};
+    if (see_through_hit(*view, v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), tmin, h)) {
+      r.t = h.t;
+      r.quad = h.quad;
+      r.texel = h.texel;
+      r.s = h.s;
+      r.tt = h.tt;
+      r.hit = 1;
+    } else {
+      r.quad = UINT32_MAX;
+    }
+    *out = r;
+  });
+}
+
+// integrator.rs:539-709 for n (px, py, sample) triples, on `threads` threads.
+int lwo_render_samples(const View *view, size_t n, const uint32_t *px, const uint32_t *py,
+                       const uint32_t *sample, float *out, unsigned threads) {
+  if (threads == 0) threads = 1;
+  std::vector<std::string> errors(threads);
+  std::atomic<size_t> next{0};
+  auto worker = [&](unsigned tid) {
+    try {
+      for (;;) {
+        size_t i0 = next.fetch_add(256);
+        if (i0 >= n) break;
+        size_t i1 = i0 + 256 < n ? i0 + 256 : n;
+        for (size_t i = i0; i < i1; i++) {
+          V3 c = render_sample(*view, px[i], py[i], sample[i]);
+          out[3 * i] = c.x;
+          out[3 * i + 1] = c.y;
+          out[3 * i + 2] = c.z;
+        }
+      }
+    } catch (const std::exception &e) {
+      errors[tid] = e.what();
+      next.store(n);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; t++) pool.emplace_back(worker, t);
+  worker(0);
+  for (auto &t : pool) t.join();
+  for (auto &e : errors)
+    if (!e.empty()) {
+      g_err = e;
+      return -1;
+    }
+  return 0;
+}
+
+// CpuBackend::render_hq (render/cpu/mod.rs:39-87).
+int lwo_render_hq(const View *view, uint32_t spp, unsigned threads, float *out_rgb) {
+  if (threads == 0) threads = 1;
+  return guarded([&] { render_hq(*view, spp, threads, out_rgb, nullptr); });
+}
+
+// integrator.rs:44-74
+int lwo_preetham_radiance(const View *view, const float *d, size_t n, float *out) {
+  return guarded([&] {
+    for (size_t i = 0; i < n; i++) {
+      V3 c = preetham_radiance(*view, v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+      out[3 * i] = c.x;
+      out[3 * i + 1] = c.y;
+      out[3 * i + 2] = c.z;
+    }
+  });
+}
+// integrator.rs:79-104
+void lwo_rgb_to_hsl(const float *rgb, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) rgb_to_hsl(v3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]), out + 3 * i);
+}
+// integrator.rs:107-127
+void lwo_rgb_from_hsl(const float *hsl, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) {
+    V3 c = rgb_from_hsl(hsl[3 * i], hsl[3 * i + 1], hsl[3 * i + 2]);
+    out[3 * i] = c.x;
+    out[3 * i + 1] = c.y;
+    out[3 * i + 2] = c.z;
+  }
+}
+// integrator.rs:133-148
+int lwo_build_sky_cache(const View *view, uint32_t res, float *out, size_t out_len) {
+  return guarded([&] { build_sky_cache(*view, res, out, out_len); });
+}
+
+// integrator.rs:158-188 (which=0), 225-231 (1), 237-243 (2), 252-277 (3).
+int lwo_sky(const View *view, int which, const float *d, size_t n, float *out) {
+  return guarded([&] {
+    for (size_t i = 0; i < n; i++) {
+      V3 dd = v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+      V3 c = which == 0 ? sky_radiance(*view, dd)
+                        : which == 1 ? sky_lens(*view, dd)
+                                     : which == 2 ? sky_light(*view, dd) : sun_texture(*view, dd);
+      out[3 * i] = c.x;
+      out[3 * i + 1] = c.y;
+      out[3 * i + 2] = c.z;
+    }
+  });
+}
+
+// integrator.rs:357-373
+int lwo_emitter_cell(const View *view, const float *p, uint32_t *start_count) {
+  return guarded([&] { emitter_cell(*view, v3(p[0], p[1], p[2]), start_count[0], start_count[1]); });
+}
+
+// integrator.rs:441-469: `draws` successive calls from one RNG stream; returns the y of
+// each in out (n floats * 3).
+int lwo_sample_emitters(const View *view, const float *pos, const float *origin, const float *n3,
+                        uint32_t rng_state, size_t draws, float *out) {
+  return guarded([&] {
+    Rng rng{rng_state};
+    V3 p = v3(pos[0], pos[1], pos[2]), o = v3(origin[0], origin[1], origin[2]),
+       n = v3(n3[0], n3[1], n3[2]);
+    for (size_t i = 0; i < draws; i++) {
+      V3 c = sample_emitters(*view, p, o, n, rng);
+      out[3 * i] = c.x;
+      out[3 * i + 1] = c.y;
+      out[3 * i + 2] = c.z;
+    }
+  });
+}
+
+// integrator.rs:477-505: returns (d, o, in_water) and the RNG state after the call.
+uint32_t lwo_water_interface(const Params *p, const float *pos, const float *d, const float *n,
+                             int front, int in_water, float eps, uint32_t rng_state,
+                             float *d_out, float *o_out, int *w_out) {
+  Rng rng{rng_state};
+  V3 dd, oo;
+  bool w;
+  water_interface(*p, v3(pos[0], pos[1], pos[2]), v3(d[0], d[1], d[2]), v3(n[0], n[1], n[2]),
+                  front != 0, in_water != 0, eps, rng, dd, oo, w);
+  d_out[0] = dd.x; d_out[1] = dd.y; d_out[2] = dd.z;
+  o_out[0] = oo.x; o_out[1] = oo.y; o_out[2] = oo.z;
+  *w_out = w ? 1 : 0;
+  return rng.state;
+}
+
+}  // extern "C"

@@ -1,0 +1,27 @@
+// ORACLE -- test infrastructure only (see lw_oracle_impl.hpp). The same restatement
+// compiled with the algorithmic-byte counters on, in its own namespace, so that the
+// uncounted build timed as the CPU baseline carries no counting overhead.
+
+#define LWO_NS lwo_counted
+#define LWO_COUNTING 1
+#include "lw_oracle_impl.hpp"
+
+using namespace lwo_counted;
+
+extern "C" {
+
+// CpuBackend::render_hq (render/cpu/mod.rs:39-87) with the scene-buffer bytes, DDA
+// steps, quads tested, trace calls and samples summed into *counters.
+int lwo_render_hq_counted(const View *view, uint32_t spp, unsigned threads, float *out_rgb,
+                          Counters *counters) {
+  if (threads == 0) threads = 1;
+  try {
+    *counters = Counters{};
+    render_hq(*view, spp, threads, out_rgb, counters);
+    return 0;
+  } catch (const std::exception &) {
+    return -1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""Parity of the HIP path with the oracle, through the C ABI, on an MI355X.
+
+Bars (the kernel crate's own outputs, restated by the oracle):
+  * trace: bit-exact -- hit or miss, quad, texel, t bits and the final DDA cell;
+  * render_sample: bit-exact wherever the path does not cross water; within 1e-4 relative
+    where it does (the device's exp is correctly rounded, libm's expf is within one ulp);
+  * render_hq: the image equals the CPU backend's bit for bit where the samples do; image
+    MAE < 1e-3 in linear radiance in any case.
+"""
+
+import numpy as np
+import pytest
+
+import rays
+import scenes
+from lumenwright_amd import frames
+from lumenwright_amd.abi import EMITTER_SAMPLING_ONE
+
+pytestmark = pytest.mark.gpu
+
+
+def trace_fields_equal(a, b):
+    """The traversal result: hit, quad, texel, t bits, final cell; s and tt by value."""
+    bad = (a["hit"] != b["hit"]) | (np.any(a["cell"] != b["cell"], axis=1))
+    hit = (a["hit"] == 1) & (b["hit"] == 1)
+    bad |= hit & ((a["quad"] != b["quad"]) | (a["texel"] != b["texel"])
+                  | (a["t"].view("<u4") != b["t"].view("<u4")) | (a["s"] != b["s"])
+                  | (a["tt"] != b["tt"]))
+    return ~bad
+
+
+@pytest.fixture(scope="module")
+def frame0():
+    return frames.make_frame(frames.CONFIGS[0])
+
+
+@pytest.fixture(scope="module")
+def frame1():
+    return frames.make_frame(frames.config(1, width=256, height=144, spp=8))
+
+
+def test_probe_and_name(hip_backend):
+    import lumenwright_amd as lw
+    desc = lw.probe()
+    print("device:", desc, "| backend:", hip_backend.name())
+    assert "gfx9" in desc and hip_backend.name().endswith("(hip)")
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_trace_is_bit_exact_on_the_config_worlds(hip_backend, oracle, frame0, frame1, which):
+    fr = frame0 if which == 0 else frame1
+    v = fr.view()
+
+    def otrace(o, d, tmin, tmax):
+        return oracle.trace(v, o, d, tmin, tmax)
+
+    o, d, tmin, tmax = rays.ray_mix(fr.params, fr.scene.quads, otrace, 20000, seed=11 + which)
+    want = oracle.trace(v, o, d, tmin, tmax)
+    got = hip_backend.trace(v, o, d, tmin, tmax)
+    ok = trace_fields_equal(got, want)
+    print(f"cfg{which}: {len(o)} rays, {int(want['hit'].sum())} hits, {int((~ok).sum())} differ")
+    assert ok.all(), (np.nonzero(~ok)[0][:10], got[~ok][:3], want[~ok][:3])
+
+
+def test_trace_meets_the_reference_known_answers(hip_backend):
+    # tests.rs:128-195, 606-721 through the device's trace.
+    v = scenes.test_scene(False).view()
+    o = np.array([[0.5, 5.5, 5.5], [-10.0, 5.5, 5.5], [0.5, 9.5, 5.5], [0.5, 5.5, 5.5],
+                  [0.5, 5.5, 5.5], [0.5, 5.5, 5.5]], "<f4")
+    d = np.array([[1, 0, 0], [1, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [1, 0, 0]], "<f4")
+    tmax = np.array([np.inf] * 5 + [10.0], "<f4")
+    h = hip_backend.trace(v, o, d, 0.0, tmax)
+    assert list(h["hit"]) == [1, 1, 1, 0, 0, 0]
+    assert abs(h["t"][0] - 19.5) < 1e-3 and h["quad"][0] == 3
+    assert abs(h["t"][1] - 30.0) < 1e-3 and abs(h["t"][2] - 31.5) < 1e-3
+    wall = scenes.wall_scene().view()
+    o, d = scenes.wall_rays()
+    d = np.array([scenes.normalize32(x) for x in d], "<f4")
+    h = hip_backend.trace(wall, o, d)
+    x = o[:, 0] + d[:, 0] * h["t"]
+    assert np.all((h["hit"] == 1) & (x > 249.99) & (x < 250.01))
+    ent = scenes.test_scene(True).view()
+    h = hip_backend.trace(ent, [[0.5, 5.5, 5.25], [0.5, 5.5, 5.75]], [[1, 0, 0], [1, 0, 0]])
+    assert h["quad"][0] == 6 and abs(h["t"][0] - 8.0) < 1e-3 and abs(h["t"][1] - 19.5) < 1e-3
+
+
+def test_trace_on_the_test_scenes_matches_the_oracle(hip_backend, oracle):
+    rng = np.random.default_rng(3)
+    for s in [scenes.test_scene(False), scenes.test_scene(True), scenes.wall_scene(),
+              scenes.lamp_scene(EMITTER_SAMPLING_ONE, True)]:
+        v = s.view()
+        dims = v.params["grid_dims"][:3].astype(np.float64) * 16
+        n = 5000
+        o = (rng.random((n, 3)) * (dims * 1.4) - dims * 0.2).astype("<f4")
+        d = rays._unit(rng.normal(size=(n, 3)))
+        d[: n // 5, rng.integers(0, 3)] = 0.0
+        d = rays._unit(d)
+        want = oracle.trace(v, o, d)
+        got = hip_backend.trace(v, o, d)
+        assert trace_fields_equal(got, want).all()
+
+
+def sample_set(params, n, spp, seed):
+    rng = np.random.default_rng(seed)
+    w, h = int(params["frame"][0]), int(params["frame"][1])
+    return rng.integers(0, w, n), rng.integers(0, h, n), rng.integers(0, spp, n)
+
+
+def radiance_report(got, want):
+    same = np.all(got.view("<u4") == want.view("<u4"), axis=1)
+    denom = np.maximum(np.abs(want), 1e-3)
+    rel = np.max(np.abs(got - want) / denom, axis=1)
+    return same, rel
+
+
+def test_render_sample_is_bit_exact_on_config_0(hip_backend, oracle, frame0):
+    v = frame0.view()
+    px, py, s = sample_set(frame0.params, 40000, 16, 5)
+    want = oracle.render_samples(v, px, py, s)
+    got = hip_backend.render_samples(v, px, py, s)
+    same, rel = radiance_report(got, want)
+    print(f"cfg0: {int(same.sum())}/{len(same)} samples bit-exact, worst rel {rel.max():.2e}")
+    assert same.all()
+
+
+def test_render_sample_parity_on_config_1(hip_backend, oracle, frame1):
+    v = frame1.view()
+    px, py, s = sample_set(frame1.params, 40000, 256, 6)
+    want = oracle.render_samples(v, px, py, s)
+    got = hip_backend.render_samples(v, px, py, s)
+    same, rel = radiance_report(got, want)
+    print(f"cfg1: {int(same.sum())}/{len(same)} samples bit-exact, worst rel {rel.max():.2e}")
+    assert np.all(np.isfinite(got) == np.isfinite(want))
+    assert rel.max() <= 1e-4
+    assert same.mean() > 0.99
+
+
+def test_render_hq_matches_the_cpu_backend(hip_backend, oracle, frame0, frame1):
+    for fr, name in [(frame0, "cfg0"), (frame1, "cfg1")]:
+        spp = 4
+        want = oracle.render_hq(fr.view(), spp).reshape(-1, 3)
+        got = hip_backend.render_hq(frames.HqFrame(fr.scene, fr.params, spp, fr.sky))
+        same = np.all(got.view("<u4") == want.view("<u4"), axis=1)
+        mae = float(np.abs(got.astype(np.float64) - want).mean())
+        print(f"{name}: {int(same.sum())}/{len(same)} pixels bit-exact, MAE {mae:.3e}")
+        assert mae < 1e-3
+        if name == "cfg0":
+            assert same.all()
+
+
+def test_render_hq_kat_scenes_on_the_device(hip_backend):
+    # tests.rs:475-492: the lamp seen from the camera is 0.5 * 13 in every sample.
+    s = scenes.lamp_scene(EMITTER_SAMPLING_ONE, False)
+    p = s.params
+    p["frame"] = [1, 1, 0, 1]
+    p["cam_pos"] = [7.5, 2.0, 7.5, 0.01]
+    p["cam_fwd"] = [0.0, 1.0, 0.0, 1.0]
+    p["cam_right"] = [1.0, 0.0, 0.0, 0.0]
+    p["cam_up"] = [0.0, 0.0, -1.0, 0.0]
+    c = hip_backend.render_samples(s.view(), np.zeros(16), np.zeros(16), np.arange(16))
+    assert np.all(np.abs(c[:, :2] - 6.5) < 1e-4)
+    # tests.rs:804-831: lens and light maps.
+    s = scenes.test_scene(False)
+    p = s.params
+    p["sky_map"] = [4.0, 2.0, 2.0, 2.0]
+    s.sky = np.concatenate([np.full(24, 5.0), np.full(12, 0.3)]).astype("<f4")
+    p["frame"] = [1, 1, 0, 1]
+    p["cam_pos"] = [40.25, 20.0, 8.25, 1e-3]
+    p["cam_right"] = [1.0, 0.0, 0.0, 0.0]
+    p["sun_dir"] = [0.0, 1.0, 0.0, 0.9995]
+    p["cam_fwd"] = [0.0, -1.0, 0.0, 1.0]
+    p["cam_up"] = [0.0, 0.0, -1.0, 0.0]
+    c = hip_backend.render_samples(s.view(), np.zeros(16), np.zeros(16), np.arange(16))
+    assert np.all(np.abs(c - 0.3) < 1e-4)
+
+
+def test_render_hq_is_deterministic_and_splits_exactly(hip_backend, frame1):
+    fr = frames.HqFrame(frame1.scene, frame1.params, 8, frame1.sky)
+    a = hip_backend.render_hq(fr)
+    b = hip_backend.render_hq(fr)
+    assert np.array_equal(a.view("<u4"), b.view("<u4"))
+    scene = hip_backend.upload(fr)
+    c = hip_backend.render_hq_scene(scene, fr.params, 8)
+    assert np.array_equal(a.view("<u4"), c.view("<u4"))
+    t = hip_backend.last_timing()
+    assert t["samples"] == 256 * 144 * 8 and t["launches"] >= 1
+    scene.free()
