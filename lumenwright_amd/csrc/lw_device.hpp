@@ -164,7 +164,7 @@ __device__ __forceinline__ bool is_see_through(const lw_params &p, uint32_t qi) 
 }
 
 // trace.rs:120-131
-__device__ __noinline__ bool see_through_hit(const DevScene &s, const lw_params &p, V3 o, V3 d,
+__device__ __forceinline__ bool see_through_hit(const DevScene &s, const lw_params &p, V3 o, V3 d,
                                              float tmin, Hit &out) {
   Hit best{INFINITY, UINT32_MAX, 0u, 0.0f, 0.0f};
   for (uint32_t i = 0; i < p.see_through[1]; i++) test_quad(s, p, p.see_through[0] + i, o, d, tmin, best);
@@ -188,7 +188,7 @@ __device__ __forceinline__ float next_boundary(float o, float inv, int32_t cell,
 }
 
 // trace.rs:137-290. cell_out (when non-null) receives the cell the traversal stopped in.
-__device__ __noinline__ bool trace(const DevScene &s, const lw_params &p, V3 o, V3 d, float tmin,
+__device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 o, V3 d, float tmin,
                                    float tmax, Hit &out, int32_t *cell_out) {
   const uint32_t dx = p.grid_dims[0], dy = p.grid_dims[1], dz = p.grid_dims[2];
   const V3 size = V3{(float)(dx * 16u), (float)(dy * 16u), (float)(dz * 16u)};
@@ -354,7 +354,7 @@ __device__ __forceinline__ V3 rgb_from_hsl(float hue, float saturation, float li
 }
 
 // integrator.rs:158-188
-__device__ __noinline__ V3 sky_radiance(const DevScene &s, const lw_params &p, V3 d) {
+__device__ __forceinline__ V3 sky_radiance(const DevScene &s, const lw_params &p, V3 d) {
   const float n = p.misc_f[2];
   if (n < 1.0f) return V3{0.0f, 0.0f, 0.0f};
   float theta = f_atan2(d.z, d.x) / (2.0f * PI_F);
@@ -380,7 +380,7 @@ __device__ __noinline__ V3 sky_radiance(const DevScene &s, const lw_params &p, V
 }
 
 // integrator.rs:196-217
-__device__ __noinline__ V3 map_lookup(const DevScene &s, uint32_t base, float w, float h, V3 d) {
+__device__ __forceinline__ V3 map_lookup(const DevScene &s, uint32_t base, float w, float h, V3 d) {
   float u = f_atan2(d.z, d.x) / (2.0f * PI_F);
   u = rem1(rem1(u) + 1.0f);
   const float v = 1.0f - f_acos(f_clamp(d.y, -1.0f, 1.0f)) / PI_F;
@@ -415,7 +415,7 @@ __device__ __forceinline__ V3 sky_light(const DevScene &s, const lw_params &p, V
 }
 
 // integrator.rs:252-277
-__device__ __noinline__ V3 sun_texture(const DevScene &s, const lw_params &p, const float *lut,
+__device__ __forceinline__ V3 sun_texture(const DevScene &s, const lw_params &p, const float *lut,
                                        V3 d) {
   const float *sprite = p.sun_sprite;
   if (sprite[2] <= 0.0f || dot(d, V3{p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]}) < 0.5f)
@@ -460,7 +460,7 @@ __device__ __forceinline__ V3 sample_sun(const lw_params &p, Rng &rng) {
 __device__ __forceinline__ V3 quad_normal(const float4 *q) { return cross(xyz(q[2]), xyz(q[1])); }
 
 // integrator.rs:318-346
-__device__ __noinline__ V3 shadow(const DevScene &s, const lw_params &p, const float *lut, V3 o,
+__device__ __forceinline__ V3 shadow(const DevScene &s, const lw_params &p, const float *lut, V3 o,
                                   V3 d, float tmax) {
   const float eps = p.misc_f[1];
   V3 tr{1.0f, 1.0f, 1.0f};
@@ -537,7 +537,7 @@ __device__ __forceinline__ V3 sample_emitter_face(const DevScene &s, const lw_pa
 }
 
 // integrator.rs:418-433
-__device__ __noinline__ V3 sample_emitter(const DevScene &s, const lw_params &p, const float *lut,
+__device__ __forceinline__ V3 sample_emitter(const DevScene &s, const lw_params &p, const float *lut,
                                           V3 origin, V3 n, const lw_emitter &e, Rng &rng) {
   const uint32_t faces = e.faces[0];
   if (faces == 0) return V3{0.0f, 0.0f, 0.0f};

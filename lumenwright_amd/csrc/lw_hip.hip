@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -25,6 +26,7 @@
 #include "lw/abi.h"
 #include "lw_device.hpp"
 #include "lw_host.hpp"
+#include "lw_path.hpp"
 
 using namespace lw;
 
@@ -134,6 +136,107 @@ __global__ void __launch_bounds__(kBlock) k_render_samples(const lw_params p, co
   out[3 * i + 2] = c.z;
 }
 
+// The samples of `samples` consecutive sample indices of a frame, as work items: item
+// ((sl * tiles) + tile) * 256 + t is pixel t of 16 x 16 tile `tile` (four 8 x 8 wave
+// tiles) at sample sample_start + sl.
+struct FrameWork {
+  uint32_t w, h, tiles_x, tiles, sample_start, total;
+  float *__restrict__ rad;
+  __device__ __forceinline__ bool decode(uint32_t item, uint32_t &px, uint32_t &py,
+                                         uint32_t &sample) const {
+    const uint32_t t = item & 255u, rest = item >> 8;
+    const uint32_t tile = rest % tiles, sl = rest / tiles;
+    const uint32_t wave = t >> 6, lane = t & 63u;
+    px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    sample = sample_start + sl;
+    return px < w && py < h;
+  }
+  __device__ __forceinline__ void write(uint32_t item, V3 c) const {
+    uint32_t px, py, sample;
+    decode(item, px, py, sample);
+    float *o = rad + ((size_t)(sample - sample_start) * h * w + (size_t)py * w + px) * 3;
+    o[0] = c.x;
+    o[1] = c.y;
+    o[2] = c.z;
+  }
+};
+
+// An explicit list of (px, py, sample) triples.
+struct ListWork {
+  const uint32_t *__restrict__ px;
+  const uint32_t *__restrict__ py;
+  const uint32_t *__restrict__ sample;
+  float *__restrict__ out;
+  uint32_t total;
+  __device__ __forceinline__ bool decode(uint32_t item, uint32_t &x, uint32_t &y,
+                                         uint32_t &s) const {
+    x = px[item];
+    y = py[item];
+    s = sample[item];
+    return true;
+  }
+  __device__ __forceinline__ void write(uint32_t item, V3 c) const {
+    out[3 * (size_t)item] = c.x;
+    out[3 * (size_t)item + 1] = c.y;
+    out[3 * (size_t)item + 2] = c.z;
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *__restrict__ p, uint32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Persistent path tracing: each lane runs one path at a time as a PathState; the wave
+// traces one ray per lane per iteration, whichever kind of ray each lane needs. A lane
+// whose path ends writes its radiance and takes the next work item: the wave's idle lanes
+// are counted by a ballot, one atomic per wave reserves that many items, and each lane
+// takes its rank among them. A wave ends once the items are exhausted and all its lanes
+// are done; every path ends (MAX_EVENTS, MAX_STEPS), so every wave ends.
+template <class Work>
+__global__ void __launch_bounds__(kBlock) k_paths(const lw_params p, const DevScene s,
+                                                  const Work work,
+                                                  uint32_t *__restrict__ counter) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const Tracer T{s, p, lut_s};
+  const uint32_t lane = __lane_id();
+  PathState P;
+  P.stage = ST_IDLE;
+  P.item = 0;
+  for (;;) {
+    const bool need = P.stage == ST_IDLE;
+    const uint64_t mask = __ballot(need);
+    if (mask != 0) {
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (need) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        const uint32_t item = base + rank;
+        if (item < work.total) {
+          uint32_t px, py, sample;
+          if (work.decode(item, px, py, sample)) {
+            P.item = item;
+            T.start(P, px, py, sample);
+          }
+        } else {
+          P.stage = ST_DONE;
+        }
+      }
+    }
+    if (__ballot(P.stage != ST_DONE) == 0) break;
+    if (P.stage == ST_EXTEND || P.stage == ST_SHADOW || P.stage == ST_EMITTER) {
+      Hit h;
+      const bool hit = trace(s, p, P.ro, P.rd, P.rtmin, P.rtmax, h, nullptr);
+      T.consume(P, hit, h);
+      if (P.stage == ST_IDLE) work.write(P.item, P.radiance);
+    }
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------
@@ -238,7 +341,12 @@ struct lw_hip_backend {
   DevBuf<float> rad;
   DevBuf<double> acc;
   DevBuf<float> out;
+  DevBuf<uint32_t> counter;
   lw_timing timing{};
+  // Launch shape: the persistent path kernels run `blocks_*` workgroups of kBlock lanes
+  // (the device's resident capacity); LW_KERNEL=direct selects one thread per sample.
+  uint32_t blocks_frame = 0, blocks_list = 0;
+  bool direct = false;
 };
 
 namespace {
@@ -421,14 +529,23 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     HIP_TRY(hipGetLastError());
   }
 
-  const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+  const uint32_t tiles_x = (w + 15) / 16;
+  const uint32_t tiles = tiles_x * ((h + 15) / 16);
+  if ((uint64_t)tiles * 256u * max_batch > 0xFFFFFFFFull) throw InvalidScene{"frame too large"};
   lw_timing tm{};
   uint32_t done = 0, batch = 1;
   while (done < spp) {
     const uint32_t nb = std::min(batch, std::min(spp - done, max_batch));
     const auto start = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    k_render<<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(p, ds, done, b->rad.ptr);
+    if (b->direct) {
+      k_render<<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(p, ds, done, b->rad.ptr);
+    } else {
+      const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
+      const uint32_t grid = std::min<uint32_t>(b->blocks_frame, (fw.total + kBlock - 1) / kBlock);
+      k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
+      k_paths<FrameWork><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(p, ds, fw, b->counter.ptr);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     k_accumulate<<<dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
@@ -543,6 +660,20 @@ int lw_hip_open(int device, lw_hip_backend **out) {
     b->name = nm;
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto &ev : b->ev) HIP_TRY(hipEventCreate(&ev));
+    b->counter.alloc(1);
+    int per_cu_frame = 0, per_cu_list = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu_frame, reinterpret_cast<const void *>(&k_paths<FrameWork>), kBlock, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu_list, reinterpret_cast<const void *>(&k_paths<ListWork>), kBlock, 0));
+    const uint32_t cus = (uint32_t)std::max(1, prop.multiProcessorCount);
+    b->blocks_frame = cus * (uint32_t)std::max(1, per_cu_frame);
+    b->blocks_list = cus * (uint32_t)std::max(1, per_cu_list);
+    if (const char *e = std::getenv("LW_PERSISTENT_BLOCKS_PER_CU")) {
+      const int v = std::atoi(e);
+      if (v > 0) b->blocks_frame = b->blocks_list = cus * (uint32_t)v;
+    }
+    if (const char *e = std::getenv("LW_KERNEL")) b->direct = std::strcmp(e, "direct") == 0;
   });
   if (rc != LW_OK) {
     lw_hip_close(b);
@@ -651,8 +782,16 @@ int lw_hip_render_samples(lw_hip_backend *b, const lw_scene_view *view, size_t n
     d_py.upload(py, n, b->stream);
     d_s.upload(sample, n, b->stream);
     d_out.alloc(3 * n);
-    k_render_samples<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
-        *view->params, sc.dev(), (uint32_t)n, d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr);
+    if (b->direct) {
+      k_render_samples<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+          *view->params, sc.dev(), (uint32_t)n, d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr);
+    } else {
+      const ListWork lw{d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr, (uint32_t)n};
+      const uint32_t grid = std::min<uint32_t>(b->blocks_list, (uint32_t)((n + kBlock - 1) / kBlock));
+      k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
+      k_paths<ListWork><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(*view->params, sc.dev(), lw,
+                                                                     b->counter.ptr);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out_rgb, d_out.ptr, 3 * n * sizeof(float), hipMemcpyDeviceToHost,
                            b->stream));
