@@ -61,6 +61,10 @@ long lw_sky_cache(const lw_params *params, float *out, size_t cap);
  * scene.rs:299-306), with libm's expf. */
 void lw_gamma_lut(float out[256]);
 
+/* The f32 e^x the device kernels evaluate (Beer-Lambert transmittance, math.rs:298-302),
+ * computed here on the host for testing against libm's expf. */
+void lw_expf_tabulated(const float *x, size_t n, float *out);
+
 /* Camera::basis (camera.rs:20-37) for Minecraft yaw/pitch in degrees. */
 void lw_camera_basis(float yaw_deg, float pitch_deg, float fwd[3], float right[3], float up[3]);
 

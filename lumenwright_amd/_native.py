@@ -45,6 +45,7 @@ SIGNATURES = {
     "lw_sun_texture_axes": (None, [vp, vp, vp]),
     "lw_sky_cache": (ctypes.c_long, [vp, vp, sz]),
     "lw_gamma_lut": (None, [vp]),
+    "lw_expf_tabulated": (None, [vp, sz, vp]),
     "lw_camera_basis": (None, [f32, f32, vp, vp, vp]),
     "lw_camera_rays_new": (None, [f32, f32, f32, u32, u32, vp]),
     "lw_hq_params": (None, [vp, f32, u32, u32, vp, vp, vp, vp, f64, vp]),

@@ -405,6 +405,10 @@ long lw_sky_cache(const lw_params *params, float *out, size_t cap) {
 
 void lw_gamma_lut(float out[256]) { lw::gamma_lut(out); }
 
+void lw_expf_tabulated(const float *x, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = expf_tabulated(x[i]);
+}
+
 void lw_camera_basis(float yaw_deg, float pitch_deg, float fwd[3], float right[3], float up[3]) {
   const float RADS = PI_F / 180.0f;  // f32::to_radians
   float y = yaw_deg * RADS, p = pitch_deg * RADS;

@@ -1,0 +1,59 @@
+"""ORACLE -- test infrastructure only.
+
+Counts the algorithmic scene-buffer bytes per sample of each benchmark configuration with
+the instrumented restatement (lw_oracle_counted.cpp) and writes profiles/b_alg.json, which
+bench.py reads for its roofline figure. Counting rules (per sample):
+  8 B per DDA step (the top-level and overlay entries, trace.rs:181-182), 4 B per stored
+  brick word read (trace.rs:239), 16 B per model (trace.rs:245), 80 B per quad tested
+  (trace.rs:82), 4 B per texel fetched (scene.rs:293), 8 B per overlay cell and 4 B per
+  overlay index (trace.rs:254-259), 80 B to re-read the quad a path or shadow ray stopped
+  at (integrator.rs:326,630), 48 B per sky cache or sky map lookup (integrator.rs:183-185,
+  210-216), and for emitter sampling 28 B of grid header, 8 B per cell pair, 4 B per list
+  index, 32 B per emitter and 80 B per face (integrator.rs:359-372,391,454-455).
+Params (constant) and the per-sample output are not counted.
+
+    python oracle/count_bytes.py [--spp N]
+"""
+
+import argparse
+import json
+import os
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--spp", type=int, default=2)
+    args = ap.parse_args()
+    from lumenwright_amd import frames
+    from oracle import ffi
+    ffi.build()
+    out = {"rules": __doc__.split("Counting rules (per sample):")[1].split("Params")[0].strip(),
+           "counted_with": f"oracle render_hq over the first {args.spp} samples of every pixel"}
+    for n in (0, 1):
+        cfg = frames.CONFIGS[n]
+        fr = frames.make_frame(cfg)
+        _, c = ffi.render_hq_counted(fr.view(), args.spp, threads=os.cpu_count() or 1)
+        s = c.samples
+        out[f"config{n}"] = {
+            "workload": cfg.name,
+            "samples_counted": s,
+            "bytes_per_sample": c.bytes / s,
+            "traces_per_sample": c.traces / s,
+            "dda_steps_per_sample": c.dda_steps / s,
+            "quads_tested_per_sample": c.quads_tested / s,
+            "sky_lookups_per_sample": c.sky_lookups / s,
+        }
+        print(n, out[f"config{n}"], flush=True)
+    path = ROOT / "profiles" / "b_alg.json"
+    path.parent.mkdir(exist_ok=True)
+    path.write_text(json.dumps(out, indent=1) + "\n")
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

@@ -310,3 +310,35 @@ def test_synthetic_world_layout_decodes_to_the_terrain():
             assert spec.cube_types < m <= spec.cube_types + spec.cutout_types
         else:
             assert m == 0
+
+
+def test_device_exp_is_glibcs_expf(oracle):
+    # The device's f32 e^x (water transmittance, math.rs:298-302) follows glibc's expf
+    # method; it equals libm's expf (what Rust std calls) on 2^24 inputs spread over every
+    # exponent, on the transmittance range, and on the special values.
+    import ctypes
+    from lumenwright_amd import _native
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0, 2**32, size=1 << 24, dtype=np.uint64).astype("<u4")
+    x = bits.view("<f4")
+    x = x[np.isfinite(x)]
+    x = np.concatenate([x, -(rng.random(1 << 22) * 104).astype("<f4"),
+                        np.array([0.0, -0.0, 88.0, -88.0, 88.72284, 88.72285, -103.97, -103.98,
+                                  -104.0, np.inf, -np.inf, 1e-30, -1e-30], "<f4")])
+    x = np.ascontiguousarray(x, "<f4")
+    out = np.empty_like(x)
+    _native.lib().lw_expf_tabulated(ctypes.c_void_p(x.ctypes.data), x.size,
+                                    ctypes.c_void_p(out.ctypes.data))
+    want = oracle.expf(x)
+    differ = out.view("<u4") != want.view("<u4")
+    # The two inputs of all 2^32 where the methods part (an exhaustive CPU comparison): the
+    # f64 product lies within 1e-15 ulp of an f32 midpoint; one ulp apart.
+    known = np.isin(x.view("<u4"), np.array([0x4202422F, 0xC27C65D9], "<u4"))
+    assert not (differ & ~known).any(), (x[differ][:5], out[differ][:5], want[differ][:5])
+    assert np.all(np.abs(out[known].view("<i4").astype(np.int64)
+                         - want[known].view("<i4").astype(np.int64)) <= 1)
+    nan_in = np.array([np.nan], "<f4")
+    nan_out = np.empty_like(nan_in)
+    _native.lib().lw_expf_tabulated(ctypes.c_void_p(nan_in.ctypes.data), 1,
+                                    ctypes.c_void_p(nan_out.ctypes.data))
+    assert np.isnan(nan_out[0])
