@@ -193,6 +193,39 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *__restrict__ p, u
 // are counted by a ballot, one atomic per wave reserves that many items, and each lane
 // takes its rank among them. A wave ends once the items are exhausted and all its lanes
 // are done; every path ends (MAX_EVENTS, MAX_STEPS), so every wave ends.
+// Gives every idle lane of the wave a path: the wave's idle lanes are counted by a ballot,
+// one atomic reserves that many work items, and each lane takes its rank among them
+// (mbcnt). A lane whose item is outside the frame asks again; a lane that finds the items
+// exhausted is done. Afterwards every lane is running a path or done.
+template <class Work>
+__device__ __forceinline__ void refill(PathState &P, const Tracer &T, const Work &work,
+                                       uint32_t *counter) {
+  const uint32_t lane = __lane_id();
+  for (;;) {
+    const bool need = P.stage == ST_IDLE;
+    const uint64_t mask = __ballot(need);
+    if (mask == 0) return;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (need) {
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      const uint32_t item = base + rank;
+      if (item < work.total) {
+        uint32_t px, py, sample;
+        if (work.decode(item, px, py, sample)) {
+          P.item = item;
+          T.start(P, px, py, sample);
+        }
+      } else {
+        P.stage = ST_DONE;
+      }
+    }
+  }
+}
+
 template <class Work>
 __global__ void __launch_bounds__(kBlock) k_paths(const lw_params p, const DevScene s,
                                                   const Work work,
@@ -200,33 +233,11 @@ __global__ void __launch_bounds__(kBlock) k_paths(const lw_params p, const DevSc
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
   const Tracer T{s, p, lut_s};
-  const uint32_t lane = __lane_id();
   PathState P;
   P.stage = ST_IDLE;
   P.item = 0;
   for (;;) {
-    const bool need = P.stage == ST_IDLE;
-    const uint64_t mask = __ballot(need);
-    if (mask != 0) {
-      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-      base = __builtin_amdgcn_readlane(base, leader);
-      if (need) {
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        const uint32_t item = base + rank;
-        if (item < work.total) {
-          uint32_t px, py, sample;
-          if (work.decode(item, px, py, sample)) {
-            P.item = item;
-            T.start(P, px, py, sample);
-          }
-        } else {
-          P.stage = ST_DONE;
-        }
-      }
-    }
+    refill(P, T, work, counter);
     if (__ballot(P.stage != ST_DONE) == 0) break;
     if (P.stage == ST_EXTEND || P.stage == ST_SHADOW || P.stage == ST_EMITTER) {
       Hit h;
@@ -235,6 +246,87 @@ __global__ void __launch_bounds__(kBlock) k_paths(const lw_params p, const DevSc
       if (P.stage == ST_IDLE) work.write(P.item, P.radiance);
     }
   }
+}
+
+// ---- the wavefront form: path state in HBM, one lean trace kernel per iteration ----
+//
+// Each of n slots holds a PathState. An iteration is three launches:
+//   k_wf_logic  consumes each slot's last trace result (Tracer::consume), writes finished
+//               paths' radiance, gives idle slots new work items (refill), and appends each
+//               slot that now needs a ray to the ray queue -- compacted: the wave's ballot
+//               counts its lanes with a ray, one atomic reserves their queue entries, each
+//               lane writes at its rank;
+//   k_wf_flip   makes the appended queue the one to trace;
+//   k_wf_trace  traces the queued rays, one lane per ray, and stores each hit by slot.
+// The trace kernel holds no path state, so it runs at the DDA's own register count and
+// far more waves hide the latency of its dependent loads.
+struct WfSlots {
+  PathState *__restrict__ st;
+  float4 *__restrict__ ray_o;  // o.xyz, tmin
+  float4 *__restrict__ ray_d;  // d.xyz, tmax
+  uint4 *__restrict__ hit;     // t bits, quad, texel, hit
+  uint32_t *__restrict__ queue;
+  uint32_t *__restrict__ ctr;  // [0] work items taken, [1] rays to trace, [2] rays appended
+  uint32_t n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_wf_init(WfSlots w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 4) w.ctr[i] = 0u;
+  if (i >= w.n) return;
+  w.st[i].stage = ST_IDLE;
+}
+
+template <class Work>
+__global__ void __launch_bounds__(kBlock) k_wf_logic(const lw_params p, const DevScene s,
+                                                     const Work work, WfSlots w) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const Tracer T{s, p, lut_s};
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // n is a multiple of kBlock
+  PathState P = w.st[i];
+  if (P.stage == ST_DONE) return;
+  if (P.stage == ST_EXTEND || P.stage == ST_SHADOW || P.stage == ST_EMITTER) {
+    const uint4 r = w.hit[i];
+    const Hit h{bits_f(r.x), r.y, r.z, 0.0f, 0.0f};
+    T.consume(P, r.w != 0u, h);
+    if (P.stage == ST_IDLE) work.write(P.item, P.radiance);
+  }
+  refill(P, T, work, &w.ctr[0]);
+  w.st[i] = P;
+  const bool pending = P.stage == ST_EXTEND || P.stage == ST_SHADOW || P.stage == ST_EMITTER;
+  const uint64_t mask = __ballot(pending);
+  if (mask == 0) return;
+  const uint32_t lane = __lane_id();
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&w.ctr[2], (uint32_t)__popcll(mask));
+  base = __builtin_amdgcn_readlane(base, leader);
+  if (pending) {
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    w.queue[base + rank] = i;
+    w.ray_o[i] = make_float4(P.ro.x, P.ro.y, P.ro.z, P.rtmin);
+    w.ray_d[i] = make_float4(P.rd.x, P.rd.y, P.rd.z, P.rtmax);
+  }
+}
+
+__global__ void k_wf_flip(uint32_t *__restrict__ ctr) {
+  if (threadIdx.x == 0) {
+    ctr[1] = ctr[2];
+    ctr[2] = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_wf_trace(const lw_params p, const DevScene s,
+                                                     WfSlots w) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= w.ctr[1]) return;
+  const uint32_t i = w.queue[j];
+  const float4 o = w.ray_o[i], d = w.ray_d[i];
+  Hit h{INFINITY, UINT32_MAX, 0u, 0.0f, 0.0f};
+  const bool hit = trace(s, p, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, o.w, d.w, h, nullptr);
+  w.hit[i] = make_uint4(f_bits(h.t), h.quad, h.texel, hit ? 1u : 0u);
 }
 
 }  // namespace
@@ -326,6 +418,7 @@ struct lw_hip_scene {
   DevBuf<float> sky;
   DevBuf<float> lut;
   LayoutKey key;
+  uint32_t max_emitter_rays = 1;  // emitter face rays one bounce can trace
   DevScene dev() const {
     return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
                     atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr};
@@ -346,7 +439,16 @@ struct lw_hip_backend {
   // Launch shape: the persistent path kernels run `blocks_*` workgroups of kBlock lanes
   // (the device's resident capacity); LW_KERNEL=direct selects one thread per sample.
   uint32_t blocks_frame = 0, blocks_list = 0;
-  bool direct = false;
+  // LW_KERNEL: "persistent" (k_paths), "direct" (one thread per sample) or "wavefront"
+  // (k_wf_*, LW_WF_SLOTS path slots).
+  int mode = 0;
+  uint32_t wf_slots = 1u << 20;
+  DevBuf<PathState> wf_st;
+  DevBuf<float4> wf_o, wf_d;
+  DevBuf<uint4> wf_hit;
+  DevBuf<uint32_t> wf_queue, wf_ctr;
+  uint32_t *pinned = nullptr;
+  uint64_t last_iterations = 0;
 };
 
 namespace {
@@ -497,6 +599,57 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.lut.upload(lut, 256, b->stream);
   HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors above die with this scope
   sc.key = layout_key(*v.params);
+  // The most face rays one bounce's emitter sampling can trace (integrator.rs:441-469).
+  uint64_t max_faces = 0, max_count = 0;
+  for (size_t e = 0; e < v.emitters_len; e++) max_faces = std::max<uint64_t>(max_faces, v.emitters[e].faces[0]);
+  if (v.emitter_grid_len >= LW_EMITTER_GRID_HEADER) {
+    const uint32_t *g = v.emitter_grid;
+    const uint64_t cells = (uint64_t)g[1] * g[2] * g[3];
+    for (uint64_t c = 0; c < cells && LW_EMITTER_GRID_HEADER + 2 * c + 1 < v.emitter_grid_len; c++)
+      max_count = std::max<uint64_t>(max_count, g[LW_EMITTER_GRID_HEADER + 2 * c + 1]);
+  }
+  const uint64_t rays = max_faces * max_count;  // ALL's bound covers ONE and ONE_BLOCK
+  sc.max_emitter_rays = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(rays, 1), 1u << 20);
+}
+
+enum { MODE_PERSISTENT = 0, MODE_DIRECT = 1, MODE_WAVEFRONT = 2 };
+
+// The wavefront loop (k_wf_*) over `work`: iterations until no slot has a ray. Bounded: a
+// path has at most MAX_EVENTS events, each with one extension ray, at most 8 shadow
+// segments and the scene's emitter rays, and each slot runs at most ceil(total / n) + 1
+// paths; past that bound the loop stops with an error instead of spinning.
+template <class Work>
+void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, const Work &work) {
+  const uint32_t n = std::max<uint32_t>(
+      kBlock, std::min<uint32_t>(b->wf_slots, (work.total + kBlock - 1) / kBlock * kBlock));
+  b->wf_st.ensure(n);
+  b->wf_o.ensure(n);
+  b->wf_d.ensure(n);
+  b->wf_hit.ensure(n);
+  b->wf_queue.ensure(n);
+  b->wf_ctr.ensure(4);
+  const WfSlots w{b->wf_st.ptr, b->wf_o.ptr, b->wf_d.ptr, b->wf_hit.ptr, b->wf_queue.ptr,
+                  b->wf_ctr.ptr, n};
+  const DevScene ds = sc.dev();
+  const dim3 grid(n / kBlock), block(kBlock);
+  k_wf_init<<<grid, block, 0, b->stream>>>(w);
+  HIP_TRY(hipGetLastError());
+  const uint64_t per_path = (uint64_t)MAX_EVENTS * (1 + 8 + sc.max_emitter_rays);
+  const uint64_t limit = ((uint64_t)work.total / n + 2) * per_path + 64;
+  uint64_t it = 0;
+  for (;; it++) {
+    if (it > limit) throw InvalidScene{"wavefront loop exceeded its iteration bound"};
+    k_wf_logic<Work><<<grid, block, 0, b->stream>>>(p, ds, work, w);
+    k_wf_flip<<<dim3(1), dim3(64), 0, b->stream>>>(w.ctr);
+    k_wf_trace<<<grid, block, 0, b->stream>>>(p, ds, w);
+    HIP_TRY(hipGetLastError());
+    if ((it & 7) == 7) {
+      HIP_TRY(hipMemcpyAsync(b->pinned, w.ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      if (*b->pinned == 0) break;
+    }
+  }
+  b->last_iterations = it + 1;
 }
 
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
@@ -538,8 +691,11 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     const uint32_t nb = std::min(batch, std::min(spp - done, max_batch));
     const auto start = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (b->direct) {
+    if (b->mode == MODE_DIRECT) {
       k_render<<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(p, ds, done, b->rad.ptr);
+    } else if (b->mode == MODE_WAVEFRONT) {
+      const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
+      run_wavefront(b, sc, p, fw);
     } else {
       const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
       const uint32_t grid = std::min<uint32_t>(b->blocks_frame, (fw.total + kBlock - 1) / kBlock);
@@ -673,7 +829,15 @@ int lw_hip_open(int device, lw_hip_backend **out) {
       const int v = std::atoi(e);
       if (v > 0) b->blocks_frame = b->blocks_list = cus * (uint32_t)v;
     }
-    if (const char *e = std::getenv("LW_KERNEL")) b->direct = std::strcmp(e, "direct") == 0;
+    if (const char *e = std::getenv("LW_KERNEL")) {
+      if (std::strcmp(e, "direct") == 0) b->mode = MODE_DIRECT;
+      if (std::strcmp(e, "wavefront") == 0) b->mode = MODE_WAVEFRONT;
+    }
+    if (const char *e = std::getenv("LW_WF_SLOTS")) {
+      const long v = std::atol(e);
+      if (v >= kBlock) b->wf_slots = (uint32_t)std::min<long>(v, 1l << 24) / kBlock * kBlock;
+    }
+    HIP_TRY(hipHostMalloc((void **)&b->pinned, 64, hipHostMallocDefault));
   });
   if (rc != LW_OK) {
     lw_hip_close(b);
@@ -698,6 +862,13 @@ void lw_hip_close(lw_hip_backend *b) {
   b->rad.free();
   b->acc.free();
   b->out.free();
+  b->wf_st.free();
+  b->wf_o.free();
+  b->wf_d.free();
+  b->wf_hit.free();
+  b->wf_queue.free();
+  b->wf_ctr.free();
+  if (b->pinned) (void)hipHostFree(b->pinned);
   for (auto &ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -782,7 +953,10 @@ int lw_hip_render_samples(lw_hip_backend *b, const lw_scene_view *view, size_t n
     d_py.upload(py, n, b->stream);
     d_s.upload(sample, n, b->stream);
     d_out.alloc(3 * n);
-    if (b->direct) {
+    if (b->mode == MODE_WAVEFRONT) {
+      const ListWork lw{d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr, (uint32_t)n};
+      run_wavefront(b, sc, *view->params, lw);
+    } else if (b->mode == MODE_DIRECT) {
       k_render_samples<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
           *view->params, sc.dev(), (uint32_t)n, d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr);
     } else {

@@ -2,11 +2,13 @@
 
 Bars (the kernel crate's own outputs, restated by the oracle):
   * trace: bit-exact -- hit or miss, quad, texel, t bits and the final DDA cell;
-  * render_sample: bit-exact wherever the path does not cross water; within 1e-4 relative
-    where it does (the device's exp is correctly rounded, libm's expf is within one ulp);
+  * render_sample: bit-exact (the device's e^x follows libm's expf method; the two f32
+    inputs where they part are one ulp apart), and in any case within 1e-4 relative;
   * render_hq: the image equals the CPU backend's bit for bit where the samples do; image
     MAE < 1e-3 in linear radiance in any case.
 """
+
+import os
 
 import numpy as np
 import pytest
@@ -14,9 +16,32 @@ import pytest
 import rays
 import scenes
 from lumenwright_amd import frames
-from lumenwright_amd.abi import EMITTER_SAMPLING_ONE
+from lumenwright_amd.abi import (EMITTER_SAMPLING_ALL, EMITTER_SAMPLING_NONE,
+                                 EMITTER_SAMPLING_ONE, EMITTER_SAMPLING_ONE_BLOCK)
 
 pytestmark = pytest.mark.gpu
+
+MODES = ["persistent", "wavefront", "direct"]
+
+
+@pytest.fixture(scope="module")
+def mode_backends():
+    """One backend per kernel form (LW_KERNEL is read when a backend opens)."""
+    import lumenwright_amd as lw
+    out = {}
+    old = os.environ.get("LW_KERNEL")
+    try:
+        for m in MODES:
+            os.environ["LW_KERNEL"] = m
+            out[m] = lw.HipBackend.open(0)
+    finally:
+        if old is None:
+            os.environ.pop("LW_KERNEL", None)
+        else:
+            os.environ["LW_KERNEL"] = old
+    yield out
+    for b in out.values():
+        b.close()
 
 
 def trace_fields_equal(a, b):
@@ -185,3 +210,48 @@ def test_render_hq_is_deterministic_and_splits_exactly(hip_backend, frame1):
     t = hip_backend.last_timing()
     assert t["samples"] == 256 * 144 * 8 and t["launches"] >= 1
     scene.free()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_every_kernel_form_matches_the_oracle(mode_backends, oracle, frame0, frame1, mode):
+    b = mode_backends[mode]
+    for fr, spp, seed in [(frame0, 16, 21), (frame1, 256, 22)]:
+        v = fr.view()
+        px, py, s = sample_set(fr.params, 20000, spp, seed)
+        want = oracle.render_samples(v, px, py, s)
+        got = b.render_samples(v, px, py, s)
+        same, rel = radiance_report(got, want)
+        print(f"{mode}: {int(same.sum())}/{len(same)} bit-exact, worst rel {rel.max():.2e}")
+        assert rel.max() <= 1e-4 and same.mean() > 0.999
+    img = b.render_hq(frames.HqFrame(frame1.scene, frame1.params, 3, frame1.sky))
+    want = oracle.render_hq(frame1.view(), 3).reshape(-1, 3)
+    assert float(np.abs(img.astype(np.float64) - want).mean()) < 1e-3
+
+
+def lamp_camera(s):
+    p = s.params
+    p["frame"] = [8, 8, 0, 3]
+    p["cam_pos"] = [8.0, 3.2, 8.0, 0.9]
+    p["cam_fwd"] = [0.0, -1.0, 0.0, 1.0]
+    p["cam_right"] = [1.0, 0.0, 0.0, 0.0]
+    p["cam_up"] = [0.0, 0.0, -1.0, 0.0]
+    return s
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("strategy", [EMITTER_SAMPLING_NONE, EMITTER_SAMPLING_ONE,
+                                      EMITTER_SAMPLING_ONE_BLOCK, EMITTER_SAMPLING_ALL])
+@pytest.mark.parametrize("decoy", [False, True])
+def test_emitter_sampling_strategies_are_bit_exact(mode_backends, oracle, mode, strategy, decoy):
+    # The lamp scenes of tests.rs:310-395, seen from above the floor: emitter sampling of
+    # each strategy (integrator.rs:418-469), through every kernel form.
+    s = lamp_camera(scenes.lamp_scene(strategy, decoy))
+    v = s.view()
+    n = 8 * 8 * 64
+    px = np.tile(np.repeat(np.arange(8), 8), 64)
+    py = np.tile(np.arange(8), 8 * 64)
+    smp = np.repeat(np.arange(64), 64)
+    want = oracle.render_samples(v, px, py, smp)
+    got = mode_backends[mode].render_samples(v, px, py, smp)
+    assert np.array_equal(got.view("<u4"), want.view("<u4")), (strategy, decoy)
+    assert n == len(got) and want.mean() > 0.0
