@@ -45,6 +45,10 @@ struct DevScene {
   const uint32_t *__restrict__ egrid;
   const float *__restrict__ sky;
   const float *__restrict__ gamma_lut;  // 256 decoded byte values
+  // Per stored brick, which of its 64 sub-blocks of 4 x 4 x 4 voxels hold a non-zero model
+  // (bit ((y/4)*4 + z/4)*4 + x/4). A voxel of an empty sub-block is model 0 without a
+  // load of its brick word.
+  const uint2 *__restrict__ occupancy;
 };
 
 struct Hit {
@@ -218,6 +222,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
   const float best_t = t_end + 1e-3f;
 
   uint32_t cached_b = UINT32_MAX, entry = EMPTY, ov = EMPTY;
+  uint2 occ = make_uint2(0u, 0u);
   for (uint32_t it = 0; it < MAX_STEPS; it++) {
     const uint32_t bx = (uint32_t)(cell[0] >> 4);
     const uint32_t by = (uint32_t)(cell[1] >> 4);
@@ -227,6 +232,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
       entry = top[b];
       ov = ov_top[b];
       cached_b = b;
+      if (entry != EMPTY && (entry & UNIFORM) == 0) occ = s.occupancy[entry];
     }
     if (cell_out) {
       cell_out[0] = cell[0];
@@ -281,8 +287,15 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
     } else if ((entry & UNIFORM) != 0) {
       model = entry & ~UNIFORM;
     } else {
-      const uint32_t w = s.grid[p.off[1] + entry * LW_BRICK_WORDS + (local >> 1)];
-      model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
+      const uint32_t sb = ((((uint32_t)cell[1] & 15u) >> 2) * 4u + (((uint32_t)cell[2] & 15u) >> 2)) * 4u +
+                          (((uint32_t)cell[0] & 15u) >> 2);
+      const uint32_t bits = sb < 32u ? occ.x : occ.y;
+      if ((bits >> (sb & 31u)) & 1u) {
+        const uint32_t w = s.grid[p.off[1] + entry * LW_BRICK_WORDS + (local >> 1)];
+        model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
+      } else {
+        model = 0;
+      }
     }
     Hit cand{f_min(t_hi, best_t), UINT32_MAX, 0u, 0.0f, 0.0f};
     if (model != 0) {

@@ -417,11 +417,12 @@ struct lw_hip_scene {
   DevBuf<uint32_t> egrid;
   DevBuf<float> sky;
   DevBuf<float> lut;
+  DevBuf<uint2> occupancy;
   LayoutKey key;
   uint32_t max_emitter_rays = 1;  // emitter face rays one bounce can trace
   DevScene dev() const {
     return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
-                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr};
+                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr, occupancy.ptr};
   }
 };
 
@@ -581,6 +582,32 @@ uint32_t quad_info(const lw_quad &q) {
          (nv ? QI_NEG_V : 0u) | (nn ? QI_NEG_N : 0u);
 }
 
+// The sub-block occupancy of every stored brick (DevScene::occupancy), indexed by the
+// brick's top-level entry.
+std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
+  const lw_params &p = *v.params;
+  const uint64_t nb = (uint64_t)p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2];
+  uint64_t stored = 0;
+  for (uint64_t b = 0; b < nb; b++) {
+    const uint32_t e = v.grid[p.off[0] + b];
+    if (e != LW_EMPTY && (e & LW_UNIFORM) == 0) stored = std::max<uint64_t>(stored, (uint64_t)e + 1);
+  }
+  std::vector<uint2> occ(std::max<uint64_t>(stored, 1), make_uint2(0u, 0u));
+  for (uint64_t e = 0; e < stored; e++) {
+    const uint32_t *words = v.grid + p.off[1] + e * LW_BRICK_WORDS;
+    uint64_t m = 0;
+    for (uint32_t local = 0; local < 4096; local++) {
+      const uint32_t id = (words[local >> 1] >> ((local & 1u) * 16u)) & 0xFFFFu;
+      if (id != 0) {
+        const uint32_t x = local & 15u, z = (local >> 4) & 15u, y = local >> 8;
+        m |= 1ull << (((y >> 2) * 4u + (z >> 2)) * 4u + (x >> 2));
+      }
+    }
+    occ[e] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+  }
+  return occ;
+}
+
 void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   validate(v);
   HIP_TRY(hipSetDevice(b->device));
@@ -597,6 +624,8 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   float lut[256];
   gamma_lut(lut);
   sc.lut.upload(lut, 256, b->stream);
+  const std::vector<uint2> occ = occupancy_masks(v);
+  sc.occupancy.upload(occ.data(), occ.size(), b->stream);
   HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors above die with this scope
   sc.key = layout_key(*v.params);
   // The most face rays one bounce's emitter sampling can trace (integrator.rs:441-469).
