@@ -1,14 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark: Msamples/s of the HQ path tracer on one MI355X.
 
-A step is one render_hq of the workload (default: config 1, 1280 x 720 at 256 spp with 8
-bounces over the 24 x 24-chunk world with water and emitters, game dusk sky) with the
-scene resident in HBM. W untimed warmup steps, then K timed steps; prints one JSON line.
+Default (config 1): a step is one render_hq of 1280 x 720 at 256 spp, 8 bounces, over the
+24 x 24-chunk world with water and emitters under the game dusk sky, with the scene
+resident in HBM. W untimed warmup steps, then K timed steps; one JSON line.
+
+--config 2: the batched workload -- 64 seeded views of the config-1 world with sizes from
+{160x90 .. 1280x720} and spp from {16 .. 1024} -- rendered once as one batch
+(lw_hip_render_hq_batch) and once as 64 plain render_hq calls; a step renders all 64 views
+both ways. The images are identical (same samples, same summation order), so the two are
+compared at equal image error.
 
 Besides the throughput the line carries:
-  roofline      the path-tracing kernel's algorithmic scene bytes per launch (per-sample
+  roofline      the path-tracing kernels' algorithmic scene bytes per launch (per-sample
                 bytes counted once by the instrumented oracle, profiles/b_alg.json, times
-                the samples of a launch) over its mean launch time (HIP events on the
+                the samples of a launch) over their mean launch time (HIP events on the
                 library's stream), against the 8 TB/s HBM peak;
   cpu_baseline  the oracle (the C++ restatement of the reference's CPU backend) on the
                 host's cores over a bounded slice of the same frame (the first S samples of
@@ -28,6 +34,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Msamples/s at 720p 8-bounce hq; batched vs plain; adjoint/forward time ratio"
+
+
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
 def host_cores() -> int:
@@ -40,7 +51,7 @@ def host_cores() -> int:
 
 def cpu_baseline(frame, seconds: float):
     """The oracle's CpuBackend::render_hq over the frame's first spp samples, doubling spp
-    until one render takes at least `seconds / 4`, then repeating to fill `seconds`."""
+    until the total time reaches `seconds`."""
     from oracle import ffi
     ffi.build()
     from lumenwright_amd.abi import HqFrame
@@ -56,47 +67,47 @@ def cpu_baseline(frame, seconds: float):
         total_n += w * h * spp
         if total_s >= seconds or spp >= frame.spp:
             break
-        if dt < seconds / 4:
-            spp = min(spp * 2, frame.spp)
+        spp = min(spp * 2, frame.spp)
     return {"value": total_n / total_s / 1e6, "unit": "Msamples/s", "cores": cores,
             "kind": "port",
-            "sample": f"{w}x{h}, the first {spp} samples of every pixel (and smaller passes "
-                      f"before it), {total_n} samples in {total_s:.1f} s"}
+            "sample": f"{w}x{h}, passes of the first 1, 2, 4, ... {spp} samples of every "
+                      f"pixel: {total_n} samples in {total_s:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=None, help="override the workload's spp")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path "
-              "shards by sample range or tile with one f64 reduction, which this build does "
-              "not run)", file=sys.stderr)
-        sys.exit(2)
+def roofline(config, samples, trace_ms, launches):
+    b_alg_path = ROOT / "profiles" / "b_alg.json"
+    if not b_alg_path.exists():
+        return None
+    b_alg = json.loads(b_alg_path.read_text()).get(f"config{config}")
+    if not b_alg or launches == 0:
+        return None
+    per_launch = samples / launches
+    mean_s = trace_ms / 1e3 / launches
+    achieved = b_alg["bytes_per_sample"] * per_launch / mean_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)",
+            "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"]}
 
-    import lumenwright_amd as lw
-    from lumenwright_amd import frames
 
+def kernel_form():
+    return os.environ.get("LW_KERNEL", "persistent") + (
+        f", waves {os.environ['LW_WAVES']}" if "LW_WAVES" in os.environ else "")
+
+
+def bench_plain(args, lw, frames):
     cfg = frames.CONFIGS[args.config]
     if args.spp:
         cfg = frames.config(args.config, spp=args.spp)
     t0 = time.perf_counter()
     frame = frames.make_frame(cfg)
-    print(f"# frame built in {time.perf_counter() - t0:.1f} s: {cfg.name}", file=sys.stderr,
-          flush=True)
-
+    log(f"frame built in {time.perf_counter() - t0:.1f} s: {cfg.name}")
     b = lw.HipBackend.open(0)
     scene = b.upload(frame)
     w, h, spp = cfg.width, cfg.height, cfg.spp
     for i in range(args.warmup):
         b.render_hq_scene(scene, frame.params, spp)
-        print(f"# warmup {i + 1}/{args.warmup}: {b.last_timing()}", file=sys.stderr, flush=True)
+        log(f"warmup {i + 1}/{args.warmup}: {b.last_timing()}")
     trace_ms, launches, acc_ms = 0.0, 0, 0.0
     t = time.perf_counter()
     for i in range(args.steps):
@@ -105,29 +116,22 @@ def main():
         trace_ms += tm["trace_ms"]
         acc_ms += tm["accumulate_ms"]
         launches += tm["launches"]
-        print(f"# step {i + 1}/{args.steps}: {tm}", file=sys.stderr, flush=True)
+        log(f"step {i + 1}/{args.steps}: {tm}")
     elapsed = time.perf_counter() - t
     samples = w * h * spp * args.steps
     value = samples / elapsed / 1e6
     assert np.isfinite(img).all()
-
-    roofline = None
-    b_alg_path = ROOT / "profiles" / "b_alg.json"
-    if b_alg_path.exists():
-        b_alg = json.loads(b_alg_path.read_text()).get(f"config{args.config}")
-        if b_alg:
-            per_launch_samples = samples / launches
-            mean_launch_s = trace_ms / 1e3 / launches
-            bytes_per_launch = b_alg["bytes_per_sample"] * per_launch_samples
-            achieved = bytes_per_launch / mean_launch_s / 1e9
-            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                        "kernel": "k_render", "mean_launch_ms": trace_ms / launches,
-                        "bytes_per_sample": b_alg["bytes_per_sample"]}
-
+    extra = {}
+    if args.copy_probe:
+        ms = b.copy_probe(args.copy_probe)
+        extra["copy_probe"] = {"bytes": args.copy_probe, "ms": ms,
+                               "GBps": 2 * args.copy_probe / ms / 1e6}
+        log(f"copy probe: {extra['copy_probe']}")
+    scene.free()
+    b.close()
     cpu = None if args.no_cpu_baseline else cpu_baseline(frame, args.cpu_seconds)
     line = {
-        "metric": "Msamples/s at 720p 8-bounce hq",
+        "metric": METRIC,
         "value": value,
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -140,18 +144,96 @@ def main():
         "dtype": "f32 path arithmetic, f64 accumulation",
         "data": "synthetic (seeded value-noise voxel world, procedural sky)",
         "config": {"workload": cfg.name, "width": w, "height": h, "spp": spp,
-                   "max_bounces": cfg.max_bounces, "parallelism": "1 GPU"},
+                   "max_bounces": cfg.max_bounces, "kernel": kernel_form(),
+                   "parallelism": "1 GPU"},
         "kernel_ms_per_step": trace_ms / args.steps,
         "accumulate_ms_per_step": acc_ms / args.steps,
         "launches_per_step": launches / args.steps,
-        "roofline": roofline,
+        "roofline": roofline(args.config, samples, trace_ms, launches),
         "cpu_baseline": cpu,
+        **extra,
     }
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_batch(args, lw, frames):
+    cfg = frames.CONFIGS[1]
+    t0 = time.perf_counter()
+    scene_buf = frames.build_world(cfg.world)
+    base, views = frames.batch_views(cfg, scene_buf, n_views=64, seed=3)
+    if args.spp_scale != 1.0:
+        views = [(p, max(1, int(s * args.spp_scale))) for p, s in views]
+    samples = sum(int(p["frame"][0]) * int(p["frame"][1]) * s for p, s in views)
+    log(f"64 views built in {time.perf_counter() - t0:.1f} s: {samples / 1e9:.2f} G samples")
+    b = lw.HipBackend.open(0)
+    scene = b.upload(base)
+    batch_imgs = b.render_hq_batch(scene, views)  # warmup
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batch_imgs = b.render_hq_batch(scene, views)
+    t_batch = (time.perf_counter() - t) / args.steps
+    tb = b.last_timing()
+    log(f"batched: {t_batch:.2f} s per step, {tb}")
+    t = time.perf_counter()
+    plain_ms = 0.0
+    for _ in range(args.steps):
+        plain_imgs = [b.render_hq_scene(scene, p, s) for p, s in views]
+    t_plain = (time.perf_counter() - t) / args.steps
+    log(f"plain: {t_plain:.2f} s per step")
+    same = all(np.array_equal(x.view("<u4"), y.view("<u4")) for x, y in zip(batch_imgs, plain_imgs))
     scene.free()
     b.close()
+    return {
+        "metric": METRIC,
+        "value": samples / t_batch / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": t_batch * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 path arithmetic, f64 accumulation",
+        "data": "synthetic (seeded value-noise voxel world, procedural sky)",
+        "config": {"workload": "cfg2: 64 seeded views of the cfg1 world, sizes 160x90..1280x720, "
+                               "spp 16..1024, 8 bounces, one batch vs 64 plain calls",
+                   "views": 64, "samples": samples, "kernel": kernel_form(),
+                   "parallelism": "1 GPU"},
+        "plain_msamples_per_s": samples / t_plain / 1e6,
+        "batched_over_plain": t_plain / t_batch,
+        "images_identical": same,
+        "batch_passes": tb["launches"],
+        "roofline": roofline(1, samples, tb["trace_ms"], max(1, tb["launches"])),
+        "cpu_baseline": None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--spp", type=int, default=None, help="override the workload's spp")
+    ap.add_argument("--spp-scale", type=float, default=1.0, help="config 2: scale every view's spp")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="also time a copy of this many bytes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path "
+              "shards by sample range or tile with one f64 reduction, which this build does "
+              "not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import lumenwright_amd as lw
+    from lumenwright_amd import frames
+
+    line = bench_batch(args, lw, frames) if args.config == 2 else bench_plain(args, lw, frames)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

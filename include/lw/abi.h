@@ -225,6 +225,28 @@ int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene,
 
 void lw_hip_scene_free(lw_hip_backend *b, lw_hip_scene *scene);
 
+/* One view of a batch: its Params and samples per pixel. The views of a batch may differ
+ * only in the camera (cam_pos, cam_fwd, cam_right, cam_up), the frame size (frame[0..1])
+ * and the seed (misc_u[0]). */
+typedef struct lw_view {
+  lw_params params;
+  uint32_t spp;
+  uint32_t pad[3];
+} lw_view;
+
+/* Renders n views (1..256) of a resident scene as one work pool, in passes; each pass is
+ * one persistent launch over all views and replays as a captured hipGraph. out_rgb[i]
+ * receives view i's mean radiance, w*h*3 floats, equal to what render_hq gives for the
+ * view alone (same samples, same summation order). No counterpart in the reference, which
+ * renders one view per call (render/mod.rs:98-102). */
+int lw_hip_render_hq_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
+                           size_t n, lw_progress_fn progress, void *user, float *const *out_rgb);
+
+/* Diagnostics (no counterpart in the reference): copies `bytes` between two device buffers
+ * with a plain streaming kernel (k_copy) twice and reports the second copy's time -- the
+ * known-size reference for hardware byte counters. */
+int lw_hip_copy_probe(lw_hip_backend *b, size_t bytes, double *ms);
+
 /* Kernel-clock timing of the last render_hq / render_hq_scene on this handle. */
 int lw_hip_last_timing(const lw_hip_backend *b, lw_timing *out);
 

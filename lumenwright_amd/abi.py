@@ -54,7 +54,11 @@ PARAMS = np.dtype([(name, t, 4) for name, t in _PARAM_FIELDS])
 HIT = np.dtype([("t", "<f4"), ("quad", "<u4"), ("texel", "<u4"), ("s", "<f4"), ("tt", "<f4"),
                 ("cell", "<i4", 3), ("hit", "<u4"), ("pad", "<u4")])
 
+# lw_view: one view of a batch.
+VIEW = np.dtype([("params", PARAMS), ("spp", "<u4"), ("pad", "<u4", 3)])
+
 assert QUAD.itemsize == 80 and MODEL.itemsize == 16 and EMITTER.itemsize == 32
+assert VIEW.itemsize == 448
 assert PARAMS.itemsize == 432 and HIT.itemsize == 40
 
 

@@ -14,7 +14,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _native
-from .abi import HIT, PARAMS, BoundView, HqFrame, TimingC
+from .abi import HIT, PARAMS, VIEW, BoundView, HqFrame, TimingC
 
 LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
 
@@ -118,6 +118,29 @@ class HipBackend:
         self._check(_native.lib().lw_hip_render_hq_scene(self.handle, scene.handle, _addr(p), spp,
                                                          cb, None, _addr(out)))
         return out
+
+    def render_hq_batch(self, scene: HipScene, views,
+                        progress: Optional[Callable[[int, int], None]] = None) -> list:
+        """Renders [(params, spp), ...] of one resident scene as one batch
+        (lw_hip_render_hq_batch); one (w*h, 3) float32 image per view."""
+        arr = np.zeros(len(views), dtype=VIEW)
+        outs = []
+        for i, (params, spp) in enumerate(views):
+            arr[i]["params"] = params
+            arr[i]["spp"] = spp
+            w, h = int(params["frame"][0]), int(params["frame"][1])
+            outs.append(np.zeros((w * h, 3), dtype="<f4"))
+        ptrs = (ctypes.c_void_p * len(views))(*[o.ctypes.data for o in outs])
+        cb = _native.PROGRESS_FN((lambda d, t, u: progress(d, t)) if progress else 0)
+        self._check(_native.lib().lw_hip_render_hq_batch(self.handle, scene.handle, _addr(arr),
+                                                         len(views), cb, None, ptrs))
+        return outs
+
+    def copy_probe(self, nbytes: int) -> float:
+        """Times a plain device-to-device copy of nbytes (ms); see lw_hip_copy_probe."""
+        ms = ctypes.c_double()
+        self._check(_native.lib().lw_hip_copy_probe(self.handle, nbytes, ctypes.byref(ms)))
+        return ms.value
 
     def last_timing(self) -> dict:
         t = TimingC()

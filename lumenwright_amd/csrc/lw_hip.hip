@@ -42,7 +42,8 @@ __device__ __forceinline__ void load_lut(float *lut_s, const float *lut) {
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kBlock) k_render(const lw_params p, const DevScene s,
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_render(const lw_params p, const DevScene s,
                                                    uint32_t sample_start, float *__restrict__ rad) {
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
@@ -152,6 +153,13 @@ struct FrameWork {
     sample = sample_start + sl;
     return px < w && py < h;
   }
+  __device__ __forceinline__ uint32_t count() const { return total; }
+  __device__ __forceinline__ bool start(const Tracer &T, PathState &P, uint32_t item) const {
+    uint32_t px, py, sample;
+    if (!decode(item, px, py, sample)) return false;
+    T.start(P, T.p, px, py, sample);
+    return true;
+  }
   __device__ __forceinline__ void write(uint32_t item, V3 c) const {
     uint32_t px, py, sample;
     decode(item, px, py, sample);
@@ -176,12 +184,157 @@ struct ListWork {
     s = sample[item];
     return true;
   }
+  __device__ __forceinline__ uint32_t count() const { return total; }
+  __device__ __forceinline__ bool start(const Tracer &T, PathState &P, uint32_t item) const {
+    T.start(P, T.p, px[item], py[item], sample[item]);
+    return true;
+  }
   __device__ __forceinline__ void write(uint32_t item, V3 c) const {
     out[3 * (size_t)item] = c.x;
     out[3 * (size_t)item + 1] = c.y;
     out[3 * (size_t)item + 2] = c.z;
   }
 };
+
+// ---- batches: many views of one scene in one work pool ----
+//
+// A batch renders its views in passes; pass k renders samples [k m_v, (k + 1) m_v) of view
+// v (m_v = its samples per pass, clipped to its spp), so every pixel's samples still reach
+// its f64 sum in sample order. k_batch_plan, one workgroup, turns the pass index kept in
+// device memory into each view's sample range and the prefix sums of the pass's work items,
+// so a pass is the same four launches every time and replays as one captured hipGraph.
+constexpr uint32_t kMaxViews = 256;
+
+struct BatchDev {
+  ViewCam cam[kMaxViews];
+  uint32_t tiles_x[kMaxViews], tiles[kMaxViews], per_pass[kMaxViews];
+  uint64_t rad_base[kMaxViews];  // the view's staging, in floats from the buffer start
+  uint32_t pix_prefix[kMaxViews + 1];
+  uint32_t nviews, pass, total, npix_total;
+  // Written by k_batch_plan for the current pass.
+  uint32_t n[kMaxViews], start[kMaxViews], item_prefix[kMaxViews + 1];
+};
+
+// The largest v with prefix[v] <= x (prefix non-decreasing, prefix[0] = 0 <= x).
+__device__ __forceinline__ uint32_t find_view(const uint32_t *prefix, uint32_t nviews, uint32_t x) {
+  uint32_t lo = 0, hi = nviews;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct BatchWork {
+  const BatchDev *__restrict__ bd;
+  float *__restrict__ rad;
+  __device__ __forceinline__ uint32_t count() const { return bd->total; }
+  __device__ __forceinline__ bool decode(uint32_t item, uint32_t &v, uint32_t &px, uint32_t &py,
+                                         uint32_t &sl) const {
+    v = find_view(bd->item_prefix, bd->nviews, item);
+    const uint32_t local = item - bd->item_prefix[v];
+    const uint32_t t = local & 255u, rest = local >> 8;
+    const uint32_t tiles = bd->tiles[v], tx = bd->tiles_x[v];
+    const uint32_t tile = rest % tiles;
+    sl = rest / tiles;
+    const uint32_t wave = t >> 6, lane = t & 63u;
+    px = (tile % tx) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    py = (tile / tx) * 16u + (wave >> 1) * 8u + (lane >> 3);
+    return px < bd->cam[v].width && py < bd->cam[v].height;
+  }
+  __device__ __forceinline__ bool start(const Tracer &T, PathState &P, uint32_t item) const {
+    uint32_t v, px, py, sl;
+    if (!decode(item, v, px, py, sl)) return false;
+    const ViewCam cam = bd->cam[v];
+    T.start(P, cam, px, py, bd->start[v] + sl);
+    return true;
+  }
+  __device__ __forceinline__ void write(uint32_t item, V3 c) const {
+    uint32_t v, px, py, sl;
+    decode(item, v, px, py, sl);
+    const uint32_t w = bd->cam[v].width, h = bd->cam[v].height;
+    float *o = rad + bd->rad_base[v] + ((size_t)sl * h * w + (size_t)py * w + px) * 3;
+    o[0] = c.x;
+    o[1] = c.y;
+    o[2] = c.z;
+  }
+};
+
+// One workgroup of kMaxViews lanes: the pass's sample ranges and item prefix sums, then the
+// pass index moves on.
+__global__ void __launch_bounds__(kMaxViews) k_batch_plan(BatchDev *__restrict__ bd) {
+  __shared__ uint32_t scan[kMaxViews];
+  const uint32_t v = threadIdx.x;
+  const uint32_t nv = bd->nviews, pass = bd->pass;
+  uint32_t items = 0;
+  if (v < nv) {
+    const uint32_t spp = bd->cam[v].spp, m = bd->per_pass[v];
+    const uint64_t s0 = (uint64_t)pass * m;
+    const uint32_t n = s0 < spp ? (uint32_t)std::min<uint64_t>(m, spp - s0) : 0u;
+    bd->n[v] = n;
+    bd->start[v] = (uint32_t)std::min<uint64_t>(s0, spp);
+    items = bd->tiles[v] * 256u * n;
+  }
+  scan[v] = items;
+  __syncthreads();
+  for (uint32_t off = 1; off < kMaxViews; off <<= 1) {
+    const uint32_t add = v >= off ? scan[v - off] : 0u;
+    __syncthreads();
+    scan[v] += add;
+    __syncthreads();
+  }
+  if (v < nv) bd->item_prefix[v] = scan[v] - items;
+  if (v + 1 == nv) {
+    bd->item_prefix[nv] = scan[v];
+    bd->total = scan[v];
+  }
+  __syncthreads();
+  if (v == 0) bd->pass = pass + 1u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_accumulate_batch(const BatchDev *__restrict__ bd,
+                                                             const float *__restrict__ rad,
+                                                             double *__restrict__ acc) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= bd->npix_total) return;
+  const uint32_t v = find_view(bd->pix_prefix, bd->nviews, g);
+  const uint32_t i = g - bd->pix_prefix[v];
+  const size_t npix = (size_t)bd->cam[v].width * bd->cam[v].height;
+  const uint32_t n = bd->n[v];
+  double a0 = acc[3 * (size_t)g], a1 = acc[3 * (size_t)g + 1], a2 = acc[3 * (size_t)g + 2];
+  for (uint32_t sl = 0; sl < n; sl++) {
+    const float *c = rad + bd->rad_base[v] + ((size_t)sl * npix + i) * 3;
+    const float r = c[0], gg = c[1], bb = c[2];
+    if (isfinite(r) && isfinite(gg) && isfinite(bb)) {
+      a0 += (double)r;
+      a1 += (double)gg;
+      a2 += (double)bb;
+    }
+  }
+  acc[3 * (size_t)g] = a0;
+  acc[3 * (size_t)g + 1] = a1;
+  acc[3 * (size_t)g + 2] = a2;
+}
+
+__global__ void __launch_bounds__(kBlock) k_resolve_batch(const BatchDev *__restrict__ bd,
+                                                          const double *__restrict__ acc,
+                                                          float *__restrict__ out) {
+  const uint32_t g3 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g3 >= 3u * bd->npix_total) return;
+  const uint32_t v = find_view(bd->pix_prefix, bd->nviews, g3 / 3u);
+  out[g3] = (float)(acc[g3] / (double)bd->cam[v].spp);
+}
+
+// A plain streaming copy, 16 B per lane per step: the reference point that turns hardware
+// byte counters into bytes for the path kernels.
+__global__ void __launch_bounds__(kBlock) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                 size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
 
 __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *__restrict__ p, uint32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -213,12 +366,8 @@ __device__ __forceinline__ void refill(PathState &P, const Tracer &T, const Work
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
           (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
       const uint32_t item = base + rank;
-      if (item < work.total) {
-        uint32_t px, py, sample;
-        if (work.decode(item, px, py, sample)) {
-          P.item = item;
-          T.start(P, px, py, sample);
-        }
+      if (item < work.count()) {
+        if (work.start(T, P, item)) P.item = item;
       } else {
         P.stage = ST_DONE;
       }
@@ -226,8 +375,8 @@ __device__ __forceinline__ void refill(PathState &P, const Tracer &T, const Work
   }
 }
 
-template <class Work>
-__global__ void __launch_bounds__(kBlock) k_paths(const lw_params p, const DevScene s,
+template <class Work, int W>
+__global__ void __launch_bounds__(kBlock, W) k_paths(const lw_params p, const DevScene s,
                                                   const Work work,
                                                   uint32_t *__restrict__ counter) {
   __shared__ float lut_s[256];
@@ -277,8 +426,8 @@ __global__ void __launch_bounds__(kBlock) k_wf_init(WfSlots w) {
   w.st[i].stage = ST_IDLE;
 }
 
-template <class Work>
-__global__ void __launch_bounds__(kBlock) k_wf_logic(const lw_params p, const DevScene s,
+template <class Work, int W>
+__global__ void __launch_bounds__(kBlock, W) k_wf_logic(const lw_params p, const DevScene s,
                                                      const Work work, WfSlots w) {
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
@@ -318,7 +467,8 @@ __global__ void k_wf_flip(uint32_t *__restrict__ ctr) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_wf_trace(const lw_params p, const DevScene s,
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_wf_trace(const lw_params p, const DevScene s,
                                                      WfSlots w) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= w.ctr[1]) return;
@@ -443,6 +593,7 @@ struct lw_hip_backend {
   // LW_KERNEL: "persistent" (k_paths), "direct" (one thread per sample) or "wavefront"
   // (k_wf_*, LW_WF_SLOTS path slots).
   int mode = 0;
+  int waves = 1, trace_waves = 1;  // LW_WAVES, LW_TRACE_WAVES: register budgets (see dispatch)
   uint32_t wf_slots = 1u << 20;
   DevBuf<PathState> wf_st;
   DevBuf<float4> wf_o, wf_d;
@@ -450,6 +601,12 @@ struct lw_hip_backend {
   DevBuf<uint32_t> wf_queue, wf_ctr;
   uint32_t *pinned = nullptr;
   uint64_t last_iterations = 0;
+  // Batches: the staging cap, a minimum pass count (0: as few as fit) and whether a pass
+  // replays as a captured hipGraph (LW_BATCH_GRAPHS=0 launches it kernel by kernel).
+  uint64_t batch_staging = 2ull << 30;
+  uint32_t batch_passes = 0;
+  bool use_graphs = true;
+  DevBuf<BatchDev> batch_dev;
 };
 
 namespace {
@@ -641,6 +798,18 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.max_emitter_rays = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(rays, 1), 1u << 20);
 }
 
+
+// Launches with the register budget b->waves selects (LW_WAVES: the minimum waves per SIMD
+// the compiler is held to; 1 leaves it free).
+#define LW_WAVES_DISPATCH(waves, KERNEL_EXPR)  \
+  do {                                         \
+    switch (waves) {                           \
+      case 3: { constexpr int W = 3; KERNEL_EXPR; } break; \
+      case 4: { constexpr int W = 4; KERNEL_EXPR; } break; \
+      default: { constexpr int W = 1; KERNEL_EXPR; } break; \
+    }                                          \
+  } while (0)
+
 enum { MODE_PERSISTENT = 0, MODE_DIRECT = 1, MODE_WAVEFRONT = 2 };
 
 // The wavefront loop (k_wf_*) over `work`: iterations until no slot has a ray. Bounded: a
@@ -648,9 +817,10 @@ enum { MODE_PERSISTENT = 0, MODE_DIRECT = 1, MODE_WAVEFRONT = 2 };
 // segments and the scene's emitter rays, and each slot runs at most ceil(total / n) + 1
 // paths; past that bound the loop stops with an error instead of spinning.
 template <class Work>
-void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, const Work &work) {
+void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, const Work &work,
+                   uint32_t total) {
   const uint32_t n = std::max<uint32_t>(
-      kBlock, std::min<uint32_t>(b->wf_slots, (work.total + kBlock - 1) / kBlock * kBlock));
+      kBlock, std::min<uint32_t>(b->wf_slots, (uint32_t)(((uint64_t)total + kBlock - 1) / kBlock * kBlock)));
   b->wf_st.ensure(n);
   b->wf_o.ensure(n);
   b->wf_d.ensure(n);
@@ -664,13 +834,16 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   k_wf_init<<<grid, block, 0, b->stream>>>(w);
   HIP_TRY(hipGetLastError());
   const uint64_t per_path = (uint64_t)MAX_EVENTS * (1 + 8 + sc.max_emitter_rays);
-  const uint64_t limit = ((uint64_t)work.total / n + 2) * per_path + 64;
+  const uint64_t limit = ((uint64_t)total / n + 2) * per_path + 64;
   uint64_t it = 0;
   for (;; it++) {
     if (it > limit) throw InvalidScene{"wavefront loop exceeded its iteration bound"};
-    k_wf_logic<Work><<<grid, block, 0, b->stream>>>(p, ds, work, w);
+    LW_WAVES_DISPATCH(b->waves, (k_wf_logic<Work, W><<<grid, block, 0, b->stream>>>(p, ds, work, w)));
     k_wf_flip<<<dim3(1), dim3(64), 0, b->stream>>>(w.ctr);
-    k_wf_trace<<<grid, block, 0, b->stream>>>(p, ds, w);
+    if (b->trace_waves >= 8)
+      k_wf_trace<8><<<grid, block, 0, b->stream>>>(p, ds, w);
+    else
+      k_wf_trace<1><<<grid, block, 0, b->stream>>>(p, ds, w);
     HIP_TRY(hipGetLastError());
     if ((it & 7) == 7) {
       HIP_TRY(hipMemcpyAsync(b->pinned, w.ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
@@ -721,15 +894,17 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     const auto start = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (b->mode == MODE_DIRECT) {
-      k_render<<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(p, ds, done, b->rad.ptr);
+      LW_WAVES_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
+                                      p, ds, done, b->rad.ptr)));
     } else if (b->mode == MODE_WAVEFRONT) {
       const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
-      run_wavefront(b, sc, p, fw);
+      run_wavefront(b, sc, p, fw, fw.total);
     } else {
       const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
       const uint32_t grid = std::min<uint32_t>(b->blocks_frame, (fw.total + kBlock - 1) / kBlock);
       k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
-      k_paths<FrameWork><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(p, ds, fw, b->counter.ptr);
+      LW_WAVES_DISPATCH(b->waves, (k_paths<FrameWork, W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+                                      p, ds, fw, b->counter.ptr)));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
@@ -763,6 +938,168 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
                          b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   tm.samples = npix * spp;
+  tm.wall_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+  b->timing = tm;
+}
+
+// The views of a batch share every Params field but the camera, the frame size and the
+// seed (the kernels read those per view from BatchDev, the rest from the first view).
+void check_batch_view(const lw_params &a, const lw_params &v) {
+  lw_params x = v;
+  std::memcpy(x.cam_pos, a.cam_pos, sizeof(x.cam_pos));
+  std::memcpy(x.cam_fwd, a.cam_fwd, sizeof(x.cam_fwd));
+  std::memcpy(x.cam_right, a.cam_right, sizeof(x.cam_right));
+  std::memcpy(x.cam_up, a.cam_up, sizeof(x.cam_up));
+  x.frame[0] = a.frame[0];
+  x.frame[1] = a.frame[1];
+  x.frame[2] = a.frame[2];
+  x.misc_u[0] = a.misc_u[0];
+  if (std::memcmp(&x, &a, sizeof(x)) != 0)
+    throw InvalidScene{"batch views differ in more than camera, frame size and seed"};
+}
+
+void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *views, size_t nv,
+                  lw_progress_fn progress, void *user, float *const *outs) {
+  auto t_wall = std::chrono::steady_clock::now();
+  if (nv == 0 || nv > kMaxViews) throw InvalidScene{"a batch holds 1 to 256 views"};
+  const lw_params &p0 = views[0].params;
+  if (!(layout_key(p0) == sc.key))
+    throw InvalidScene{"params do not match the layout the scene was validated with"};
+  std::vector<BatchDev> host(1);
+  BatchDev &bd = host[0];
+  std::memset(&bd, 0, sizeof(bd));
+  bd.nviews = (uint32_t)nv;
+  uint64_t npix_total = 0, samples = 0, max_spp = 0, staging_all = 0;
+  for (size_t v = 0; v < nv; v++) {
+    const lw_params &p = views[v].params;
+    check_batch_view(p0, p);
+    if (!outs[v]) throw InvalidScene{"null output for a view"};
+    const uint32_t w = p.frame[0], h = p.frame[1], spp = std::max(views[v].spp, 1u);
+    if (w == 0 || h == 0 || (uint64_t)w * h > (1u << 26)) throw InvalidScene{"bad view size"};
+    ViewCam &c = bd.cam[v];
+    std::memcpy(c.cam_pos, p.cam_pos, sizeof(c.cam_pos));
+    std::memcpy(c.cam_fwd, p.cam_fwd, sizeof(c.cam_fwd));
+    std::memcpy(c.cam_right, p.cam_right, sizeof(c.cam_right));
+    std::memcpy(c.cam_up, p.cam_up, sizeof(c.cam_up));
+    c.width = w;
+    c.height = h;
+    c.seed = p.misc_u[0];
+    c.spp = spp;
+    bd.tiles_x[v] = (w + 15) / 16;
+    bd.tiles[v] = bd.tiles_x[v] * ((h + 15) / 16);
+    bd.pix_prefix[v] = (uint32_t)npix_total;
+    npix_total += (uint64_t)w * h;
+    samples += (uint64_t)w * h * spp;
+    staging_all += (uint64_t)w * h * spp * 3 * sizeof(float);
+    max_spp = std::max<uint64_t>(max_spp, spp);
+  }
+  if (npix_total * 3 > 0xFFFFFFFFull) throw InvalidScene{"batch too large"};
+  bd.pix_prefix[nv] = (uint32_t)npix_total;
+  bd.npix_total = (uint32_t)npix_total;
+  // Passes: as few as fit the staging cap and keep a pass's items below 2^31.
+  uint64_t passes = std::max<uint64_t>(1, (staging_all + b->batch_staging - 1) / b->batch_staging);
+  if (b->batch_passes) passes = std::max<uint64_t>(passes, b->batch_passes);
+  passes = std::min<uint64_t>(passes, max_spp);
+  uint64_t staging = 0, items = 0;
+  for (;;) {
+    staging = 0;
+    items = 0;
+    for (size_t v = 0; v < nv; v++) {
+      const uint32_t m = (uint32_t)((bd.cam[v].spp + passes - 1) / passes);
+      bd.per_pass[v] = m;
+      bd.rad_base[v] = staging / sizeof(float);
+      staging += (uint64_t)bd.cam[v].width * bd.cam[v].height * m * 3 * sizeof(float);
+      items += (uint64_t)bd.tiles[v] * 256 * m;
+    }
+    if ((staging <= b->batch_staging && items < (1ull << 31)) || passes >= max_spp) break;
+    passes++;
+  }
+  uint32_t npass = 0;
+  for (size_t v = 0; v < nv; v++)
+    npass = std::max<uint32_t>(npass, (bd.cam[v].spp + bd.per_pass[v] - 1) / bd.per_pass[v]);
+  HIP_TRY(hipSetDevice(b->device));
+  const DevScene ds = sc.dev();
+  b->batch_dev.ensure(1);
+  b->rad.ensure(staging / sizeof(float) + 1);
+  b->acc.ensure(npix_total * 3);
+  b->out.ensure(npix_total * 3);
+  HIP_TRY(hipMemcpyAsync(b->batch_dev.ptr, &bd, sizeof(BatchDev), hipMemcpyHostToDevice, b->stream));
+  k_zero<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+      b->acc.ptr, npix_total * 3);
+  HIP_TRY(hipGetLastError());
+  const BatchWork work{b->batch_dev.ptr, b->rad.ptr};
+  const dim3 acc_grid((unsigned)((npix_total + kBlock - 1) / kBlock));
+  auto pass_ops = [&](hipStream_t st) {
+    k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, st>>>(b->batch_dev.ptr);
+    k_fill_u32<<<dim3(1), dim3(64), 0, st>>>(b->counter.ptr, 0u);
+    LW_WAVES_DISPATCH(b->waves, (k_paths<BatchWork, W><<<dim3(b->blocks_frame), dim3(kBlock), 0, st>>>(
+                                    p0, ds, work, b->counter.ptr)));
+    k_accumulate_batch<<<acc_grid, dim3(kBlock), 0, st>>>(b->batch_dev.ptr, b->rad.ptr, b->acc.ptr);
+  };
+  lw_timing tm{};
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  if (b->mode == MODE_WAVEFRONT) {
+    // The wavefront loop's length depends on the pass, so each pass is launched from here.
+    for (uint32_t k = 0; k < npass; k++) {
+      k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, b->stream>>>(b->batch_dev.ptr);
+      run_wavefront(b, sc, p0, work, (uint32_t)items);
+      k_accumulate_batch<<<acc_grid, dim3(kBlock), 0, b->stream>>>(b->batch_dev.ptr, b->rad.ptr,
+                                                                    b->acc.ptr);
+      HIP_TRY(hipGetLastError());
+      if (progress) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        progress(k + 1, npass, user);
+      }
+    }
+  } else if (b->use_graphs) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    pass_ops(b->stream);
+    HIP_TRY(hipStreamEndCapture(b->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      throw HipError{e, "hipGraphInstantiate"};
+    }
+    for (uint32_t k = 0; k < npass; k++) {
+      e = hipGraphLaunch(exec, b->stream);
+      if (e == hipSuccess && progress) {
+        e = hipStreamSynchronize(b->stream);
+        if (e == hipSuccess) progress(k + 1, npass, user);
+      }
+      if (e != hipSuccess) break;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw HipError{e, "hipGraphLaunch"};
+  } else {
+    for (uint32_t k = 0; k < npass; k++) {
+      pass_ops(b->stream);
+      HIP_TRY(hipGetLastError());
+      if (progress) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        progress(k + 1, npass, user);
+      }
+    }
+  }
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  k_resolve_batch<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                    b->stream>>>(b->batch_dev.ptr, b->acc.ptr, b->out.ptr);
+  HIP_TRY(hipGetLastError());
+  for (size_t v = 0; v < nv; v++) {
+    const size_t n3 = (size_t)bd.cam[v].width * bd.cam[v].height * 3;
+    HIP_TRY(hipMemcpyAsync(outs[v], b->out.ptr + (size_t)bd.pix_prefix[v] * 3, n3 * sizeof(float),
+                           hipMemcpyDeviceToHost, b->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  tm.trace_ms = ms;
+  tm.launches = npass;
+  tm.samples = samples;
   tm.wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
   b->timing = tm;
@@ -846,11 +1183,15 @@ int lw_hip_open(int device, lw_hip_backend **out) {
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto &ev : b->ev) HIP_TRY(hipEventCreate(&ev));
     b->counter.alloc(1);
+    if (const char *e = std::getenv("LW_WAVES")) b->waves = std::atoi(e);
+    if (const char *e = std::getenv("LW_TRACE_WAVES")) b->trace_waves = std::atoi(e);
     int per_cu_frame = 0, per_cu_list = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu_frame, reinterpret_cast<const void *>(&k_paths<FrameWork>), kBlock, 0));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu_list, reinterpret_cast<const void *>(&k_paths<ListWork>), kBlock, 0));
+    LW_WAVES_DISPATCH(b->waves, {
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu_frame, reinterpret_cast<const void *>(&k_paths<FrameWork, W>), kBlock, 0));
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu_list, reinterpret_cast<const void *>(&k_paths<ListWork, W>), kBlock, 0));
+    });
     const uint32_t cus = (uint32_t)std::max(1, prop.multiProcessorCount);
     b->blocks_frame = cus * (uint32_t)std::max(1, per_cu_frame);
     b->blocks_list = cus * (uint32_t)std::max(1, per_cu_list);
@@ -867,6 +1208,8 @@ int lw_hip_open(int device, lw_hip_backend **out) {
       if (v >= kBlock) b->wf_slots = (uint32_t)std::min<long>(v, 1l << 24) / kBlock * kBlock;
     }
     HIP_TRY(hipHostMalloc((void **)&b->pinned, 64, hipHostMallocDefault));
+    if (const char *e = std::getenv("LW_BATCH_GRAPHS")) b->use_graphs = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("LW_BATCH_PASSES")) b->batch_passes = (uint32_t)std::max(0, std::atoi(e));
   });
   if (rc != LW_OK) {
     lw_hip_close(b);
@@ -897,6 +1240,7 @@ void lw_hip_close(lw_hip_backend *b) {
   b->wf_hit.free();
   b->wf_queue.free();
   b->wf_ctr.free();
+  b->batch_dev.free();
   if (b->pinned) (void)hipHostFree(b->pinned);
   for (auto &ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -935,6 +1279,32 @@ int lw_hip_render_hq(lw_hip_backend *b, const lw_scene_view *view, uint32_t spp,
     lw_hip_scene sc;
     upload_scene(b, *view, sc);
     render_frame(b, sc, *view->params, spp, progress, user, out_rgb);
+  });
+}
+
+int lw_hip_render_hq_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
+                           size_t n, lw_progress_fn progress, void *user, float *const *out_rgb) {
+  if (!b || !scene || !views || !out_rgb) return LW_EINVAL;
+  return guarded(b, [&] { render_batch(b, *scene, views, n, progress, user, out_rgb); });
+}
+
+int lw_hip_copy_probe(lw_hip_backend *b, size_t bytes, double *ms) {
+  if (!b || !ms || bytes < 16) return LW_EINVAL;
+  return guarded(b, [&] {
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t n = bytes / 16;
+    DevBuf<uint4> src, dst;
+    src.alloc(n);
+    dst.alloc(n);
+    k_copy<<<dim3(2048), dim3(kBlock), 0, b->stream>>>(src.ptr, dst.ptr, n);  // warm
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    k_copy<<<dim3(2048), dim3(kBlock), 0, b->stream>>>(src.ptr, dst.ptr, n);
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+    *ms = t;
   });
 }
 
@@ -984,7 +1354,7 @@ int lw_hip_render_samples(lw_hip_backend *b, const lw_scene_view *view, size_t n
     d_out.alloc(3 * n);
     if (b->mode == MODE_WAVEFRONT) {
       const ListWork lw{d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr, (uint32_t)n};
-      run_wavefront(b, sc, *view->params, lw);
+      run_wavefront(b, sc, *view->params, lw, lw.total);
     } else if (b->mode == MODE_DIRECT) {
       k_render_samples<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
           *view->params, sc.dev(), (uint32_t)n, d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr);
@@ -992,8 +1362,8 @@ int lw_hip_render_samples(lw_hip_backend *b, const lw_scene_view *view, size_t n
       const ListWork lw{d_px.ptr, d_py.ptr, d_s.ptr, d_out.ptr, (uint32_t)n};
       const uint32_t grid = std::min<uint32_t>(b->blocks_list, (uint32_t)((n + kBlock - 1) / kBlock));
       k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
-      k_paths<ListWork><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(*view->params, sc.dev(), lw,
-                                                                     b->counter.ptr);
+      LW_WAVES_DISPATCH(b->waves, (k_paths<ListWork, W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+                                      *view->params, sc.dev(), lw, b->counter.ptr)));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out_rgb, d_out.ptr, 3 * n * sizeof(float), hipMemcpyDeviceToHost,

@@ -41,6 +41,19 @@ struct PathState {
   float em_scale, em_dist;
 };
 
+// The camera of one view of a batch: what render_sample reads from Params to make its
+// camera ray (integrator.rs:541-556), and the view's sample count.
+struct ViewCam {
+  float cam_pos[4], cam_fwd[4], cam_right[4], cam_up[4];
+  uint32_t width, height, seed, spp;
+};
+__device__ __forceinline__ uint32_t cam_width(const lw_params &c) { return c.frame[0]; }
+__device__ __forceinline__ uint32_t cam_height(const lw_params &c) { return c.frame[1]; }
+__device__ __forceinline__ uint32_t cam_seed(const lw_params &c) { return c.misc_u[0]; }
+__device__ __forceinline__ uint32_t cam_width(const ViewCam &c) { return c.width; }
+__device__ __forceinline__ uint32_t cam_height(const ViewCam &c) { return c.height; }
+__device__ __forceinline__ uint32_t cam_seed(const ViewCam &c) { return c.seed; }
+
 struct Tracer {
   const DevScene &s;
   const lw_params &p;
@@ -51,20 +64,21 @@ struct Tracer {
   }
 
   // The camera ray of (px, py, sample) (integrator.rs:540-569).
-  __device__ __forceinline__ void start(PathState &P, uint32_t px, uint32_t py,
+  template <class Cam>
+  __device__ __forceinline__ void start(PathState &P, const Cam &c, uint32_t px, uint32_t py,
                                         uint32_t sample) const {
-    const uint32_t width = p.frame[0], height = p.frame[1];
-    P.rng = Rng::make(py * width + px, sample, p.misc_u[0]);
-    const float tan_half = p.cam_pos[3];
-    const float aspect = p.cam_fwd[3];
+    const uint32_t width = cam_width(c), height = cam_height(c);
+    P.rng = Rng::make(py * width + px, sample, cam_seed(c));
+    const float tan_half = c.cam_pos[3];
+    const float aspect = c.cam_fwd[3];
     const float jx = P.rng.next();
     const float jy = P.rng.next();
     const float sx = (2.0f * ((float)px + jx) / (float)width - 1.0f) * aspect * tan_half;
     const float sy = (1.0f - 2.0f * ((float)py + jy) / (float)height) * tan_half;
-    P.rd = normalize(V3{p.cam_fwd[0], p.cam_fwd[1], p.cam_fwd[2]} +
-                     V3{p.cam_right[0], p.cam_right[1], p.cam_right[2]} * sx +
-                     V3{p.cam_up[0], p.cam_up[1], p.cam_up[2]} * sy);
-    P.ro = V3{p.cam_pos[0], p.cam_pos[1], p.cam_pos[2]};
+    P.rd = normalize(V3{c.cam_fwd[0], c.cam_fwd[1], c.cam_fwd[2]} +
+                     V3{c.cam_right[0], c.cam_right[1], c.cam_right[2]} * sx +
+                     V3{c.cam_up[0], c.cam_up[1], c.cam_up[2]} * sy);
+    P.ro = V3{c.cam_pos[0], c.cam_pos[1], c.cam_pos[2]};
     P.radiance = V3{0.0f, 0.0f, 0.0f};
     P.beta = V3{1.0f, 1.0f, 1.0f};
     P.bounces = 0;

@@ -329,3 +329,41 @@ def make_frame(cfg: FrameConfig, scene: Optional[HqScene] = None) -> HqFrame:
     else:
         raise ValueError(cfg.sky)
     return HqFrame(scene=scene, params=params, spp=cfg.spp, sky=sky_buf)
+
+
+def batch_views(cfg: FrameConfig, scene: HqScene, n_views: int = 64, seed: int = 3,
+                sizes=((160, 90), (320, 180), (640, 360), (1280, 720)),
+                spps=(16, 64, 256, 1024), max_bounces: int = 8):
+    """The batched workload over a world: `n_views` seeded cameras, each at a point 10 to 40
+    blocks above the terrain looking at a seeded surface point, with a resolution and spp
+    drawn uniformly from `sizes` and `spps`; per-view seed = view index. Returns
+    [(params, spp)] sharing the frame's sky; every Params field but the camera, the size
+    and the seed is the base frame's."""
+    import math
+    base = make_frame(replace(cfg, max_bounces=max_bounces), scene)
+    rng = np.random.default_rng(seed)
+    gx, gz = cfg.world.chunks_x * 16, cfg.world.chunks_z * 16
+    views = []
+    for v in range(n_views):
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        spp = int(spps[int(rng.integers(len(spps)))])
+        ex, ez = rng.uniform(16, gx - 16), rng.uniform(16, gz - 16)
+        ey = world_height(cfg.world, int(ex), int(ez)) + rng.uniform(10, 40)
+        tx, tz = rng.uniform(16, gx - 16), rng.uniform(16, gz - 16)
+        ty = world_height(cfg.world, int(tx), int(tz)) + 0.5
+        dx, dy, dz = tx - ex, ty - ey, tz - ez
+        norm = math.sqrt(dx * dx + dy * dy + dz * dz)
+        dx, dy, dz = dx / norm, dy / norm, dz / norm
+        pitch = math.degrees(math.asin(-dy))          # camera.rs:40-45 (look_at)
+        yaw = math.degrees(math.atan2(-dx, dz))
+        rays = CameraRaysC()
+        _native.lib().lw_camera_rays_new(yaw, pitch, cfg.fov_y, w, h, ctypes.byref(rays))
+        p = base.params.copy()
+        p["cam_pos"] = [ex, ey, ez, rays.tan_half_fov]
+        p["cam_fwd"] = list(rays.forward) + [rays.aspect]
+        p["cam_right"] = list(rays.right) + [0.0]
+        p["cam_up"] = list(rays.up) + [0.0]
+        p["frame"][0], p["frame"][1] = w, h
+        p["misc_u"][0] = v
+        views.append((p, spp))
+    return base, views
