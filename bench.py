@@ -91,8 +91,11 @@ def roofline(config, samples, trace_ms, launches):
 
 
 def kernel_form():
-    return os.environ.get("LW_KERNEL", "persistent") + (
-        f", waves {os.environ['LW_WAVES']}" if "LW_WAVES" in os.environ else "")
+    form = os.environ.get("LW_KERNEL", "direct")
+    form += f", waves {os.environ.get('LW_WAVES', '5')}"
+    if os.environ.get("LW_XCD_REMAP", "0") != "0":
+        form += ", xcd remap"
+    return form
 
 
 def bench_plain(args, lw, frames):

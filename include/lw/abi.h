@@ -242,6 +242,58 @@ typedef struct lw_view {
 int lw_hip_render_hq_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
                            size_t n, lw_progress_fn progress, void *user, float *const *out_rgb);
 
+/* ---- differentiable rendering (no counterpart in the reference, which renders with
+ * fixed materials and sky) ---- */
+
+#define LW_NO_GROUP 0xFFFFFFFFu
+
+/* The parameters of a differentiable render and their derivative inputs. At palette and
+ * intensity factors of 1 and sky parameters as the Params hold them, render_diff gives
+ * render_hq's image.
+ *   palette:   the albedo of a hit on quad q (integrator.rs:633) is multiplied by the
+ *              RGB factor palette[3 g..3 g+2], g = quad_palette[q]; the same for the colour
+ *              emitter sampling meets (integrator.rs:411-412). Translucent quads take none.
+ *   intensity: the emittance of quad q times params.misc_f.x (integrator.rs:661,413) is
+ *              multiplied by intensity[quad_intensity[q]].
+ *   sky:       with sky_params, the gradient also covers the Preetham turbidity and the
+ *              sun's altitude and azimuth (radians): d_sky_cache holds three tables shaped
+ *              like the sky cache (d cache / d turbidity, altitude, azimuth), and d_sun_dir,
+ *              d_sun_color, d_sky_light the derivatives of params.sun_dir.xyz,
+ *              params.sun_color.xyz and params.sky_sun.w (rows: turbidity, altitude,
+ *              azimuth). The Preetham sky only (no sky maps), no sun sprite.
+ * Factors must be finite and > 0. Emitter sampling strategies NONE and ONE. rr_off = 1
+ * turns Russian roulette off (integrator.rs:696-702), which makes the render a smooth
+ * function of every parameter but the sun's, for finite-difference checks. */
+typedef struct lw_diff_desc {
+  uint32_t n_palette;
+  uint32_t n_intensity;
+  const uint32_t *quad_palette;   /* quads_len entries: a palette group or LW_NO_GROUP */
+  const uint32_t *quad_intensity; /* quads_len entries: an intensity group or LW_NO_GROUP */
+  const float *palette;           /* 3 * n_palette */
+  const float *intensity;         /* n_intensity */
+  uint32_t sky_params;
+  uint32_t rr_off;
+  const float *d_sky_cache;       /* sky_params: 3 * sky_len floats */
+  float d_sun_dir[3][3];
+  float d_sun_color[3][3];
+  float d_sky_light[3];
+  uint32_t pad;
+} lw_diff_desc;
+
+/* The image of params under diff's parameters: w*h*3 floats of mean radiance, as
+ * lw_hip_render_hq_scene. */
+int lw_hip_render_diff(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                       uint32_t spp, const lw_diff_desc *diff, float *out_rgb);
+
+/* The adjoint of lw_hip_render_diff with the same arguments: replays every sample from its
+ * seed and writes grads[i] = sum over pixels and channels of d_loss_d_image * dI/dtheta_i
+ * (w*h*3 floats in, 3 n_palette + n_intensity + 3 doubles out: palette RGB factors,
+ * intensities, then turbidity, sun altitude, sun azimuth, the last three 0 without
+ * sky_params). params.frame[3] (max bounces) must be at most 16. */
+int lw_hip_adjoint(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                   uint32_t spp, const lw_diff_desc *diff, const float *d_loss_d_image,
+                   double *grads);
+
 /* Diagnostics (no counterpart in the reference): copies `bytes` between two device buffers
  * with a plain streaming kernel (k_copy) twice and reports the second copy's time -- the
  * known-size reference for hardware byte counters. */

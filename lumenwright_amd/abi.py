@@ -92,6 +92,25 @@ class SceneViewC(ctypes.Structure):
     ]
 
 
+NO_GROUP = 0xFFFFFFFF
+
+
+class DiffDescC(ctypes.Structure):
+    """lw_diff_desc: the parameters of a differentiable render."""
+
+    _fields_ = [
+        ("n_palette", ctypes.c_uint32), ("n_intensity", ctypes.c_uint32),
+        ("quad_palette", ctypes.c_void_p), ("quad_intensity", ctypes.c_void_p),
+        ("palette", ctypes.c_void_p), ("intensity", ctypes.c_void_p),
+        ("sky_params", ctypes.c_uint32), ("rr_off", ctypes.c_uint32),
+        ("d_sky_cache", ctypes.c_void_p),
+        ("d_sun_dir", (ctypes.c_float * 3) * 3),
+        ("d_sun_color", (ctypes.c_float * 3) * 3),
+        ("d_sky_light", ctypes.c_float * 3),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
 class TimingC(ctypes.Structure):
     """lw_timing."""
 

@@ -631,25 +631,41 @@ __device__ __forceinline__ bool under_water_plane(const lw_params &p, V3 o, V3 d
   return p.water_plane[1] != 0.0f && o.y < p.water_plane[0] && d.y <= 0.0f;
 }
 
+// The camera of one view of a batch: what render_sample reads from Params to make its
+// camera ray (integrator.rs:541-556), and the view's sample count.
+struct ViewCam {
+  float cam_pos[4], cam_fwd[4], cam_right[4], cam_up[4];
+  uint32_t width, height, seed, spp;
+};
+__device__ __forceinline__ uint32_t cam_width(const lw_params &c) { return c.frame[0]; }
+__device__ __forceinline__ uint32_t cam_height(const lw_params &c) { return c.frame[1]; }
+__device__ __forceinline__ uint32_t cam_seed(const lw_params &c) { return c.misc_u[0]; }
+__device__ __forceinline__ uint32_t cam_width(const ViewCam &c) { return c.width; }
+__device__ __forceinline__ uint32_t cam_height(const ViewCam &c) { return c.height; }
+__device__ __forceinline__ uint32_t cam_seed(const ViewCam &c) { return c.seed; }
+
 // integrator.rs:539-709: the radiance of sample `sample` of pixel (px, py).
-__device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &p, const float *lut,
-                                            uint32_t px, uint32_t py, uint32_t sample) {
-  const uint32_t width = p.frame[0];
-  const uint32_t height = p.frame[1];
+// The camera ray comes from `c` (the Params themselves, or one view of a batch).
+template <class Cam>
+__device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &p, const Cam &c,
+                                            const float *lut, uint32_t px, uint32_t py,
+                                            uint32_t sample) {
+  const uint32_t width = cam_width(c);
+  const uint32_t height = cam_height(c);
   const uint32_t max_bounces = p.frame[3];
   const float eps = p.misc_f[1];
-  Rng rng = Rng::make(py * width + px, sample, p.misc_u[0]);
+  Rng rng = Rng::make(py * width + px, sample, cam_seed(c));
 
-  const float tan_half = p.cam_pos[3];
-  const float aspect = p.cam_fwd[3];
+  const float tan_half = c.cam_pos[3];
+  const float aspect = c.cam_fwd[3];
   const float jx = rng.next();
   const float jy = rng.next();
   const float sx = (2.0f * ((float)px + jx) / (float)width - 1.0f) * aspect * tan_half;
   const float sy = (1.0f - 2.0f * ((float)py + jy) / (float)height) * tan_half;
-  V3 d = normalize(V3{p.cam_fwd[0], p.cam_fwd[1], p.cam_fwd[2]} +
-                   V3{p.cam_right[0], p.cam_right[1], p.cam_right[2]} * sx +
-                   V3{p.cam_up[0], p.cam_up[1], p.cam_up[2]} * sy);
-  V3 o = V3{p.cam_pos[0], p.cam_pos[1], p.cam_pos[2]};
+  V3 d = normalize(V3{c.cam_fwd[0], c.cam_fwd[1], c.cam_fwd[2]} +
+                   V3{c.cam_right[0], c.cam_right[1], c.cam_right[2]} * sx +
+                   V3{c.cam_up[0], c.cam_up[1], c.cam_up[2]} * sy);
+  V3 o = V3{c.cam_pos[0], c.cam_pos[1], c.cam_pos[2]};
 
   V3 radiance{0.0f, 0.0f, 0.0f};
   V3 beta{1.0f, 1.0f, 1.0f};
@@ -761,6 +777,11 @@ __device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &
     specular = false;
   }
   return radiance;
+}
+
+__device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &p, const float *lut,
+                                            uint32_t px, uint32_t py, uint32_t sample) {
+  return render_sample(s, p, p, lut, px, py, sample);
 }
 
 }  // namespace lw

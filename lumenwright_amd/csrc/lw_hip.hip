@@ -27,6 +27,7 @@
 #include "lw_device.hpp"
 #include "lw_host.hpp"
 #include "lw_path.hpp"
+#include "lw_diff.hpp"
 
 using namespace lw;
 
@@ -42,14 +43,23 @@ __device__ __forceinline__ void load_lut(float *lut_s, const float *lut) {
   __syncthreads();
 }
 
+// Workgroups go to the 8 XCDs in turn (workgroup b to XCD b mod 8), each XCD with its own
+// L2. xcd_tile renumbers them so that each XCD takes one contiguous run of the n tiles
+// (neighbouring tiles trace through the same bricks): a bijection on [0, n).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t q = n >> 3, r = n & 7u, xcd = b & 7u, idx = b >> 3;
+  return (xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q) + idx;
+}
+
 template <int W>
 __global__ void __launch_bounds__(kBlock, W) k_render(const lw_params p, const DevScene s,
-                                                   uint32_t sample_start, float *__restrict__ rad) {
+                                                   uint32_t sample_start, uint32_t xcd_remap,
+                                                   float *__restrict__ rad) {
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
   const uint32_t w = p.frame[0], h = p.frame[1];
   const uint32_t tiles_x = (w + 15u) / 16u;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_remap ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
   const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
@@ -328,6 +338,35 @@ __global__ void __launch_bounds__(kBlock) k_resolve_batch(const BatchDev *__rest
   out[g3] = (float)(acc[g3] / (double)bd->cam[v].spp);
 }
 
+// The direct form of a batch pass: one workgroup per 256 work items (a 16 x 16 tile of one
+// view at one sample), one lane per sample. Every view's items are a multiple of 256, so a
+// workgroup lies in one view; workgroups past the pass's items return at once (the grid is
+// sized for the first pass, the largest).
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, const DevScene s,
+                                                         const BatchDev *__restrict__ bd,
+                                                         float *__restrict__ rad) {
+  const uint32_t base = blockIdx.x * (uint32_t)kBlock;
+  if (base >= bd->total) return;
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t v = find_view(bd->item_prefix, bd->nviews, base);
+  const uint32_t rest = (base - bd->item_prefix[v]) >> 8;
+  const uint32_t tiles = bd->tiles[v], tx = bd->tiles_x[v];
+  const uint32_t tile = rest % tiles, sl = rest / tiles;
+  const ViewCam cam = bd->cam[v];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tx) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tx) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  if (px >= cam.width || py >= cam.height) return;
+  const V3 c = render_sample(s, p, cam, lut_s, px, py, bd->start[v] + sl);
+  float *o = rad + bd->rad_base[v] +
+             ((size_t)sl * cam.height * cam.width + (size_t)py * cam.width + px) * 3;
+  o[0] = c.x;
+  o[1] = c.y;
+  o[2] = c.z;
+}
+
 // A plain streaming copy, 16 B per lane per step: the reference point that turns hardware
 // byte counters into bytes for the path kernels.
 __global__ void __launch_bounds__(kBlock) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
@@ -479,6 +518,68 @@ __global__ void __launch_bounds__(kBlock, W) k_wf_trace(const lw_params p, const
   w.hit[i] = make_uint4(f_bits(h.t), h.quad, h.texel, hit ? 1u : 0u);
 }
 
+// ---- differentiable rendering (lw_diff.hpp) ----
+
+// The forward under the parameters: k_render with render_sample_diff.
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_render_diff(const lw_params p, const DevScene s,
+                                                        const DiffDev D, uint32_t sample_start,
+                                                        float *__restrict__ rad) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint32_t tiles_x = (w + 15u) / 16u;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  if (px >= w || py >= h) return;
+  const uint32_t sl = blockIdx.y;
+  const GradSink G{nullptr, 0u, 0u};
+  const SuffixList sfx{nullptr, 0u};
+  const V3 c = render_sample_diff<false>(s, p, p, lut_s, D, px, py, sample_start + sl, G, sfx,
+                                         V3{0.0f, 0.0f, 0.0f});
+  float *o = rad + ((size_t)sl * h * w + (size_t)py * w + px) * 3;
+  o[0] = c.x;
+  o[1] = c.y;
+  o[2] = c.z;
+}
+
+// The adjoint: each lane replays one sample with dl = dL/dI(pixel) / spp; the workgroup
+// sums its gradients in LDS (f64) and adds them to D.grads once per slot. Dynamic LDS:
+// diff_slots(D) doubles.
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const DevScene s,
+                                                    const DiffDev D, uint32_t sample_start,
+                                                    float inv_spp) {
+  __shared__ float lut_s[256];
+  __shared__ uint32_t sfx_s[kMaxSuffix * kBlock];
+  extern __shared__ double grad_s[];
+  const uint32_t ns = diff_slots(D);
+  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) grad_s[i] = 0.0;
+  load_lut(lut_s, s.gamma_lut);  // ends with a barrier
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint32_t tiles_x = (w + 15u) / 16u;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  if (px < w && py < h) {
+    const float *g = D.dldi + ((size_t)py * w + px) * 3;
+    const V3 dl = V3{g[0], g[1], g[2]} * inv_spp;
+    if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
+      const GradSink G{grad_s, D.n_pal, D.n_int};
+      const SuffixList sfx{sfx_s + threadIdx.x, (uint32_t)kBlock};
+      render_sample_diff<true>(s, p, p, lut_s, D, px, py, sample_start + blockIdx.y, G, sfx, dl);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+    const double v = grad_s[i];
+    if (v != 0.0) atomicAdd(&D.grads[i], v);
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------
@@ -500,6 +601,9 @@ struct HipError {
   } while (0)
 
 struct InvalidScene {
+  std::string what;
+};
+struct Unsupported {  // a valid request this backend does not serve (LW_EUNSUPPORTED)
   std::string what;
 };
 
@@ -568,6 +672,7 @@ struct lw_hip_scene {
   DevBuf<float> sky;
   DevBuf<float> lut;
   DevBuf<uint2> occupancy;
+  std::vector<uint32_t> quad_flags;  // host copy of each quad's flags (p0.w)
   LayoutKey key;
   uint32_t max_emitter_rays = 1;  // emitter face rays one bounce can trace
   DevScene dev() const {
@@ -588,12 +693,19 @@ struct lw_hip_backend {
   DevBuf<uint32_t> counter;
   lw_timing timing{};
   // Launch shape: the persistent path kernels run `blocks_*` workgroups of kBlock lanes
-  // (the device's resident capacity); LW_KERNEL=direct selects one thread per sample.
+  // (the device's resident capacity).
   uint32_t blocks_frame = 0, blocks_list = 0;
-  // LW_KERNEL: "persistent" (k_paths), "direct" (one thread per sample) or "wavefront"
-  // (k_wf_*, LW_WF_SLOTS path slots).
-  int mode = 0;
-  int waves = 1, trace_waves = 1;  // LW_WAVES, LW_TRACE_WAVES: register budgets (see dispatch)
+  // LW_KERNEL: "direct" (one lane per sample, the default and the fastest measured at the
+  // 720p 256-spp frame), "persistent" (k_paths) or "wavefront" (k_wf_*, LW_WF_SLOTS path
+  // slots).
+  int mode = 1;
+  // LW_WAVES, LW_TRACE_WAVES: register budgets (see dispatch). The direct form measured
+  // fastest at 5 waves per SIMD (96 VGPRs): 500 Msamples/s at the 720p 256-spp frame
+  // against 451 at 4, 492 at 6 and 414 at 8. The persistent form, which has no 5-wave
+  // variant, then runs at its own best, 4.
+  int waves = 5, trace_waves = 1;
+  // LW_XCD_REMAP=1: the direct kernel gives each XCD a contiguous run of tiles.
+  uint32_t xcd_remap = 0;
   uint32_t wf_slots = 1u << 20;
   DevBuf<PathState> wf_st;
   DevBuf<float4> wf_o, wf_d;
@@ -607,6 +719,10 @@ struct lw_hip_backend {
   uint32_t batch_passes = 0;
   bool use_graphs = true;
   DevBuf<BatchDev> batch_dev;
+  // Differentiable rendering: the parameters and derivative inputs of the last call.
+  DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
+  DevBuf<uint32_t> diff_qg, diff_qig;
+  DevBuf<double> diff_grads;
 };
 
 namespace {
@@ -773,6 +889,8 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.quads.upload(v.quads, v.quads_len, b->stream);
   std::vector<uint32_t> info(v.quads_len);
   for (size_t q = 0; q < v.quads_len; q++) info[q] = quad_info(v.quads[q]);
+  sc.quad_flags.resize(v.quads_len);
+  for (size_t q = 0; q < v.quads_len; q++) std::memcpy(&sc.quad_flags[q], &v.quads[q].p0[3], 4);
   sc.qinfo.upload(info.data(), info.size(), b->stream);
   sc.atlas.upload(v.atlas, v.atlas_len, b->stream);
   sc.emitters.upload(v.emitters, v.emitters_len, b->stream);
@@ -804,9 +922,34 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
 #define LW_WAVES_DISPATCH(waves, KERNEL_EXPR)  \
   do {                                         \
     switch (waves) {                           \
+      case 1: { constexpr int W = 1; KERNEL_EXPR; } break; \
+      case 3: { constexpr int W = 3; KERNEL_EXPR; } break; \
+      default: { constexpr int W = 4; KERNEL_EXPR; } break; \
+    }                                          \
+  } while (0)
+
+// The direct kernels' budgets: also 5, 6 and 8 waves per SIMD (96, 80 and 64 VGPRs).
+#define LW_DIRECT_DISPATCH(waves, KERNEL_EXPR) \
+  do {                                         \
+    switch (waves) {                           \
       case 3: { constexpr int W = 3; KERNEL_EXPR; } break; \
       case 4: { constexpr int W = 4; KERNEL_EXPR; } break; \
+      case 5: { constexpr int W = 5; KERNEL_EXPR; } break; \
+      case 6: { constexpr int W = 6; KERNEL_EXPR; } break; \
+      case 8: { constexpr int W = 8; KERNEL_EXPR; } break; \
       default: { constexpr int W = 1; KERNEL_EXPR; } break; \
+    }                                          \
+  } while (0)
+
+// The differentiable kernels: 4 waves per SIMD, or free (LW_WAVES=1).
+#define LW_DIFF_DISPATCH(waves, KERNEL_EXPR)   \
+  do {                                         \
+    if ((waves) == 1) {                        \
+      constexpr int W = 1;                     \
+      KERNEL_EXPR;                             \
+    } else {                                   \
+      constexpr int W = 4;                     \
+      KERNEL_EXPR;                             \
     }                                          \
   } while (0)
 
@@ -857,8 +1000,10 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
 constexpr uint32_t kMaxBatch = 64;         // render/cuda.rs:165
 
+// With `diff`, the samples are render_sample_diff's under its parameters (direct form).
 void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
-                  uint32_t spp_in, lw_progress_fn progress, void *user, float *out_rgb) {
+                  uint32_t spp_in, lw_progress_fn progress, void *user, float *out_rgb,
+                  const DiffDev *diff = nullptr) {
   auto t_wall = std::chrono::steady_clock::now();
   if (!(layout_key(params_in) == sc.key))
     throw InvalidScene{"params do not match the layout the scene was validated with"};
@@ -893,9 +1038,12 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     const uint32_t nb = std::min(batch, std::min(spp - done, max_batch));
     const auto start = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (b->mode == MODE_DIRECT) {
-      LW_WAVES_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
-                                      p, ds, done, b->rad.ptr)));
+    if (diff) {
+      LW_DIFF_DISPATCH(b->waves, (k_render_diff<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
+                                     p, ds, *diff, done, b->rad.ptr)));
+    } else if (b->mode == MODE_DIRECT) {
+      LW_DIRECT_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
+                                       p, ds, done, b->xcd_remap, b->rad.ptr)));
     } else if (b->mode == MODE_WAVEFRONT) {
       const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
       run_wavefront(b, sc, p, fw, fw.total);
@@ -1030,11 +1178,18 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   HIP_TRY(hipGetLastError());
   const BatchWork work{b->batch_dev.ptr, b->rad.ptr};
   const dim3 acc_grid((unsigned)((npix_total + kBlock - 1) / kBlock));
+  // Pass 0 has the most items (a view's samples per pass only shrink at its last pass).
+  const uint32_t direct_blocks = (uint32_t)std::max<uint64_t>(1, items / kBlock);
   auto pass_ops = [&](hipStream_t st) {
     k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, st>>>(b->batch_dev.ptr);
     k_fill_u32<<<dim3(1), dim3(64), 0, st>>>(b->counter.ptr, 0u);
-    LW_WAVES_DISPATCH(b->waves, (k_paths<BatchWork, W><<<dim3(b->blocks_frame), dim3(kBlock), 0, st>>>(
-                                    p0, ds, work, b->counter.ptr)));
+    if (b->mode == MODE_DIRECT) {
+      LW_DIRECT_DISPATCH(b->waves, (k_render_batch<W><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
+                                       p0, ds, b->batch_dev.ptr, b->rad.ptr)));
+    } else {
+      LW_WAVES_DISPATCH(b->waves, (k_paths<BatchWork, W><<<dim3(b->blocks_frame), dim3(kBlock), 0, st>>>(
+                                      p0, ds, work, b->counter.ptr)));
+    }
     k_accumulate_batch<<<acc_grid, dim3(kBlock), 0, st>>>(b->batch_dev.ptr, b->rad.ptr, b->acc.ptr);
   };
   lw_timing tm{};
@@ -1105,6 +1260,120 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   b->timing = tm;
 }
 
+// ---- differentiable rendering: argument checks, device copies, the adjoint loop ----
+
+constexpr uint32_t kMaxDiffSlots = 6144;  // f64 gradient slots a workgroup keeps in LDS (48 KiB)
+
+// Checks a lw_diff_desc against the scene and params and uploads its arrays; the returned
+// DiffDev points into b's buffers.
+DiffDev prepare_diff(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p,
+                     const lw_diff_desc &dd) {
+  auto fail = [](const std::string &m) { throw InvalidScene{m}; };
+  const uint64_t slots = 3ull * dd.n_palette + dd.n_intensity + 3ull;
+  if (slots > kMaxDiffSlots) fail("too many palette and intensity groups");
+  const size_t nq = sc.quad_flags.size();
+  if (!dd.quad_palette || !dd.quad_intensity) fail("quad group arrays are null");
+  if ((dd.n_palette && !dd.palette) || (dd.n_intensity && !dd.intensity)) fail("factors are null");
+  for (size_t q = 0; q < nq; q++) {
+    const uint32_t g = dd.quad_palette[q], ig = dd.quad_intensity[q];
+    if (g != LW_NO_GROUP && g >= dd.n_palette) fail("palette group out of range");
+    if (ig != LW_NO_GROUP && ig >= dd.n_intensity) fail("intensity group out of range");
+    if (g != LW_NO_GROUP && (sc.quad_flags[q] & LW_FLAG_ALPHA_MASK) == LW_ALPHA_TRANSLUCENT)
+      fail("a translucent quad has a palette group");
+  }
+  for (uint32_t i = 0; i < 3 * dd.n_palette; i++)
+    if (!(std::isfinite(dd.palette[i]) && dd.palette[i] > 0.0f)) fail("palette factors must be finite and > 0");
+  for (uint32_t i = 0; i < dd.n_intensity; i++)
+    if (!(std::isfinite(dd.intensity[i]) && dd.intensity[i] > 0.0f)) fail("intensity factors must be finite and > 0");
+  const uint32_t strategy = p.misc_u[3] > 0 ? p.emitter[0] : LW_EMITTER_SAMPLING_NONE;
+  if (strategy != LW_EMITTER_SAMPLING_NONE && strategy != LW_EMITTER_SAMPLING_ONE)
+    throw Unsupported{"differentiable rendering samples emitters with NONE or ONE only"};
+  uint32_t cache_len = 0;
+  if (dd.sky_params) {
+    if (!(p.sky_map[0] < 1.0f) || !(p.misc_f[2] >= 1.0f))
+      throw Unsupported{"sky parameters need the Preetham sky cache (no sky maps)"};
+    if (p.sun_sprite[2] > 0.0f) throw Unsupported{"sky parameters with the sun sprite drawn"};
+    const uint64_t n = (uint64_t)p.misc_f[2] + 1;
+    if (n * n * 3 > sc.sky.n) fail("sky cache shorter than the params' resolution");
+    cache_len = (uint32_t)(n * n * 3);
+    if (!dd.d_sky_cache) fail("d_sky_cache is null");
+  }
+  HIP_TRY(hipSetDevice(b->device));
+  b->diff_pal.upload(dd.palette, 3 * (size_t)dd.n_palette, b->stream);
+  b->diff_inten.upload(dd.intensity, dd.n_intensity, b->stream);
+  b->diff_qg.upload(dd.quad_palette, nq, b->stream);
+  b->diff_qig.upload(dd.quad_intensity, nq, b->stream);
+  b->diff_dsky.upload(dd.d_sky_cache, dd.sky_params ? 3 * (size_t)cache_len : 0, b->stream);
+  HIP_TRY(hipStreamSynchronize(b->stream));  // the caller's arrays may go after the call
+  DiffDev D{};
+  D.pal = b->diff_pal.ptr;
+  D.inten = b->diff_inten.ptr;
+  D.qgroup = b->diff_qg.ptr;
+  D.qigroup = b->diff_qig.ptr;
+  D.dsky = b->diff_dsky.ptr;
+  for (int k = 0; k < 3; k++) {
+    for (int c = 0; c < 3; c++) {
+      D.dsun[k][c] = dd.sky_params ? dd.d_sun_dir[k][c] : 0.0f;
+      D.dsun_color[k][c] = dd.sky_params ? dd.d_sun_color[k][c] : 0.0f;
+    }
+    D.dsky_light[k] = dd.sky_params ? dd.d_sky_light[k] : 0.0f;
+  }
+  D.n_pal = dd.n_palette;
+  D.n_int = dd.n_intensity;
+  D.rr_off = dd.rr_off ? 1u : 0u;
+  D.sky_params = dd.sky_params ? 1u : 0u;
+  D.cache_len = cache_len;
+  return D;
+}
+
+void adjoint_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
+                   uint32_t spp_in, const lw_diff_desc &dd, const float *dldi, double *grads) {
+  auto t_wall = std::chrono::steady_clock::now();
+  if (!(layout_key(params_in) == sc.key))
+    throw InvalidScene{"params do not match the layout the scene was validated with"};
+  lw_params p = params_in;
+  p.frame[2] = 0;
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  if (w == 0 || h == 0) throw InvalidScene{"empty frame"};
+  if (p.frame[3] > kMaxSuffix) throw InvalidScene{"the adjoint takes at most 16 bounces"};
+  const uint64_t npix = (uint64_t)w * h;
+  if (npix > (1ull << 31) / 3) throw InvalidScene{"frame too large"};
+  const uint32_t spp = std::max(spp_in, 1u);
+  DiffDev D = prepare_diff(b, sc, p, dd);
+  const uint32_t slots = 3u * D.n_pal + D.n_int + 3u;
+  b->diff_dldi.upload(dldi, npix * 3, b->stream);
+  b->diff_grads.ensure(slots);
+  D.dldi = b->diff_dldi.ptr;
+  D.grads = b->diff_grads.ptr;
+  k_zero<<<dim3((slots + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream>>>(D.grads, slots);
+  HIP_TRY(hipGetLastError());
+  const DevScene ds = sc.dev();
+  const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+  const float inv_spp = 1.0f / (float)spp;
+  const size_t lds = (size_t)slots * sizeof(double);
+  lw_timing tm{};
+  for (uint32_t done = 0; done < spp;) {
+    const uint32_t nb = std::min(spp - done, kMaxBatch);
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    LW_DIFF_DISPATCH(b->waves, (k_adjoint<W><<<dim3(tiles, nb), dim3(kBlock), lds, b->stream>>>(
+                                   p, ds, D, done, inv_spp)));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    tm.trace_ms += ms;
+    tm.launches += 1;
+    done += nb;
+  }
+  HIP_TRY(hipMemcpyAsync(grads, D.grads, slots * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  tm.samples = npix * spp;
+  tm.wall_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+  b->timing = tm;
+}
+
 template <typename F>
 int guarded(lw_hip_backend *b, F &&f) {
   try {
@@ -1114,6 +1383,9 @@ int guarded(lw_hip_backend *b, F &&f) {
   } catch (const InvalidScene &e) {
     set_err(b, "invalid scene: " + e.what);
     return LW_EINVAL;
+  } catch (const Unsupported &e) {
+    set_err(b, "unsupported: " + e.what);
+    return LW_EUNSUPPORTED;
   } catch (const HipError &e) {
     set_err(b, "HIP: " + e.what);
     return e.code == hipErrorOutOfMemory ? LW_ENOMEM : LW_EHIP;
@@ -1201,8 +1473,10 @@ int lw_hip_open(int device, lw_hip_backend **out) {
     }
     if (const char *e = std::getenv("LW_KERNEL")) {
       if (std::strcmp(e, "direct") == 0) b->mode = MODE_DIRECT;
+      if (std::strcmp(e, "persistent") == 0) b->mode = MODE_PERSISTENT;
       if (std::strcmp(e, "wavefront") == 0) b->mode = MODE_WAVEFRONT;
     }
+    if (const char *e = std::getenv("LW_XCD_REMAP")) b->xcd_remap = std::atoi(e) != 0;
     if (const char *e = std::getenv("LW_WF_SLOTS")) {
       const long v = std::atol(e);
       if (v >= kBlock) b->wf_slots = (uint32_t)std::min<long>(v, 1l << 24) / kBlock * kBlock;
@@ -1286,6 +1560,22 @@ int lw_hip_render_hq_batch(lw_hip_backend *b, const lw_hip_scene *scene, const l
                            size_t n, lw_progress_fn progress, void *user, float *const *out_rgb) {
   if (!b || !scene || !views || !out_rgb) return LW_EINVAL;
   return guarded(b, [&] { render_batch(b, *scene, views, n, progress, user, out_rgb); });
+}
+
+int lw_hip_render_diff(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                       uint32_t spp, const lw_diff_desc *diff, float *out_rgb) {
+  if (!b || !scene || !params || !diff || !out_rgb) return LW_EINVAL;
+  return guarded(b, [&] {
+    const DiffDev D = prepare_diff(b, *scene, *params, *diff);
+    render_frame(b, *scene, *params, spp, nullptr, nullptr, out_rgb, &D);
+  });
+}
+
+int lw_hip_adjoint(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                   uint32_t spp, const lw_diff_desc *diff, const float *d_loss_d_image,
+                   double *grads) {
+  if (!b || !scene || !params || !diff || !d_loss_d_image || !grads) return LW_EINVAL;
+  return guarded(b, [&] { adjoint_frame(b, *scene, *params, spp, *diff, d_loss_d_image, grads); });
 }
 
 int lw_hip_copy_probe(lw_hip_backend *b, size_t bytes, double *ms) {

@@ -41,19 +41,6 @@ struct PathState {
   float em_scale, em_dist;
 };
 
-// The camera of one view of a batch: what render_sample reads from Params to make its
-// camera ray (integrator.rs:541-556), and the view's sample count.
-struct ViewCam {
-  float cam_pos[4], cam_fwd[4], cam_right[4], cam_up[4];
-  uint32_t width, height, seed, spp;
-};
-__device__ __forceinline__ uint32_t cam_width(const lw_params &c) { return c.frame[0]; }
-__device__ __forceinline__ uint32_t cam_height(const lw_params &c) { return c.frame[1]; }
-__device__ __forceinline__ uint32_t cam_seed(const lw_params &c) { return c.misc_u[0]; }
-__device__ __forceinline__ uint32_t cam_width(const ViewCam &c) { return c.width; }
-__device__ __forceinline__ uint32_t cam_height(const ViewCam &c) { return c.height; }
-__device__ __forceinline__ uint32_t cam_seed(const ViewCam &c) { return c.seed; }
-
 struct Tracer {
   const DevScene &s;
   const lw_params &p;

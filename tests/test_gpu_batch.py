@@ -38,8 +38,11 @@ def open_backend(env):
 
 
 @pytest.mark.parametrize("env", [{}, {"LW_BATCH_GRAPHS": "0"}, {"LW_BATCH_PASSES": "5"},
+                                 {"LW_KERNEL": "persistent"},
+                                 {"LW_KERNEL": "persistent", "LW_BATCH_PASSES": "3"},
                                  {"LW_KERNEL": "wavefront"}],
-                         ids=["graphs", "no-graphs", "5-passes", "wavefront"])
+                         ids=["graphs", "no-graphs", "5-passes", "persistent",
+                              "persistent-3-passes", "wavefront"])
 def test_batch_equals_single_view_renders(batch_setup, env):
     base, views = batch_setup
     b = open_backend(env)
@@ -88,3 +91,27 @@ def test_batch_rejects_views_that_differ_beyond_the_camera(batch_setup):
         scene.free()
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("env", [{"LW_XCD_REMAP": "1"}, {"LW_WAVES": "6"}, {"LW_WAVES": "8"}],
+                         ids=["xcd-remap", "6-waves", "8-waves"])
+def test_launch_variants_give_the_same_image(batch_setup, env):
+    # The XCD tile renumbering and the register budgets change only where and how a sample
+    # runs, never its value.
+    base, views = batch_setup
+    p, spp = views[3]
+    ref = open_backend({})
+    alt = open_backend(env)
+    try:
+        s1, s2 = ref.upload(base), alt.upload(base)
+        a = ref.render_hq_scene(s1, p, spp)
+        b = alt.render_hq_scene(s2, p, spp)
+        assert np.array_equal(a.view("<u4"), b.view("<u4"))
+        outs = alt.render_hq_batch(s2, views[:4])
+        for (pv, sv), img in zip(views[:4], outs):
+            assert np.array_equal(img.view("<u4"), ref.render_hq_scene(s1, pv, sv).view("<u4"))
+        s1.free()
+        s2.free()
+    finally:
+        ref.close()
+        alt.close()
