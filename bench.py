@@ -214,12 +214,81 @@ def bench_batch(args, lw, frames):
     }
 
 
+def bench_diff(args, lw, frames):
+    """cfg3: one forward (lw_hip_render_diff) and one adjoint (lw_hip_adjoint) of the L2 loss
+    against a target rendered at perturbed parameters, per step; the adjoint covers all 115
+    parameters (96 albedo factors, 16 intensities, turbidity and the sun's two angles)."""
+    from lumenwright_amd import diff
+    t0 = time.perf_counter()
+    prob = diff.config3_problem()
+    cfg = prob.cfg
+    fr = diff.frame_at(cfg, prob.scene, prob.start.sky)
+    tfr = diff.frame_at(cfg, prob.scene, prob.target.sky)
+    derivs = diff.sky_derivatives(cfg, prob.scene, prob.start.sky, 3e-4)
+    log(f"problem built in {time.perf_counter() - t0:.1f} s: {cfg.name}")
+    b = lw.HipBackend.open(0)
+    sc = b.upload(fr)
+    tsc = b.upload(tfr)
+    target = b.render_diff(tsc, tfr.params, cfg.spp, prob.target.desc())
+    desc_f = prob.start.desc()
+    start_a = prob.start.with_values()
+    desc_a = start_a.desc(derivs)
+    samples = cfg.width * cfg.height * cfg.spp
+    for _ in range(args.warmup):
+        img = b.render_diff(sc, fr.params, cfg.spp, desc_f)
+        _, dldi = diff.l2_loss(img, target)
+        b.adjoint(sc, fr.params, cfg.spp, desc_a, start_a.n_grads, dldi)
+    fwd_ms = adj_ms = fwd_wall = adj_wall = 0.0
+    for i in range(args.steps):
+        t = time.perf_counter()
+        img = b.render_diff(sc, fr.params, cfg.spp, desc_f)
+        fwd_wall += time.perf_counter() - t
+        fwd_ms += b.last_timing()["trace_ms"]
+        loss, dldi = diff.l2_loss(img, target)
+        t = time.perf_counter()
+        g = b.adjoint(sc, fr.params, cfg.spp, desc_a, start_a.n_grads, dldi)
+        adj_wall += time.perf_counter() - t
+        adj_ms += b.last_timing()["trace_ms"]
+        log(f"step {i + 1}/{args.steps}: loss {loss:.6g}, |grad| {np.linalg.norm(g):.4g}")
+    sc.free()
+    tsc.free()
+    b.close()
+    k = args.steps
+    return {
+        "metric": METRIC,
+        "value": samples * k / (adj_ms / 1e3) / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": args.warmup,
+        "ms_per_step": (fwd_wall + adj_wall) / k * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 path arithmetic, f64 accumulation and gradient sums",
+        "data": "synthetic (seeded value-noise voxel world, procedural sky)",
+        "config": {"workload": cfg.name + "; forward + adjoint of an L2 image loss over 115 "
+                               "parameters (96 albedo factors, 16 emitter intensities, "
+                               "turbidity, sun altitude and azimuth)",
+                   "width": cfg.width, "height": cfg.height, "spp": cfg.spp,
+                   "max_bounces": cfg.max_bounces, "parallelism": "1 GPU"},
+        "value_is": "adjoint pass Msamples/s (kernel time)",
+        "forward_msamples_per_s": samples * k / (fwd_ms / 1e3) / 1e6,
+        "adjoint_over_forward": adj_ms / fwd_ms,
+        "adjoint_over_forward_wall": adj_wall / fwd_wall,
+        "forward_kernel_ms": fwd_ms / k,
+        "adjoint_kernel_ms": adj_ms / k,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3])
     ap.add_argument("--spp", type=int, default=None, help="override the workload's spp")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="config 2: scale every view's spp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -235,7 +304,12 @@ def main():
     import lumenwright_amd as lw
     from lumenwright_amd import frames
 
-    line = bench_batch(args, lw, frames) if args.config == 2 else bench_plain(args, lw, frames)
+    if args.config == 2:
+        line = bench_batch(args, lw, frames)
+    elif args.config == 3:
+        line = bench_diff(args, lw, frames)
+    else:
+        line = bench_plain(args, lw, frames)
     print(json.dumps(line), flush=True)
 
 

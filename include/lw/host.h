@@ -121,7 +121,9 @@ typedef struct lw_world_spec {
   int32_t water_level;         /* non-solid cells below this height hold water; < 0: none */
   float emitter_fraction;      /* of solid cells with a non-solid neighbour */
   float cutout_fraction;       /* of columns whose surface is above the water */
-  uint32_t pad;
+  uint32_t emitter_blocks;     /* this many more emitters, each the surface block of a seeded
+                                  column with z >= half the depth (the half the default camera
+                                  faces), emittance 1 on every face */
 } lw_world_spec;
 
 /* The HQ scene of a synthetic world (an owned HqScene). */
@@ -139,6 +141,11 @@ typedef struct lw_world {
   uint32_t *emitter_grid;
   size_t emitter_grid_len;
   lw_scene_info info;
+  /* per quad (quads_len each): the block type it belongs to (cube types 0..cube_types-1,
+   * then the cutout types; LW_NO_GROUP for the water surface), and the emitter it belongs to
+   * (the emitter's index in `emitters`, LW_NO_GROUP for none) */
+  uint32_t *quad_type;
+  uint32_t *quad_emitter;
   /* what the builder made (voxel.rs HqStats) */
   uint64_t bricks, uniform_bricks, buried_bricks, solid_cells, water_cells, cutout_cells;
 } lw_world;

@@ -136,6 +136,26 @@ class HipBackend:
                                                          len(views), cb, None, ptrs))
         return outs
 
+    def render_diff(self, scene: HipScene, params: np.ndarray, spp: int, desc) -> np.ndarray:
+        """lw_hip_render_diff: the image under a lw_diff_desc's parameters, (w*h, 3) f32."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        w, h = int(p["frame"][0]), int(p["frame"][1])
+        out = np.zeros((w * h, 3), dtype="<f4")
+        self._check(_native.lib().lw_hip_render_diff(self.handle, scene.handle, _addr(p), spp,
+                                                     ctypes.byref(desc), _addr(out)))
+        return out
+
+    def adjoint(self, scene: HipScene, params: np.ndarray, spp: int, desc, n_grads: int,
+                d_loss_d_image: np.ndarray) -> np.ndarray:
+        """lw_hip_adjoint: sum over pixels of dL/dI . dI/dtheta, (n_grads,) f64."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        g = np.ascontiguousarray(d_loss_d_image, dtype="<f4").reshape(-1)
+        assert g.size == int(p["frame"][0]) * int(p["frame"][1]) * 3
+        out = np.zeros(n_grads, dtype="<f8")
+        self._check(_native.lib().lw_hip_adjoint(self.handle, scene.handle, _addr(p), spp,
+                                                 ctypes.byref(desc), _addr(g), _addr(out)))
+        return out
+
     def copy_probe(self, nbytes: int) -> float:
         """Times a plain device-to-device copy of nbytes (ms); see lw_hip_copy_probe."""
         ms = ctypes.c_double()

@@ -107,7 +107,9 @@ inline uint64_t hash5(uint64_t seed, uint64_t a, uint64_t b, uint64_t c, uint64_
 }
 inline double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
 
-enum : uint64_t { H_LATTICE = 1, H_TYPE, H_EMIT, H_PLANT, H_PLANT_TYPE, H_TEXEL, H_ALPHA, H_EMITTANCE };
+enum : uint64_t {
+  H_LATTICE = 1, H_TYPE, H_EMIT, H_PLANT, H_PLANT_TYPE, H_TEXEL, H_ALPHA, H_EMITTANCE, H_EMIT_COLUMN
+};
 
 double value_noise(uint64_t seed, int32_t x, int32_t z) {
   static const int lattice[4] = {32, 16, 8, 4};
@@ -577,18 +579,25 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
 
     std::vector<lw_model> models(1, lw_model{0, 0, 0, 0});
     std::vector<lw_quad> quads;
+    std::vector<uint32_t> quad_type, quad_emitter;  // kept level with `quads`
+    auto tag = [&](uint32_t type, uint32_t emitter) {
+      quad_type.resize(quads.size(), type);
+      quad_emitter.resize(quads.size(), emitter);
+    };
     const float no_emit[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t t = 0; t < n_cube; t++) {
       uint32_t sx, sy;
       sprite_xy(t, sx, sy);
       models.push_back(lw_model{(uint32_t)quads.size(), 6, 0, 0});
       cube_faces(quads, LW_ALPHA_OPAQUE | LW_FLAG_FRONT_ONLY, sx, sy, no_emit);
+      tag(t, LW_NO_GROUP);
     }
     for (uint32_t t = 0; t < n_cut; t++) {
       uint32_t sx, sy;
       sprite_xy(n_cube + t, sx, sy);
       models.push_back(lw_model{(uint32_t)quads.size(), 4, 0, 0});
       cross_faces(quads, sx, sy);
+      tag(n_cube + t, LW_NO_GROUP);
     }
     {
       uint32_t sx, sy;
@@ -596,6 +605,23 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
       models.push_back(lw_model{(uint32_t)quads.size(), 1, 0, 0});
       const float p0[3] = {0.0f, 0.875f, 0.0f}, eu[3] = {1, 0, 0}, ev[3] = {0, 0, 1};
       quads.push_back(make_quad(p0, eu, ev, LW_FLAG_WATER, 0.0f, sx, sy));
+      tag(LW_NO_GROUP, LW_NO_GROUP);
+    }
+    // Seeded emitter columns: their surface block becomes an emitter.
+    std::vector<uint8_t> emitter_column((size_t)gx * gz, 0);
+    if (spec->emitter_blocks > 0) {
+      if (gx <= 16 || gz <= 16) throw std::invalid_argument("world too small for emitter blocks");
+      const uint32_t want = std::min<uint32_t>(spec->emitter_blocks, (uint32_t)((gx - 16) * (gz / 2 - 8)));
+      uint32_t placed = 0;
+      for (uint64_t k = 0; placed < want; k++) {
+        const int32_t x = 8 + (int32_t)(hash5(seed, H_EMIT_COLUMN, k, 0, 0) % (uint64_t)(gx - 16));
+        const int32_t z = gz / 2 + (int32_t)(hash5(seed, H_EMIT_COLUMN, k, 1, 0) % (uint64_t)(gz / 2 - 8));
+        uint8_t &c = emitter_column[(size_t)x * gz + z];
+        if (!c && H(x, z) < gy) {
+          c = 1;
+          placed++;
+        }
+      }
     }
 
     const uint64_t nb = (uint64_t)dx * dy * dz;
@@ -637,18 +663,22 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
         uint32_t id = 0;
         if (solid(x, y, z)) {
           w->solid_cells++;
-          if (spec->emitter_fraction > 0.0f && exposed(x, y, z) &&
-              u01(hash5(seed, H_EMIT, (uint64_t)x, (uint64_t)y, (uint64_t)z)) < spec->emitter_fraction) {
+          const bool column_emitter = emitter_column[(size_t)x * gz + z] && y == H(x, z);
+          if (column_emitter ||
+              (spec->emitter_fraction > 0.0f && exposed(x, y, z) &&
+               u01(hash5(seed, H_EMIT, (uint64_t)x, (uint64_t)y, (uint64_t)z)) < spec->emitter_fraction)) {
             const uint32_t t = (uint32_t)(hash5(seed, H_TYPE, (uint64_t)x, (uint64_t)y, (uint64_t)z) % n_cube);
             uint32_t sx, sy;
             sprite_xy(t, sx, sy);
             float em[6];
             const uint32_t e_index = (uint32_t)sites.size();
             for (int f = 0; f < 6; f++)
-              em[f] = (float)(1.0 + 3.0 * u01(hash5(tseed, H_EMITTANCE, e_index, (uint64_t)f, 0)));
+              em[f] = column_emitter ? 1.0f
+                                     : (float)(1.0 + 3.0 * u01(hash5(tseed, H_EMITTANCE, e_index, (uint64_t)f, 0)));
             id = (uint32_t)models.size();
             models.push_back(lw_model{(uint32_t)quads.size(), 6, 0, 0});
             cube_faces(quads, LW_ALPHA_OPAQUE | LW_FLAG_FRONT_ONLY, sx, sy, em);
+            tag(t, e_index);
             sites.push_back({{x, y, z}, id});
           } else {
             id = 1 + (uint32_t)(hash5(seed, H_TYPE, (uint64_t)x, (uint64_t)y, (uint64_t)z) % n_cube);
@@ -726,6 +756,8 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
     w->emitters_len = emitters.size();
     w->emitter_grid = to_heap(egrid);
     w->emitter_grid_len = egrid.size();
+    w->quad_type = to_heap(quad_type);
+    w->quad_emitter = to_heap(quad_emitter);
     lw_scene_info &info = w->info;
     std::memset(&info, 0, sizeof(info));
     info.dims[0] = dx;
@@ -756,6 +788,8 @@ void lw_world_free(lw_world *w) {
   std::free(w->atlas);
   std::free(w->emitters);
   std::free(w->emitter_grid);
+  std::free(w->quad_type);
+  std::free(w->quad_emitter);
   std::free(w);
 }
 

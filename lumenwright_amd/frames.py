@@ -64,7 +64,7 @@ class WorldSpecC(ctypes.Structure):
                 ("texture_seed", ctypes.c_uint64), ("cube_types", ctypes.c_uint32),
                 ("cutout_types", ctypes.c_uint32), ("water_level", ctypes.c_int32),
                 ("emitter_fraction", ctypes.c_float), ("cutout_fraction", ctypes.c_float),
-                ("pad", ctypes.c_uint32)]
+                ("emitter_blocks", ctypes.c_uint32)]
 
 
 class WorldC(ctypes.Structure):
@@ -75,6 +75,7 @@ class WorldC(ctypes.Structure):
                 ("emitters", ctypes.c_void_p), ("emitters_len", ctypes.c_size_t),
                 ("emitter_grid", ctypes.c_void_p), ("emitter_grid_len", ctypes.c_size_t),
                 ("info", SceneInfoC),
+                ("quad_type", ctypes.c_void_p), ("quad_emitter", ctypes.c_void_p),
                 ("bricks", ctypes.c_uint64), ("uniform_bricks", ctypes.c_uint64),
                 ("buried_bricks", ctypes.c_uint64), ("solid_cells", ctypes.c_uint64),
                 ("water_cells", ctypes.c_uint64), ("cutout_cells", ctypes.c_uint64)]
@@ -202,11 +203,12 @@ class WorldSpec:
     water_level: int = -1
     emitter_fraction: float = 0.0
     cutout_fraction: float = 0.0
+    emitter_blocks: int = 0
 
     def c(self) -> WorldSpecC:
         return WorldSpecC(self.chunks_x, self.chunks_z, self.sections, self.seed,
                           self.texture_seed, self.cube_types, self.cutout_types, self.water_level,
-                          self.emitter_fraction, self.cutout_fraction, 0)
+                          self.emitter_fraction, self.cutout_fraction, self.emitter_blocks)
 
 
 def world_height(spec: WorldSpec, x: int, z: int) -> int:
@@ -246,6 +248,10 @@ def build_world(spec: WorldSpec) -> HqScene:
         scene.stats = {k: int(getattr(w, k)) for k in ("bricks", "uniform_bricks", "buried_bricks",
                                                         "solid_cells", "water_cells",
                                                         "cutout_cells")}
+        # Per quad: its block type and emitter (NO_GROUP for none), for differentiable
+        # rendering's parameter groups.
+        scene.quad_type = _copy(w.quad_type, w.quads_len, "<u4")
+        scene.quad_emitter = _copy(w.quad_emitter, w.quads_len, "<u4")
         return scene
     finally:
         L.lw_world_free(h)
@@ -273,6 +279,9 @@ class FrameConfig:
     camera_height: float = 20.0
     seed: int = 0
     turbidity: float = 2.5
+    # The sun from altitude and azimuth in radians (sky.rs:41-44) instead of the time of day.
+    sun_altitude: Optional[float] = None
+    sun_azimuth: Optional[float] = None
 
 
 CONFIGS = {
@@ -283,6 +292,14 @@ CONFIGS = {
                    WorldSpec(24, 24, 24, cube_types=56, cutout_types=8, water_level=62,
                              emitter_fraction=0.01, cutout_fraction=0.05),
                    1280, 720, 256, 8, "game-dusk", 12800, GAME_EMITTER_INTENSITY),
+    # The differentiable render: the cfg0 world with 16 emitter blocks (emittance 1 times the
+    # default emitter intensity, request.rs:126), its sun (tick 4000: altitude 67.955 deg,
+    # azimuth 90 deg) given by angles so that they can be parameters.
+    3: FrameConfig("cfg3: differentiable render, 8x8 chunks with 16 emitter blocks, 256x144, "
+                   "64 spp, 4 bounces, Preetham day sky",
+                   WorldSpec(8, 8, 24, cube_types=32, emitter_blocks=16), 256, 144, 64, 4,
+                   "chunky", 4000, DEFAULT_EMITTER_INTENSITY,
+                   sun_altitude=1.186048, sun_azimuth=1.5707963),
 }
 
 
@@ -310,7 +327,11 @@ def make_frame(cfg: FrameConfig, scene: Optional[HqScene] = None) -> HqFrame:
     gx, gz = cfg.world.chunks_x * 16, cfg.world.chunks_z * 16
     cx, cz = gx // 2, gz // 2
     eye = [cx + 0.5, world_height(cfg.world, cx, cz) + cfg.camera_height, cz + 0.5]
-    sun = sun_direction_from_angle(celestial_angle(cfg.ticks))
+    if cfg.sun_altitude is not None and cfg.sun_azimuth is not None:
+        import math
+        sun = sun_direction(math.degrees(cfg.sun_altitude), math.degrees(cfg.sun_azimuth))
+    else:
+        sun = sun_direction_from_angle(celestial_angle(cfg.ticks))
     sky = hq_sky(sun, 1.25, 1.0)
     params = hq_params(scene, width=cfg.width, height=cfg.height, eye=eye, yaw=cfg.yaw,
                        pitch=cfg.pitch, fov_y=cfg.fov_y, sky=sky, max_bounces=cfg.max_bounces,

@@ -72,6 +72,14 @@ def lib():
         L.lwo_render_hq.restype = i32
         L.lwo_render_hq_counted.argtypes = [vp, u32, ctypes.c_uint, vp, vp]
         L.lwo_render_hq_counted.restype = i32
+        L.lwo_render_diff.argtypes = [vp, u32, vp, ctypes.c_uint, vp]
+        L.lwo_render_diff.restype = i32
+        L.lwo_adjoint.argtypes = [vp, u32, vp, vp, ctypes.c_uint, vp]
+        L.lwo_adjoint.restype = i32
+        L.lwo_render_diff_dot.argtypes = [vp, u32, vp, vp, ctypes.c_uint, vp]
+        L.lwo_render_diff_dot.restype = i32
+        L.lwo_sky_tangent.argtypes = [vp, vp, i32, vp, sz, vp]
+        L.lwo_sky_tangent.restype = i32
         L.lwo_preetham_radiance.argtypes = [vp, vp, sz, vp]
         L.lwo_preetham_radiance.restype = i32
         L.lwo_rgb_to_hsl.argtypes = [vp, sz, vp]
@@ -208,6 +216,43 @@ def render_samples(view, px, py, sample, threads: int = 0):
     threads = threads or os.cpu_count() or 1
     _check(lib().lwo_render_samples(view.ref, n, _addr(px), _addr(py), _addr(sample), _addr(out),
                                     threads))
+    return out
+
+
+def render_diff(view, spp: int, desc, threads: int = 0):
+    """The CPU reference of lw_hip_render_diff: (h, w, 3) f32 mean radiance under the
+    parameters of `desc` (a lumenwright_amd.abi.DiffDescC)."""
+    w, h = int(view.params["frame"][0]), int(view.params["frame"][1])
+    out = np.zeros((h, w, 3), dtype="<f4")
+    threads = threads or os.cpu_count() or 1
+    _check(lib().lwo_render_diff(view.ref, spp, ctypes.byref(desc), threads, _addr(out)))
+    return out
+
+
+def render_diff_dot(view, spp: int, desc, weights, threads: int = 0) -> float:
+    """sum(weights * image) of render_diff, the image kept in f64."""
+    g = np.ascontiguousarray(weights, dtype="<f4").reshape(-1)
+    out = ctypes.c_double()
+    threads = threads or os.cpu_count() or 1
+    _check(lib().lwo_render_diff_dot(view.ref, spp, ctypes.byref(desc), _addr(g), threads,
+                                     ctypes.byref(out)))
+    return out.value
+
+
+def adjoint(view, spp: int, desc, n_grads: int, dldi, threads: int = 0):
+    """The CPU reference gradient (path replay) of sum(dldi * image): (n_grads,) f64."""
+    g = np.ascontiguousarray(dldi, dtype="<f4").reshape(-1)
+    out = np.zeros(n_grads, dtype="<f8")
+    threads = threads or os.cpu_count() or 1
+    _check(lib().lwo_adjoint(view.ref, spp, ctypes.byref(desc), _addr(g), threads, _addr(out)))
+    return out
+
+
+def sky_tangent(view, desc, k: int, d):
+    """d sky_radiance(d) / d(sky parameter k) through the desc's derivative tables."""
+    d = _f32(d).reshape(-1, 3)
+    out = np.zeros_like(d)
+    _check(lib().lwo_sky_tangent(view.ref, ctypes.byref(desc), k, _addr(d), d.shape[0], _addr(out)))
     return out
 
 

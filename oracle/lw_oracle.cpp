@@ -5,6 +5,7 @@
 #define LWO_NS lwo
 #define LWO_COUNTING 0
 #include "lw_oracle_impl.hpp"
+#include "lw_oracle_diff.hpp"
 
 #include <cstdio>
 
@@ -248,6 +249,47 @@ uint32_t lwo_water_interface(const Params *p, const float *pos, const float *d, 
   o_out[0] = oo.x; o_out[1] = oo.y; o_out[2] = oo.z;
   *w_out = w ? 1 : 0;
   return rng.state;
+}
+
+
+// Differentiable rendering's CPU reference (lw_oracle_diff.hpp): the image under the
+// parameters of a lw_diff_desc, and the gradient of sum(dldi * image).
+int lwo_render_diff(const View *view, uint32_t spp, const DiffDesc *diff, unsigned threads,
+                    float *out_rgb) {
+  if (threads == 0) threads = 1;
+  return guarded([&] { render_diff(*view, spp, *diff, threads, out_rgb); });
+}
+
+// sum(weights * image) of render_diff with the image in f64 (finite differences).
+int lwo_render_diff_dot(const View *view, uint32_t spp, const DiffDesc *diff, const float *weights,
+                        unsigned threads, double *out) {
+  if (threads == 0) threads = 1;
+  return guarded([&] {
+    std::vector<float> img((size_t)view->params->frame[0] * view->params->frame[1] * 3);
+    render_diff(*view, spp, *diff, threads, img.data(), weights, out);
+  });
+}
+
+// d sky_radiance(d) / d(sky parameter k) for n directions (lw_oracle_diff.hpp).
+int lwo_sky_tangent(const View *view, const DiffDesc *diff, int k, const float *d, size_t n,
+                    float *out) {
+  return guarded([&] {
+    if (k < 0 || k > 2) throw OracleError("k is 0, 1 or 2");
+    check_diff(*view, *diff);
+    const DiffCtx C{*diff, view->quads_len, cache_len_of(*view), nullptr};
+    for (size_t i = 0; i < n; i++) {
+      V3 c = sky_radiance_tangent(*view, C, k, v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+      out[3 * i] = c.x;
+      out[3 * i + 1] = c.y;
+      out[3 * i + 2] = c.z;
+    }
+  });
+}
+
+int lwo_adjoint(const View *view, uint32_t spp, const DiffDesc *diff, const float *dldi,
+                unsigned threads, double *grads) {
+  if (threads == 0) threads = 1;
+  return guarded([&] { adjoint(*view, spp, *diff, dldi, threads, grads); });
 }
 
 }  // extern "C"

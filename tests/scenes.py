@@ -236,3 +236,51 @@ def normalize32(v):
     d = np.float32(v[0] * v[0] + v[1] * v[1]) + np.float32(v[2] * v[2])
     l = np.sqrt(np.float32(d), dtype=np.float32)
     return (v * (np.float32(1.0) / l)).astype(np.float32)
+
+
+def floor_scene(sun_altitude: float, sun_azimuth: float, sun_radius: float, width: int = 32,
+                height: int = 32):
+    """A white 64 x 64 floor (the top of a layer of cubes at y = 0) under the sun alone: no
+    sky light (a sky cache of zeros), one bounce, a camera 11 blocks above looking down.
+    The image is sun_color * E[max(n.l, 0)] over the sun's cone, a closed form:
+    (1 + cos a) / 2 * sin(altitude) for a cone of half-angle a wholly above the floor's
+    horizon. Returns the Scene and d sun_dir / d(altitude, azimuth)."""
+    from lumenwright_amd.abi import FLAG_FRONT_ONLY
+    quads = cube(ALPHA_OPAQUE | FLAG_FRONT_ONLY)
+    for q in quads:
+        q["uv"][2] = 1.0  # texel 0 only: white
+    models = [model(0, 0), model(0, 6)]
+    brick = np.zeros(BRICK_WORDS, dtype="<u4")
+    for z in range(16):
+        for x in range(16):
+            local = z * 16 + x
+            brick[local >> 1] |= 1 << ((local & 1) * 16)
+    nb = 16
+    grid = [0] * nb
+    off_bricks = len(grid); grid += list(brick)
+    off_ov_top = len(grid); grid += [EMPTY] * nb
+    off_ov_cells = len(grid)
+    grid.append(0)
+    alt, az = np.float64(sun_altitude), np.float64(sun_azimuth)
+    s = np.array([np.sin(az) * np.cos(alt), np.sin(alt), -np.cos(az) * np.cos(alt)])
+    d_alt = np.array([-np.sin(az) * np.sin(alt), np.cos(alt), np.cos(az) * np.sin(alt)])
+    d_az = np.array([np.cos(az) * np.cos(alt), 0.0, np.sin(az) * np.cos(alt)])
+    n_cache = 4
+    p = new_params()
+    p["grid_dims"] = [4, 1, 4, nb]
+    p["off"] = [0, off_bricks, off_ov_top, off_ov_cells]
+    p["off2"] = [off_ov_cells, 2, 1, len(quads)]
+    p["misc_f"] = [0.0, 1e-3, float(n_cache), 0.0]
+    p["misc_u"] = [3, 1, 2, 0]
+    p["frame"] = [width, height, 0, 1]
+    p["cam_pos"] = [32.0, 12.0, 32.0, 0.4]
+    p["cam_fwd"] = [0.0, -1.0, 0.0, width / height]
+    p["cam_right"] = [1.0, 0.0, 0.0, 0.0]
+    p["cam_up"] = [0.0, 0.0, -1.0, 0.0]
+    p["sun_dir"] = [s[0], s[1], s[2], np.cos(np.float64(sun_radius))]
+    p["sun_color"] = [1.0, 1.0, 1.0, 0.0]
+    p["sky_sun"] = [s[0], s[1], s[2], 1.0]
+    sky = np.zeros((n_cache + 1) ** 2 * 3, dtype="<f4")
+    scene = Scene(p, grid, models, quads, [0xFFFFFFFF, 0x00000000], np.zeros(1, EMITTER),
+                  np.zeros(EMITTER_GRID_HEADER, "<u4"), sky)
+    return scene, np.stack([d_alt, d_az])

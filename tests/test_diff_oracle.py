@@ -1,0 +1,173 @@
+"""The CPU reference of differentiable rendering (oracle/lw_oracle_diff.hpp), pinned
+without the device:
+
+  * at identity parameters its forward is the oracle's render_hq bit for bit (the
+    restatement of the reference's CPU backend);
+  * its gradient agrees with central finite differences of its own forward: palette and
+    intensity factors (step 1e-3), turbidity (step 3e-4; at 1e-3 both sides straddle kinks
+    of the HSL cache, |2l - 1| and the c <= 1 clamp of integrator.rs:108-127, and part by
+    0.4%), with Russian roulette off so that the forward is smooth in them;
+  * the sky tangent agrees with finite differences of sky_radiance over the sphere;
+  * the sun angles' gradient (the moving cone's rim estimate) meets the closed form of a
+    sunlit floor: d/d altitude of (1 + cos a) / 2 * sin(altitude) = (1 + cos a) / 2 *
+    cos(altitude), and 0 for the azimuth.
+"""
+
+import numpy as np
+import pytest
+
+import scenes
+from lumenwright_amd import diff, frames
+from lumenwright_amd.abi import NO_GROUP, EMITTER_SAMPLING_ONE
+
+
+@pytest.fixture(scope="module")
+def problem():
+    return diff.config3_problem()
+
+
+def small(problem, w=48, h=27, spp=8):
+    cfg = frames.config(3, width=w, height=h, spp=spp)
+    return cfg, diff.frame_at(cfg, problem.scene, problem.start.sky)
+
+
+def test_config3_world_has_its_groups(problem):
+    qp, qi = problem.start.quad_palette, problem.start.quad_intensity
+    assert sorted(np.unique(qp[qp != NO_GROUP])) == list(range(32))
+    assert sorted(np.unique(qi[qi != NO_GROUP])) == list(range(16))
+    assert problem.scene.emitter_count == 16
+    assert problem.start.n_grads == 96 + 16 + 3
+    # every emitter block's six faces carry emittance 1
+    em = qi != NO_GROUP
+    assert np.all(problem.scene.quads["ev"][em, 3] == 1.0)
+
+
+def test_identity_parameters_give_render_hq(oracle, problem):
+    cfg, fr = small(problem, 64, 36, 4)
+    v = fr.view()
+    a = oracle.render_hq(v, 4)
+    b = oracle.render_diff(v, 4, problem.start.desc())
+    assert np.array_equal(a.view("<u4"), b.view("<u4"))
+    assert fr.params["emitter"][0] == EMITTER_SAMPLING_ONE and a.mean() > 0
+
+
+def test_identity_parameters_give_render_hq_with_water_and_plants(oracle):
+    cfg = frames.config(1, width=40, height=24, spp=2, world_chunks_x=4, world_chunks_z=4)
+    fr = frames.make_frame(cfg)
+    sc = fr.scene
+    p = diff.identity(diff.block_type_groups(sc), diff.emitter_groups(sc, 8),
+                      int(sc.quad_type[sc.quad_type != NO_GROUP].max()) + 1, 8)
+    v = fr.view()
+    a = oracle.render_hq(v, 2)
+    b = oracle.render_diff(v, 2, p.desc())
+    assert np.array_equal(a.view("<u4"), b.view("<u4"))
+
+
+def smooth(p):
+    """A copy of p with Russian roulette off."""
+    q = p.with_values()
+    q.rr_off = True
+    return q
+
+
+def jvp_fd(oracle, problem, cfg, fr, G, which, h):
+    """Central differences of sum(G * image) in each parameter of `which`."""
+    start = smooth(problem.start)
+
+    def J(p, frame):
+        return oracle.render_diff_dot(frame.view(), cfg.spp, p.desc(), G)
+
+    out = {}
+    if which == "palette":
+        for i in range(3 * start.n_palette):
+            pp, pm = start.palette.ravel().copy(), start.palette.ravel().copy()
+            pp[i] += h
+            pm[i] -= h
+            out[i] = (J(smooth(start.with_values(palette=pp)), fr)
+                      - J(smooth(start.with_values(palette=pm)), fr)) / (2 * h)
+    elif which == "intensity":
+        for j in range(start.n_intensity):
+            ip, im = start.intensity.copy(), start.intensity.copy()
+            ip[j] += h
+            im[j] -= h
+            out[3 * start.n_palette + j] = (J(smooth(start.with_values(intensity=ip)), fr)
+                                            - J(smooth(start.with_values(intensity=im)), fr)) / (2 * h)
+    else:
+        e = np.array([h, 0.0, 0.0])
+        fp = diff.frame_at(cfg, problem.scene, start.sky + e)
+        fm = diff.frame_at(cfg, problem.scene, start.sky - e)
+        out[start.n_grads - 3] = (J(start, fp) - J(start, fm)) / (2 * h)
+    return out
+
+
+@pytest.fixture(scope="module")
+def adjoint_cpu(oracle, problem):
+    cfg, fr = small(problem)
+    G = np.random.default_rng(1).standard_normal(cfg.width * cfg.height * 3).astype("<f4")
+    derivs = diff.sky_derivatives(cfg, problem.scene, problem.start.sky, 3e-4)
+    start = smooth(problem.start)
+    adj = oracle.adjoint(fr.view(), cfg.spp, start.desc(derivs), start.n_grads, G)
+    return cfg, fr, G, adj
+
+
+@pytest.mark.parametrize("which,h,tol", [("palette", 1e-3, 1e-3), ("intensity", 1e-3, 1e-3),
+                                         ("turbidity", 3e-4, 2e-3)])
+def test_cpu_gradient_matches_central_differences(oracle, problem, adjoint_cpu, which, h, tol):
+    cfg, fr, G, adj = adjoint_cpu
+    fd = jvp_fd(oracle, problem, cfg, fr, G, which, h)
+    idx = np.array(sorted(fd))
+    f = np.array([fd[i] for i in idx])
+    a = adj[idx]
+    floor = 1e-2 * np.abs(f).max()
+    rel = np.abs(a - f) / np.maximum(np.abs(f), floor)
+    print(which, "worst rel", rel.max(), "max |fd|", np.abs(f).max())
+    assert np.abs(f).max() > 0.1
+    assert rel.max() <= tol
+
+
+def test_sky_tangent_matches_differences_of_sky_radiance(oracle, problem):
+    cfg, fr = small(problem)
+    h = 3e-4
+    derivs = diff.sky_derivatives(cfg, problem.scene, problem.start.sky, h)
+    desc = problem.start.desc(derivs)
+    d = np.random.default_rng(0).standard_normal((20000, 3))
+    d[:, 1] = np.abs(d[:, 1])
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype("<f4")
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = h
+        fp = diff.frame_at(cfg, problem.scene, problem.start.sky + e)
+        fm = diff.frame_at(cfg, problem.scene, problem.start.sky - e)
+        fd = (oracle.sky_radiance(fp.view(), d).astype("<f8") - oracle.sky_radiance(fm.view(), d)) / (2 * h)
+        t = oracle.sky_tangent(fr.view(), desc, k, d).astype("<f8")
+        assert abs(t.sum() - fd.sum()) <= 1e-3 * np.abs(fd).sum()
+        close = np.all(np.abs(t - fd) <= 1e-2 * np.abs(fd).max(), axis=1)
+        assert close.mean() > 0.995, (k, close.mean())
+
+
+def floor_desc(dsun):
+    p = diff.identity(np.full(6, NO_GROUP, "<u4"), np.full(6, NO_GROUP, "<u4"), 0, 0)
+    derivs = diff.SkyDerivs(np.zeros((3, 75), "<f4"),
+                            np.vstack([np.zeros(3), dsun]), np.zeros((3, 3)), np.zeros(3))
+    return p, p.desc(derivs)
+
+
+@pytest.mark.parametrize("alt,az", [(0.9, 0.4), (1.3, 2.5)])
+def test_sun_angle_gradient_meets_the_closed_form(oracle, alt, az):
+    radius = 0.05
+    s, dsun = scenes.floor_scene(alt, az, radius)
+    v = s.view()
+    w, h = int(s.params["frame"][0]), int(s.params["frame"][1])
+    spp = 64  # the rim estimate's relative spread is about 0.5% at this many samples
+    p, desc = floor_desc(dsun)
+    img = oracle.render_hq(v, spp)
+    cos_a = float(np.float32(np.cos(radius)))
+    want_img = (1 + cos_a) / 2 * np.sin(alt)
+    assert abs(img.mean() - want_img) < 2e-3 * want_img
+    G = np.ones(w * h * 3, "<f4")
+    g = oracle.adjoint(v, spp, desc, p.n_grads, G)
+    want = w * h * 3 * (1 + cos_a) / 2 * np.cos(alt)
+    print("altitude", g[-2], want, "azimuth", g[-1])
+    assert abs(g[-2] - want) <= 0.02 * abs(want)
+    assert abs(g[-1]) <= 0.02 * abs(want)
+    assert g[-3] == 0.0
