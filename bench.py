@@ -75,6 +75,10 @@ def cpu_baseline(frame, seconds: float):
 
 
 def roofline(config, samples, trace_ms, launches):
+    """The dominant kernel against HBM: achieved = algorithmic scene bytes per launch (the
+    instrumented oracle's bytes per sample times the launch's samples) over the mean launch
+    time; traffic = the HBM bytes per launch the FETCH_SIZE / WRITE_SIZE counters measured for
+    the same kernel and workload (profiles/traffic.json), when this run is that kernel."""
     b_alg_path = ROOT / "profiles" / "b_alg.json"
     if not b_alg_path.exists():
         return None
@@ -84,10 +88,18 @@ def roofline(config, samples, trace_ms, launches):
     per_launch = samples / launches
     mean_s = trace_ms / 1e3 / launches
     achieved = b_alg["bytes_per_sample"] * per_launch / mean_s / 1e9
+    traffic = None
+    traffic_path = ROOT / "profiles" / "traffic.json"
+    if traffic_path.exists() and kernel_form() == "direct, waves 5":
+        t = json.loads(traffic_path.read_text()).get(f"config{config}")
+        if t:
+            traffic = t["hbm_bytes_per_sample"] * per_launch
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)",
-            "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"]}
+            "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"],
+            "note": "VALU-issue bound: 99% VALU busy at 15% lane utilisation, L2 hit 94% "
+                    "(profiles/k_render_cfg1_720p_valu_l2.md)" if config == 1 else None}
 
 
 def kernel_form():
@@ -283,12 +295,88 @@ def bench_diff(args, lw, frames):
     }
 
 
+def bench_fit(args, lw, frames):
+    """cfg4: the batched inverse fit. 32 seeded views of the cfg1 world (640x360, 128 spp,
+    6 bounces); targets rendered with seeded ground-truth factors (192 albedo factors of the
+    64 block types, 128 emitter-intensity groups); Adam on the device from factors of 1. A
+    step is one batched forward, the loss and dL/dI, one batched adjoint (path replay) and the
+    update, all on the device."""
+    from lumenwright_amd import diff
+    t0 = time.perf_counter()
+    spp = args.spp or 128
+    bp = diff.config4_problem(spp=spp)
+    log(f"problem built in {time.perf_counter() - t0:.1f} s: {len(bp.views)} views, "
+        f"{bp.start.n_palette} palettes, {bp.start.n_intensity} intensity groups")
+    b = lw.HipBackend.open(0)
+    sc = b.upload(bp.base)
+    t = time.perf_counter()
+    targets = b.render_diff_batch(sc, bp.views, bp.truth.desc())
+    log(f"targets rendered in {time.perf_counter() - t:.1f} s")
+    fit = b.fit(sc, bp.views, bp.start.desc(), targets, lr=1e-2)
+    e_start = diff.parameter_error(bp.start, bp.truth)
+    for i in range(args.warmup):
+        st = fit.step()
+        log(f"warmup {i + 1}: {st}")
+    steps = []
+    t = time.perf_counter()
+    n_steps = max(args.steps, args.fit_steps)
+    for i in range(n_steps):
+        st = fit.step()
+        steps.append(st)
+        if i < args.steps or (i + 1) % 10 == 0:
+            log(f"step {i + 1}/{n_steps}: loss {st['loss']:.6g}, forward {st['forward_ms']:.0f} ms, "
+                f"adjoint {st['adjoint_ms']:.0f} ms, wall {st['wall_ms']:.0f} ms")
+    wall = time.perf_counter() - t
+    pal, inten, _ = fit.params()
+    e_end = diff.parameter_error(bp.start.with_values(pal, inten), bp.truth)
+    fit.free()
+    sc.free()
+    b.close()
+    timed = steps[:args.steps]
+    fwd = sum(s["forward_ms"] for s in timed)
+    adj = sum(s["adjoint_ms"] for s in timed)
+    step_wall = sum(s["wall_ms"] for s in timed) / len(timed)
+    samples = timed[0]["samples"]
+    return {
+        "metric": METRIC,
+        "value": samples / (step_wall / 1e3) / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 path arithmetic, f64 accumulation and gradient sums",
+        "data": "synthetic (seeded value-noise voxel world, game dusk sky maps)",
+        "config": {"workload": "cfg4: batched inverse fit, 32 seeded views of the cfg1 world, "
+                               f"640x360, {spp} spp, 6 bounces; 192 albedo factors and 128 "
+                               "emitter-intensity groups; Adam lr 1e-2 on the device",
+                   "views": len(bp.views), "spp": spp, "parallelism": "1 GPU"},
+        "value_is": "view samples per second of a whole fit step (forward, loss, adjoint, update)",
+        "forward_msamples_per_s": samples * len(timed) / (fwd / 1e3) / 1e6,
+        "adjoint_msamples_per_s": samples * len(timed) / (adj / 1e3) / 1e6,
+        "adjoint_over_forward": adj / fwd,
+        "fit_steps": n_steps,
+        "loss_first": steps[0]["loss"],
+        "loss_last": steps[-1]["loss"],
+        "param_error_start": {"albedo": e_start[0], "intensity": e_start[1]},
+        "param_error_end": {"albedo": e_end[0], "intensity": e_end[1]},
+        "fit_wall_s": wall,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4])
+    ap.add_argument("--fit-steps", type=int, default=0,
+                    help="config 4: run this many fit steps in all (the first --steps are timed)")
     ap.add_argument("--spp", type=int, default=None, help="override the workload's spp")
     ap.add_argument("--spp-scale", type=float, default=1.0, help="config 2: scale every view's spp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -308,6 +396,8 @@ def main():
         line = bench_batch(args, lw, frames)
     elif args.config == 3:
         line = bench_diff(args, lw, frames)
+    elif args.config == 4:
+        line = bench_fit(args, lw, frames)
     else:
         line = bench_plain(args, lw, frames)
     print(json.dumps(line), flush=True)

@@ -281,9 +281,52 @@ typedef struct lw_diff_desc {
 } lw_diff_desc;
 
 /* The image of params under diff's parameters: w*h*3 floats of mean radiance, as
- * lw_hip_render_hq_scene. */
+ * lw_hip_render_hq_scene; out_f64 (optional, w*h*3 doubles) receives the same means before
+ * their rounding to f32, for finite differences and losses that need them. */
 int lw_hip_render_diff(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
-                       uint32_t spp, const lw_diff_desc *diff, float *out_rgb);
+                       uint32_t spp, const lw_diff_desc *diff, float *out_rgb, double *out_f64);
+
+/* The views of a batch (lw_hip_render_hq_batch's rules) under diff's parameters, in one
+ * work pool: out_rgb[i] gets view i's image, as lw_hip_render_diff would give it alone. */
+int lw_hip_render_diff_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
+                             size_t n, const lw_diff_desc *diff, float *const *out_rgb);
+
+/* The adjoint of lw_hip_render_diff_batch: grads = sum over views, pixels and channels of
+ * d_loss_d_image[i] * dI_i/dtheta (one w*h*3 array per view). Palette and intensity
+ * parameters only (sky_params 0 or 1 with the views' shared Preetham sky). */
+int lw_hip_adjoint_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
+                         size_t n, const lw_diff_desc *diff, const float *const *d_loss_d_image,
+                         double *grads);
+
+/* An inverse fit on the device: the views of a batch, their target images, and the palette
+ * and intensity factors of a lw_diff_desc fitted to them by Adam (D. P. Kingma, J. Ba, 2015)
+ * on L = sum over views, pixels and channels of (I - T)^2 / N. A step renders the batch,
+ * forms L and dL/dI, replays the batch for the gradient and updates the factors, all on the
+ * device (factors are kept >= min_factor). The views keep their seeds from step to step. */
+typedef struct lw_fit_options {
+  float lr, beta1, beta2, eps, min_factor;
+  uint32_t pad[3];
+} lw_fit_options;
+typedef struct lw_fit_stats {
+  double loss;        /* L at the factors the step started from */
+  double forward_ms;  /* HIP-event time of the step's forward path kernels */
+  double adjoint_ms;  /* HIP-event time of its adjoint kernels */
+  double wall_ms;
+  uint64_t samples;   /* samples of one forward (the adjoint replays as many) */
+  uint32_t step;
+  uint32_t pad;
+} lw_fit_stats;
+typedef struct lw_hip_fit lw_hip_fit;
+int lw_hip_fit_new(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views, size_t n,
+                   const lw_diff_desc *diff, const float *const *targets,
+                   const lw_fit_options *opt, lw_hip_fit **out);
+int lw_hip_fit_step(lw_hip_backend *b, const lw_hip_scene *scene, lw_hip_fit *fit,
+                    lw_fit_stats *stats);
+/* The current factors (3 n_palette and n_intensity floats; either may be null) and the last
+ * step's gradient (3 n_palette + n_intensity + 3 doubles; may be null). */
+int lw_hip_fit_params(lw_hip_backend *b, const lw_hip_fit *fit, float *palette,
+                      float *intensity, double *grads);
+void lw_hip_fit_free(lw_hip_backend *b, lw_hip_fit *fit);
 
 /* The adjoint of lw_hip_render_diff with the same arguments: replays every sample from its
  * seed and writes grads[i] = sum over pixels and channels of d_loss_d_image * dI/dtheta_i

@@ -37,8 +37,14 @@ SIGNATURES = {
     "lw_hip_last_timing": (i32, [vp, vp]),
     "lw_hip_render_hq_batch": (i32, [vp, vp, vp, sz, PROGRESS_FN, vp, vp]),
     "lw_hip_copy_probe": (i32, [vp, sz, ctypes.POINTER(f64)]),
-    "lw_hip_render_diff": (i32, [vp, vp, vp, u32, vp, vp]),
+    "lw_hip_render_diff": (i32, [vp, vp, vp, u32, vp, vp, vp]),
     "lw_hip_adjoint": (i32, [vp, vp, vp, u32, vp, vp, vp]),
+    "lw_hip_render_diff_batch": (i32, [vp, vp, vp, sz, vp, vp]),
+    "lw_hip_adjoint_batch": (i32, [vp, vp, vp, sz, vp, vp, vp]),
+    "lw_hip_fit_new": (i32, [vp, vp, vp, sz, vp, vp, vp, ctypes.POINTER(vp)]),
+    "lw_hip_fit_step": (i32, [vp, vp, vp, vp]),
+    "lw_hip_fit_params": (i32, [vp, vp, vp, vp, vp]),
+    "lw_hip_fit_free": (None, [vp, vp]),
     # host.h
     "lw_celestial_angle": (f32, [ctypes.c_int64]),
     "lw_sun_direction_from_angle": (None, [f32, vp]),

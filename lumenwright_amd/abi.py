@@ -111,6 +111,22 @@ class DiffDescC(ctypes.Structure):
     ]
 
 
+class FitOptionsC(ctypes.Structure):
+    """lw_fit_options: Adam's step size and moment decays, its epsilon, the factors' floor."""
+
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("min_factor", ctypes.c_float),
+                ("pad", ctypes.c_uint32 * 3)]
+
+
+class FitStatsC(ctypes.Structure):
+    """lw_fit_stats."""
+
+    _fields_ = [("loss", ctypes.c_double), ("forward_ms", ctypes.c_double),
+                ("adjoint_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("samples", ctypes.c_uint64), ("step", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
 class TimingC(ctypes.Structure):
     """lw_timing."""
 

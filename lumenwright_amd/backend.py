@@ -14,7 +14,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _native
-from .abi import HIT, PARAMS, VIEW, BoundView, HqFrame, TimingC
+from .abi import HIT, PARAMS, VIEW, BoundView, FitOptionsC, FitStatsC, HqFrame, TimingC
 
 LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
 
@@ -136,14 +136,58 @@ class HipBackend:
                                                          len(views), cb, None, ptrs))
         return outs
 
-    def render_diff(self, scene: HipScene, params: np.ndarray, spp: int, desc) -> np.ndarray:
-        """lw_hip_render_diff: the image under a lw_diff_desc's parameters, (w*h, 3) f32."""
+    def render_diff(self, scene: HipScene, params: np.ndarray, spp: int, desc,
+                    f64: bool = False):
+        """lw_hip_render_diff: the image under a lw_diff_desc's parameters, (w*h, 3) f32; with
+        f64, also the means before their rounding to f32."""
         p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
         w, h = int(p["frame"][0]), int(p["frame"][1])
         out = np.zeros((w * h, 3), dtype="<f4")
-        self._check(_native.lib().lw_hip_render_diff(self.handle, scene.handle, _addr(p), spp,
-                                                     ctypes.byref(desc), _addr(out)))
+        out64 = np.zeros((w * h, 3), dtype="<f8") if f64 else None
+        self._check(_native.lib().lw_hip_render_diff(
+            self.handle, scene.handle, _addr(p), spp, ctypes.byref(desc), _addr(out),
+            _addr(out64) if f64 else None))
+        return (out, out64) if f64 else out
+
+    def _views(self, views):
+        arr = np.zeros(len(views), dtype=VIEW)
+        for i, (params, spp) in enumerate(views):
+            arr[i]["params"] = params
+            arr[i]["spp"] = spp
+        return arr
+
+    def render_diff_batch(self, scene: HipScene, views, desc) -> list:
+        """lw_hip_render_diff_batch over [(params, spp), ...]; one (w*h, 3) f32 per view."""
+        arr = self._views(views)
+        outs = [np.zeros((int(p["frame"][0]) * int(p["frame"][1]), 3), dtype="<f4") for p, _ in views]
+        ptrs = (ctypes.c_void_p * len(views))(*[o.ctypes.data for o in outs])
+        self._check(_native.lib().lw_hip_render_diff_batch(self.handle, scene.handle, _addr(arr),
+                                                           len(views), ctypes.byref(desc), ptrs))
+        return outs
+
+    def adjoint_batch(self, scene: HipScene, views, desc, n_grads: int, dldis) -> np.ndarray:
+        """lw_hip_adjoint_batch: the gradient of sum over views of dldis[i] . I_i."""
+        arr = self._views(views)
+        ds = [np.ascontiguousarray(d, dtype="<f4").reshape(-1) for d in dldis]
+        ptrs = (ctypes.c_void_p * len(views))(*[d.ctypes.data for d in ds])
+        out = np.zeros(n_grads, dtype="<f8")
+        self._check(_native.lib().lw_hip_adjoint_batch(self.handle, scene.handle, _addr(arr),
+                                                       len(views), ctypes.byref(desc), ptrs,
+                                                       _addr(out)))
         return out
+
+    def fit(self, scene: HipScene, views, desc, targets, lr: float = 1e-2, beta1: float = 0.9,
+            beta2: float = 0.999, eps: float = 1e-8, min_factor: float = 1e-3) -> "HipFit":
+        """lw_hip_fit_new: an inverse fit of desc's factors to the views' target images."""
+        arr = self._views(views)
+        ts = [np.ascontiguousarray(t, dtype="<f4").reshape(-1) for t in targets]
+        ptrs = (ctypes.c_void_p * len(views))(*[t.ctypes.data for t in ts])
+        opt = FitOptionsC(lr, beta1, beta2, eps, min_factor)
+        h = ctypes.c_void_p()
+        self._check(_native.lib().lw_hip_fit_new(self.handle, scene.handle, _addr(arr), len(views),
+                                                 ctypes.byref(desc), ptrs, ctypes.byref(opt),
+                                                 ctypes.byref(h)))
+        return HipFit(self, scene, h.value, desc.n_palette, desc.n_intensity)
 
     def adjoint(self, scene: HipScene, params: np.ndarray, spp: int, desc, n_grads: int,
                 d_loss_d_image: np.ndarray) -> np.ndarray:
@@ -200,3 +244,38 @@ class HipBackend:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class HipFit:
+    """An inverse fit resident on the device (lw_hip_fit_*)."""
+
+    def __init__(self, backend: HipBackend, scene: HipScene, handle: int, n_pal: int, n_int: int):
+        self.backend, self.scene, self.handle = backend, scene, handle
+        self.n_pal, self.n_int = n_pal, n_int
+
+    def step(self) -> dict:
+        st = FitStatsC()
+        self.backend._check(_native.lib().lw_hip_fit_step(self.backend.handle, self.scene.handle,
+                                                          self.handle, ctypes.byref(st)))
+        return {"loss": st.loss, "forward_ms": st.forward_ms, "adjoint_ms": st.adjoint_ms,
+                "wall_ms": st.wall_ms, "samples": int(st.samples), "step": int(st.step)}
+
+    def params(self):
+        pal = np.zeros((self.n_pal, 3), dtype="<f4")
+        inten = np.zeros(self.n_int, dtype="<f4")
+        grads = np.zeros(3 * self.n_pal + self.n_int + 3, dtype="<f8")
+        self.backend._check(_native.lib().lw_hip_fit_params(
+            self.backend.handle, self.handle, _addr(pal) if pal.size else None,
+            _addr(inten) if inten.size else None, _addr(grads)))
+        return pal, inten, grads
+
+    def free(self):
+        if self.handle:
+            _native.lib().lw_hip_fit_free(self.backend.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

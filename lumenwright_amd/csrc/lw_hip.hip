@@ -580,6 +580,126 @@ __global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const 
   }
 }
 
+// A batch pass's view, tile and sample slice of workgroup `base` (k_render_batch's layout).
+struct BatchItem {
+  uint32_t v, sl, px, py;
+  bool in_frame;
+};
+__device__ __forceinline__ BatchItem batch_item(const BatchDev *__restrict__ bd, uint32_t base) {
+  BatchItem it;
+  it.v = find_view(bd->item_prefix, bd->nviews, base);
+  const uint32_t rest = (base - bd->item_prefix[it.v]) >> 8;
+  const uint32_t tiles = bd->tiles[it.v], tx = bd->tiles_x[it.v];
+  const uint32_t tile = rest % tiles;
+  it.sl = rest / tiles;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  it.px = (tile % tx) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  it.py = (tile / tx) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  it.in_frame = it.px < bd->cam[it.v].width && it.py < bd->cam[it.v].height;
+  return it;
+}
+
+// The differentiable forward of a batch pass (k_render_batch with render_sample_diff).
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_render_diff_batch(const lw_params p, const DevScene s,
+                                                              const DiffDev D,
+                                                              const BatchDev *__restrict__ bd,
+                                                              float *__restrict__ rad) {
+  const uint32_t base = blockIdx.x * (uint32_t)kBlock;
+  if (base >= bd->total) return;
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const BatchItem it = batch_item(bd, base);
+  if (!it.in_frame) return;
+  const ViewCam cam = bd->cam[it.v];
+  const GradSink G{nullptr, 0u, 0u};
+  const SuffixList sfx{nullptr, 0u};
+  const V3 c = render_sample_diff<false>(s, p, cam, lut_s, D, it.px, it.py, bd->start[it.v] + it.sl,
+                                         G, sfx, V3{0.0f, 0.0f, 0.0f});
+  float *o = rad + bd->rad_base[it.v] +
+             ((size_t)it.sl * cam.height * cam.width + (size_t)it.py * cam.width + it.px) * 3;
+  o[0] = c.x;
+  o[1] = c.y;
+  o[2] = c.z;
+}
+
+// The adjoint of a batch pass: D.dldi holds every view's dL/dI, view v's pixels from
+// pix_prefix[v]; each sample's dl is its pixel's dL/dI / the view's spp.
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_adjoint_batch(const lw_params p, const DevScene s,
+                                                          const DiffDev D,
+                                                          const BatchDev *__restrict__ bd) {
+  const uint32_t base = blockIdx.x * (uint32_t)kBlock;
+  if (base >= bd->total) return;  // uniform over the workgroup: no barrier is skipped by part
+  __shared__ float lut_s[256];
+  __shared__ uint32_t sfx_s[kMaxSuffix * kBlock];
+  extern __shared__ double grad_s[];
+  const uint32_t ns = diff_slots(D);
+  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) grad_s[i] = 0.0;
+  load_lut(lut_s, s.gamma_lut);  // ends with a barrier
+  const BatchItem it = batch_item(bd, base);
+  if (it.in_frame) {
+    const ViewCam cam = bd->cam[it.v];
+    const float *g = D.dldi + ((size_t)bd->pix_prefix[it.v] + (size_t)it.py * cam.width + it.px) * 3;
+    const V3 dl = V3{g[0], g[1], g[2]} * (1.0f / (float)cam.spp);
+    if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
+      const GradSink G{grad_s, D.n_pal, D.n_int};
+      const SuffixList sfx{sfx_s + threadIdx.x, (uint32_t)kBlock};
+      render_sample_diff<true>(s, p, cam, lut_s, D, it.px, it.py, bd->start[it.v] + it.sl, G, sfx, dl);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
+    const double v = grad_s[i];
+    if (v != 0.0) atomicAdd(&D.grads[i], v);
+  }
+}
+
+// The L2 loss of a batch against its targets, L = sum over views, pixels and channels of
+// (I - T)^2 / N with I = the f64 sum / spp: dL/dI into dldi (f32) and L into *loss (f64,
+// one atomic per workgroup).
+__global__ void __launch_bounds__(kBlock) k_fit_loss(const BatchDev *__restrict__ bd,
+                                                     const double *__restrict__ acc,
+                                                     const float *__restrict__ target,
+                                                     float *__restrict__ dldi, double inv_n,
+                                                     double *__restrict__ loss) {
+  __shared__ double part[kBlock / 64];
+  const uint32_t g3 = blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (g3 < 3u * bd->npix_total) {
+    const uint32_t v = find_view(bd->pix_prefix, bd->nviews, g3 / 3u);
+    const double d = acc[g3] / (double)bd->cam[v].spp - (double)target[g3];
+    dldi[g3] = (float)(2.0 * d * inv_n);
+    sq = d * d * inv_n;
+  }
+  for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off);
+  if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; w++) t += part[w];
+    atomicAdd(loss, t);
+  }
+}
+
+// One Adam step (D. P. Kingma, J. Ba, "Adam: A Method for Stochastic Optimization", ICLR
+// 2015) on the factors x (f32) from their gradient g (f64), with moments m, v; x stays at
+// or above lo. bc1, bc2 = 1 - beta^t.
+__global__ void __launch_bounds__(kBlock) k_adam(float *__restrict__ x, const double *__restrict__ g,
+                                                 float *__restrict__ m, float *__restrict__ v,
+                                                 uint32_t n, float lr, float beta1, float beta2,
+                                                 float eps, float bc1, float bc2, float lo) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = (float)g[i];
+  const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
+  const float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float step = lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  x[i] = fmaxf(x[i] - step, lo);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------
@@ -681,6 +801,16 @@ struct lw_hip_scene {
   }
 };
 
+// A batch's views laid out for the device and its passes: as few as fit the staging cap,
+// each with fewer than 2^31 work items.
+struct BatchPlan {
+  std::vector<BatchDev> host = std::vector<BatchDev>(1);
+  uint32_t npass = 0;
+  uint64_t items = 0, staging = 0, npix_total = 0, samples = 0;
+  BatchDev &bd() { return host[0]; }
+  const BatchDev &bd() const { return host[0]; }
+};
+
 struct lw_hip_backend {
   int device = 0;
   std::string name;
@@ -719,6 +849,7 @@ struct lw_hip_backend {
   uint32_t batch_passes = 0;
   bool use_graphs = true;
   DevBuf<BatchDev> batch_dev;
+  BatchPlan plan;  // the last batch's layout (its sums stay in acc when it keeps them)
   // Differentiable rendering: the parameters and derivative inputs of the last call.
   DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
   DevBuf<uint32_t> diff_qg, diff_qig;
@@ -998,12 +1129,15 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
 }
 
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
+constexpr uint64_t kFillBlocks = 2048;     // 8 workgroups of 256 lanes per CU
 constexpr uint32_t kMaxBatch = 64;         // render/cuda.rs:165
 
 // With `diff`, the samples are render_sample_diff's under its parameters (direct form).
+// With `out_f64`, it also receives the per-pixel means in f64 (sum / spp before the f32
+// rounding of out_rgb).
 void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
                   uint32_t spp_in, lw_progress_fn progress, void *user, float *out_rgb,
-                  const DiffDev *diff = nullptr) {
+                  const DiffDev *diff = nullptr, double *out_f64 = nullptr) {
   auto t_wall = std::chrono::steady_clock::now();
   if (!(layout_key(params_in) == sc.key))
     throw InvalidScene{"params do not match the layout the scene was validated with"};
@@ -1033,7 +1167,11 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   const uint32_t tiles = tiles_x * ((h + 15) / 16);
   if ((uint64_t)tiles * 256u * max_batch > 0xFFFFFFFFull) throw InvalidScene{"frame too large"};
   lw_timing tm{};
+  // render/cuda.rs:131-175 starts at one sample per pixel and doubles while launches stay
+  // short. A frame too small to fill the GPU with one sample per pixel starts at the first
+  // power of two that gives it kFillBlocks workgroups (same samples, same summation order).
   uint32_t done = 0, batch = 1;
+  while (batch < max_batch && (uint64_t)tiles * batch < kFillBlocks) batch *= 2;
   while (done < spp) {
     const uint32_t nb = std::min(batch, std::min(spp - done, max_batch));
     const auto start = std::chrono::steady_clock::now();
@@ -1084,7 +1222,12 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   }
   HIP_TRY(hipMemcpyAsync(out_rgb, b->out.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost,
                          b->stream));
+  if (out_f64)
+    HIP_TRY(hipMemcpyAsync(out_f64, b->acc.ptr, npix * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                           b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  if (out_f64)
+    for (size_t i = 0; i < npix * 3; i++) out_f64[i] /= (double)spp;
   tm.samples = npix * spp;
   tm.wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
@@ -1107,22 +1250,19 @@ void check_batch_view(const lw_params &a, const lw_params &v) {
     throw InvalidScene{"batch views differ in more than camera, frame size and seed"};
 }
 
-void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *views, size_t nv,
-                  lw_progress_fn progress, void *user, float *const *outs) {
-  auto t_wall = std::chrono::steady_clock::now();
+BatchPlan plan_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *views, size_t nv) {
   if (nv == 0 || nv > kMaxViews) throw InvalidScene{"a batch holds 1 to 256 views"};
   const lw_params &p0 = views[0].params;
   if (!(layout_key(p0) == sc.key))
     throw InvalidScene{"params do not match the layout the scene was validated with"};
-  std::vector<BatchDev> host(1);
-  BatchDev &bd = host[0];
+  BatchPlan plan;
+  BatchDev &bd = plan.bd();
   std::memset(&bd, 0, sizeof(bd));
   bd.nviews = (uint32_t)nv;
   uint64_t npix_total = 0, samples = 0, max_spp = 0, staging_all = 0;
   for (size_t v = 0; v < nv; v++) {
     const lw_params &p = views[v].params;
     check_batch_view(p0, p);
-    if (!outs[v]) throw InvalidScene{"null output for a view"};
     const uint32_t w = p.frame[0], h = p.frame[1], spp = std::max(views[v].spp, 1u);
     if (w == 0 || h == 0 || (uint64_t)w * h > (1u << 26)) throw InvalidScene{"bad view size"};
     ViewCam &c = bd.cam[v];
@@ -1166,6 +1306,31 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   uint32_t npass = 0;
   for (size_t v = 0; v < nv; v++)
     npass = std::max<uint32_t>(npass, (bd.cam[v].spp + bd.per_pass[v] - 1) / bd.per_pass[v]);
+  if (items >= (1ull << 31)) throw InvalidScene{"batch too large for its passes"};
+  plan.npass = npass;
+  plan.items = items;
+  plan.staging = staging;
+  plan.npix_total = npix_total;
+  plan.samples = samples;
+  return plan;
+}
+
+// With `diff`, the samples are render_sample_diff's (direct form). Without `outs`, the views'
+// f64 sums stay in b->acc (laid out by pix_prefix) for the caller.
+void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *views, size_t nv,
+                  lw_progress_fn progress, void *user, float *const *outs,
+                  const DiffDev *diff = nullptr) {
+  auto t_wall = std::chrono::steady_clock::now();
+  if (outs)
+    for (size_t v = 0; v < nv; v++)
+      if (!outs[v]) throw InvalidScene{"null output for a view"};
+  b->plan = plan_batch(b, sc, views, nv);
+  const BatchPlan &plan = b->plan;
+  const BatchDev &bd = plan.bd();
+  const lw_params &p0 = views[0].params;
+  const uint32_t npass = plan.npass;
+  const uint64_t items = plan.items, staging = plan.staging, npix_total = plan.npix_total,
+                 samples = plan.samples;
   HIP_TRY(hipSetDevice(b->device));
   const DevScene ds = sc.dev();
   b->batch_dev.ensure(1);
@@ -1183,7 +1348,10 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   auto pass_ops = [&](hipStream_t st) {
     k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, st>>>(b->batch_dev.ptr);
     k_fill_u32<<<dim3(1), dim3(64), 0, st>>>(b->counter.ptr, 0u);
-    if (b->mode == MODE_DIRECT) {
+    if (diff) {
+      LW_DIFF_DISPATCH(b->waves, (k_render_diff_batch<W><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
+                                     p0, ds, *diff, b->batch_dev.ptr, b->rad.ptr)));
+    } else if (b->mode == MODE_DIRECT) {
       LW_DIRECT_DISPATCH(b->waves, (k_render_batch<W><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
                                        p0, ds, b->batch_dev.ptr, b->rad.ptr)));
     } else {
@@ -1194,7 +1362,7 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   };
   lw_timing tm{};
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-  if (b->mode == MODE_WAVEFRONT) {
+  if (b->mode == MODE_WAVEFRONT && !diff) {
     // The wavefront loop's length depends on the pass, so each pass is launched from here.
     for (uint32_t k = 0; k < npass; k++) {
       k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, b->stream>>>(b->batch_dev.ptr);
@@ -1241,13 +1409,15 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
     }
   }
   HIP_TRY(hipEventRecord(b->ev[1], b->stream));
-  k_resolve_batch<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                    b->stream>>>(b->batch_dev.ptr, b->acc.ptr, b->out.ptr);
-  HIP_TRY(hipGetLastError());
-  for (size_t v = 0; v < nv; v++) {
-    const size_t n3 = (size_t)bd.cam[v].width * bd.cam[v].height * 3;
-    HIP_TRY(hipMemcpyAsync(outs[v], b->out.ptr + (size_t)bd.pix_prefix[v] * 3, n3 * sizeof(float),
-                           hipMemcpyDeviceToHost, b->stream));
+  if (outs) {
+    k_resolve_batch<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                      b->stream>>>(b->batch_dev.ptr, b->acc.ptr, b->out.ptr);
+    HIP_TRY(hipGetLastError());
+    for (size_t v = 0; v < nv; v++) {
+      const size_t n3 = (size_t)bd.cam[v].width * bd.cam[v].height * 3;
+      HIP_TRY(hipMemcpyAsync(outs[v], b->out.ptr + (size_t)bd.pix_prefix[v] * 3, n3 * sizeof(float),
+                             hipMemcpyDeviceToHost, b->stream));
+    }
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   float ms = 0.0f;
@@ -1374,6 +1544,37 @@ void adjoint_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   b->timing = tm;
 }
 
+// The adjoint of the last batch render_batch planned (b->plan), with D.dldi holding every
+// view's dL/dI (laid out by pix_prefix) and D.grads the device gradient slots, which it
+// zeroes first.
+void adjoint_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p0,
+                   const DiffDev &D, lw_timing &tm) {
+  if (p0.frame[3] > kMaxSuffix) throw InvalidScene{"the adjoint takes at most 16 bounces"};
+  const BatchPlan &plan = b->plan;
+  const uint32_t slots = 3u * D.n_pal + D.n_int + 3u;
+  const DevScene ds = sc.dev();
+  HIP_TRY(hipMemcpyAsync(b->batch_dev.ptr, &plan.bd(), sizeof(BatchDev), hipMemcpyHostToDevice,
+                         b->stream));
+  k_zero<<<dim3((slots + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream>>>(D.grads, slots);
+  HIP_TRY(hipGetLastError());
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, plan.items / kBlock);
+  const size_t lds = (size_t)slots * sizeof(double);
+  HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+  for (uint32_t k = 0; k < plan.npass; k++) {
+    k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, b->stream>>>(b->batch_dev.ptr);
+    LW_DIFF_DISPATCH(b->waves, (k_adjoint_batch<W><<<dim3(blocks), dim3(kBlock), lds, b->stream>>>(
+                                   p0, ds, D, b->batch_dev.ptr)));
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+  HIP_TRY(hipEventSynchronize(b->ev[3]));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev[2], b->ev[3]));
+  tm.trace_ms = ms;
+  tm.launches = plan.npass;
+  tm.samples = plan.samples;
+}
+
 template <typename F>
 int guarded(lw_hip_backend *b, F &&f) {
   try {
@@ -1396,6 +1597,19 @@ int guarded(lw_hip_backend *b, F &&f) {
 }
 
 }  // namespace
+
+// An inverse fit's state on the device (lw_hip_fit_*): the views, the parameter groups, the
+// factors with their Adam moments, the targets, dL/dI and the gradient.
+struct lw_hip_fit {
+  std::vector<lw_view> views;
+  uint32_t n_pal = 0, n_int = 0, t = 0;
+  lw_fit_options opt{};
+  DevBuf<float> x, m, v, target, dldi;
+  DevBuf<uint32_t> qg, qig;
+  DevBuf<double> grads, loss;
+  DevBuf<float> dummy;  // the derivative table a fit without sky parameters points at
+  uint64_t npix_total = 0;
+};
 
 extern "C" {
 
@@ -1563,11 +1777,11 @@ int lw_hip_render_hq_batch(lw_hip_backend *b, const lw_hip_scene *scene, const l
 }
 
 int lw_hip_render_diff(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
-                       uint32_t spp, const lw_diff_desc *diff, float *out_rgb) {
+                       uint32_t spp, const lw_diff_desc *diff, float *out_rgb, double *out_f64) {
   if (!b || !scene || !params || !diff || !out_rgb) return LW_EINVAL;
   return guarded(b, [&] {
     const DiffDev D = prepare_diff(b, *scene, *params, *diff);
-    render_frame(b, *scene, *params, spp, nullptr, nullptr, out_rgb, &D);
+    render_frame(b, *scene, *params, spp, nullptr, nullptr, out_rgb, &D, out_f64);
   });
 }
 
@@ -1576,6 +1790,171 @@ int lw_hip_adjoint(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params
                    double *grads) {
   if (!b || !scene || !params || !diff || !d_loss_d_image || !grads) return LW_EINVAL;
   return guarded(b, [&] { adjoint_frame(b, *scene, *params, spp, *diff, d_loss_d_image, grads); });
+}
+
+int lw_hip_render_diff_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
+                             size_t n, const lw_diff_desc *diff, float *const *out_rgb) {
+  if (!b || !scene || !views || !diff || !out_rgb) return LW_EINVAL;
+  return guarded(b, [&] {
+    if (n == 0) throw InvalidScene{"a batch holds 1 to 256 views"};
+    const DiffDev D = prepare_diff(b, *scene, views[0].params, *diff);
+    render_batch(b, *scene, views, n, nullptr, nullptr, out_rgb, &D);
+  });
+}
+
+int lw_hip_adjoint_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
+                         size_t n, const lw_diff_desc *diff, const float *const *d_loss_d_image,
+                         double *grads) {
+  if (!b || !scene || !views || !diff || !d_loss_d_image || !grads) return LW_EINVAL;
+  return guarded(b, [&] {
+    if (n == 0) throw InvalidScene{"a batch holds 1 to 256 views"};
+    DiffDev D = prepare_diff(b, *scene, views[0].params, *diff);
+    b->plan = plan_batch(b, *scene, views, n);
+    const BatchDev &bd = b->plan.bd();
+    const uint32_t slots = 3u * D.n_pal + D.n_int + 3u;
+    b->diff_dldi.ensure(b->plan.npix_total * 3);
+    for (size_t v = 0; v < n; v++) {
+      if (!d_loss_d_image[v]) throw InvalidScene{"null dL/dI for a view"};
+      HIP_TRY(hipMemcpyAsync(b->diff_dldi.ptr + (size_t)bd.pix_prefix[v] * 3, d_loss_d_image[v],
+                             (size_t)bd.cam[v].width * bd.cam[v].height * 3 * sizeof(float),
+                             hipMemcpyHostToDevice, b->stream));
+    }
+    b->diff_grads.ensure(slots);
+    b->batch_dev.ensure(1);
+    D.dldi = b->diff_dldi.ptr;
+    D.grads = b->diff_grads.ptr;
+    lw_timing tm{};
+    adjoint_batch(b, *scene, views[0].params, D, tm);
+    HIP_TRY(hipMemcpyAsync(grads, D.grads, slots * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->timing = tm;
+  });
+}
+
+int lw_hip_fit_new(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views, size_t n,
+                   const lw_diff_desc *diff, const float *const *targets,
+                   const lw_fit_options *opt, lw_hip_fit **out) {
+  if (!b || !scene || !views || !diff || !targets || !opt || !out) return LW_EINVAL;
+  *out = nullptr;
+  auto *f = new (std::nothrow) lw_hip_fit();
+  if (!f) return LW_ENOMEM;
+  const int rc = guarded(b, [&] {
+    if (n == 0) throw InvalidScene{"a batch holds 1 to 256 views"};
+    if (diff->sky_params) throw Unsupported{"an inverse fit over the sky parameters"};
+    if (!(opt->lr > 0.0f) || !(opt->beta1 >= 0.0f && opt->beta1 < 1.0f) ||
+        !(opt->beta2 >= 0.0f && opt->beta2 < 1.0f) || !(opt->eps > 0.0f) || !(opt->min_factor > 0.0f))
+      throw InvalidScene{"bad Adam options"};
+    if (views[0].params.frame[3] > kMaxSuffix) throw InvalidScene{"the adjoint takes at most 16 bounces"};
+    (void)prepare_diff(b, *scene, views[0].params, *diff);  // checks the groups and factors
+    BatchPlan plan = plan_batch(b, *scene, views, n);
+    const BatchDev &bd = plan.bd();
+    f->views.assign(views, views + n);
+    f->n_pal = diff->n_palette;
+    f->n_int = diff->n_intensity;
+    f->opt = *opt;
+    f->npix_total = plan.npix_total;
+    const uint32_t np = 3 * f->n_pal + f->n_int;
+    std::vector<float> x0(std::max(np, 1u), 0.0f);
+    for (uint32_t i = 0; i < 3 * f->n_pal; i++) x0[i] = diff->palette[i];
+    for (uint32_t i = 0; i < f->n_int; i++) x0[3 * f->n_pal + i] = diff->intensity[i];
+    f->x.upload(x0.data(), x0.size(), b->stream);
+    std::vector<float> zeros(x0.size(), 0.0f);
+    f->m.upload(zeros.data(), zeros.size(), b->stream);
+    f->v.upload(zeros.data(), zeros.size(), b->stream);
+    const size_t nq = scene->quad_flags.size();
+    f->qg.upload(diff->quad_palette, nq, b->stream);
+    f->qig.upload(diff->quad_intensity, nq, b->stream);
+    f->target.alloc(plan.npix_total * 3);
+    f->dldi.alloc(plan.npix_total * 3);
+    for (size_t v = 0; v < n; v++) {
+      if (!targets[v]) throw InvalidScene{"null target for a view"};
+      HIP_TRY(hipMemcpyAsync(f->target.ptr + (size_t)bd.pix_prefix[v] * 3, targets[v],
+                             (size_t)bd.cam[v].width * bd.cam[v].height * 3 * sizeof(float),
+                             hipMemcpyHostToDevice, b->stream));
+    }
+    f->grads.alloc(np + 3);
+    f->loss.alloc(1);
+    f->dummy.alloc(1);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  });
+  if (rc != LW_OK) {
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return LW_OK;
+}
+
+int lw_hip_fit_step(lw_hip_backend *b, const lw_hip_scene *scene, lw_hip_fit *f,
+                    lw_fit_stats *stats) {
+  if (!b || !scene || !f || !stats) return LW_EINVAL;
+  return guarded(b, [&] {
+    auto t_wall = std::chrono::steady_clock::now();
+    DiffDev D{};
+    D.pal = f->x.ptr;
+    D.inten = f->x.ptr + 3 * f->n_pal;
+    D.qgroup = f->qg.ptr;
+    D.qigroup = f->qig.ptr;
+    D.dsky = f->dummy.ptr;
+    D.dldi = f->dldi.ptr;
+    D.grads = f->grads.ptr;
+    D.n_pal = f->n_pal;
+    D.n_int = f->n_int;
+    const lw_view *views = f->views.data();
+    const size_t n = f->views.size();
+    render_batch(b, *scene, views, n, nullptr, nullptr, nullptr, &D);
+    const double fwd_ms = b->timing.trace_ms;
+    k_zero<<<dim3(1), dim3(kBlock), 0, b->stream>>>(f->loss.ptr, 1);
+    const uint64_t n3 = f->npix_total * 3;
+    k_fit_loss<<<dim3((unsigned)((n3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        b->batch_dev.ptr, b->acc.ptr, f->target.ptr, f->dldi.ptr, 1.0 / (double)n3, f->loss.ptr);
+    HIP_TRY(hipGetLastError());
+    lw_timing ta{};
+    adjoint_batch(b, *scene, views[0].params, D, ta);
+    f->t += 1;
+    const float bc1 = 1.0f - std::pow(f->opt.beta1, (float)f->t);
+    const float bc2 = 1.0f - std::pow(f->opt.beta2, (float)f->t);
+    const uint32_t np = 3 * f->n_pal + f->n_int;
+    if (np > 0) {
+      k_adam<<<dim3((np + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream>>>(
+          f->x.ptr, f->grads.ptr, f->m.ptr, f->v.ptr, np, f->opt.lr, f->opt.beta1, f->opt.beta2,
+          f->opt.eps, bc1, bc2, f->opt.min_factor);
+      HIP_TRY(hipGetLastError());
+    }
+    double loss = 0.0;
+    HIP_TRY(hipMemcpyAsync(&loss, f->loss.ptr, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    stats->loss = loss;
+    stats->forward_ms = fwd_ms;
+    stats->adjoint_ms = ta.trace_ms;
+    stats->wall_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+    stats->step = f->t;
+    stats->samples = b->plan.samples;
+  });
+}
+
+int lw_hip_fit_params(lw_hip_backend *b, const lw_hip_fit *f, float *palette, float *intensity,
+                      double *grads) {
+  if (!b || !f) return LW_EINVAL;
+  return guarded(b, [&] {
+    const uint32_t np = 3 * f->n_pal + f->n_int;
+    std::vector<float> x(std::max(np, 1u));
+    HIP_TRY(hipMemcpyAsync(x.data(), f->x.ptr, x.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           b->stream));
+    if (grads)
+      HIP_TRY(hipMemcpyAsync(grads, f->grads.ptr, (np + 3) * sizeof(double), hipMemcpyDeviceToHost,
+                             b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (palette) std::memcpy(palette, x.data(), 3 * f->n_pal * sizeof(float));
+    if (intensity) std::memcpy(intensity, x.data() + 3 * f->n_pal, f->n_int * sizeof(float));
+  });
+}
+
+void lw_hip_fit_free(lw_hip_backend *b, lw_hip_fit *f) {
+  if (!f) return;
+  if (b) (void)hipSetDevice(b->device);
+  delete f;
 }
 
 int lw_hip_copy_probe(lw_hip_backend *b, size_t bytes, double *ms) {

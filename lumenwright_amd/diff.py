@@ -202,3 +202,47 @@ def config3_problem(cfg: Optional[frames.FrameConfig] = None, seed: int = 7) -> 
     sky = base_sky(cfg).copy()
     sky[0] = 3.5
     return Problem(cfg, scene, start, start.with_values(pal, inten, sky))
+
+
+@dataclass
+class BatchProblem:
+    """An inverse fit over a batch of views: the views, their parameter groups, the start and
+    the seeded ground truth the targets are rendered with."""
+
+    cfg: frames.FrameConfig
+    scene: object
+    base: frames.HqFrame
+    views: list
+    start: DiffParams
+    truth: DiffParams
+
+
+def config4_problem(n_views: int = 32, size=(640, 360), spp: int = 128, max_bounces: int = 6,
+                    n_groups: int = 128, seed: int = 4, world=None) -> BatchProblem:
+    """cfg4: seeded views of the cfg1 world; 64 block-type palettes (192 albedo factors) and
+    the emitters in 128 seeded intensity groups. Ground truth: albedo factors U(0.8, 1.2),
+    intensities U(0.6, 1.4); the fit starts from 1."""
+    cfg = frames.CONFIGS[1] if world is None else replace(frames.CONFIGS[1], world=world)
+    scene = frames.build_world(cfg.world)
+    base, views = frames.batch_views(cfg, scene, n_views=n_views, seed=seed, sizes=(size,),
+                                     spps=(spp,), max_bounces=max_bounces)
+    n_types = cfg.world.cube_types + cfg.world.cutout_types
+    qp = block_type_groups(scene, n_types)
+    qi = emitter_groups(scene, n_groups, seed=seed)
+    start = identity(qp, qi, n_types, n_groups)
+    rng = np.random.default_rng(seed + 100)
+    truth = start.with_values(rng.uniform(0.8, 1.2, size=(n_types, 3)).astype("<f4"),
+                              rng.uniform(0.6, 1.4, size=n_groups).astype("<f4"))
+    return BatchProblem(cfg, scene, base, views, start, truth)
+
+
+def parameter_error(p: DiffParams, truth: DiffParams, used_palette=None, used_intensity=None):
+    """Mean absolute error of the palette and intensity factors against the truth, over the
+    groups the views can see (all when the masks are None)."""
+    dp = np.abs(p.palette - truth.palette)
+    di = np.abs(p.intensity - truth.intensity)
+    if used_palette is not None:
+        dp = dp[used_palette]
+    if used_intensity is not None:
+        di = di[used_intensity]
+    return float(dp.mean()) if dp.size else 0.0, float(di.mean()) if di.size else 0.0

@@ -102,8 +102,10 @@ def full_size(hip_backend, problem):
 
 
 def J(backend, sc, params, spp, p, G):
-    return float(np.dot(backend.render_diff(sc, params, spp, p.desc()).reshape(-1).astype("<f8"),
-                        G.astype("<f8")))
+    # sum(G * I) over the f64 means: the f32 image's rounding would add about 4e-3 of noise to
+    # each difference at this size.
+    _, img64 = backend.render_diff(sc, params, spp, p.desc(), f64=True)
+    return float(np.dot(img64.reshape(-1), G.astype("<f8")))
 
 
 def test_adjoint_matches_central_differences_at_full_size(hip_backend, problem, full_size):
@@ -211,3 +213,64 @@ def test_unsupported_and_invalid_requests_fail(hip_backend, problem):
             hip_backend.render_diff(sc, fr.params, 2, bad.desc())
     finally:
         sc.free()
+
+
+@pytest.fixture(scope="module")
+def batch_problem():
+    return diff.config4_problem(n_views=4, size=(96, 54), spp=8, n_groups=16,
+                                world=frames.WorldSpec(8, 8, 24, cube_types=24, cutout_types=4,
+                                                       water_level=62, emitter_fraction=0.01,
+                                                       cutout_fraction=0.05))
+
+
+def test_batched_forward_equals_single_views(hip_backend, batch_problem):
+    bp = batch_problem
+    sc = hip_backend.upload(bp.base)
+    outs = hip_backend.render_diff_batch(sc, bp.views, bp.truth.desc())
+    for (p, spp), img in zip(bp.views, outs):
+        one = hip_backend.render_diff(sc, p, spp, bp.truth.desc())
+        assert np.array_equal(img.view("<u4"), one.view("<u4"))
+    sc.free()
+
+
+def test_batched_adjoint_is_the_sum_of_the_views(hip_backend, batch_problem):
+    bp = batch_problem
+    sc = hip_backend.upload(bp.base)
+    rng = np.random.default_rng(2)
+    Gs = [rng.standard_normal(int(p["frame"][0]) * int(p["frame"][1]) * 3).astype("<f4")
+          for p, _ in bp.views]
+    n = bp.start.n_grads
+    got = hip_backend.adjoint_batch(sc, bp.views, bp.truth.desc(), n, Gs)
+    want = sum(hip_backend.adjoint(sc, p, spp, bp.truth.desc(), n, G)
+               for (p, spp), G in zip(bp.views, Gs))
+    assert np.abs(want).max() > 0
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
+    sc.free()
+
+
+def test_fit_reduces_the_loss_and_moves_towards_the_truth(hip_backend, batch_problem):
+    bp = batch_problem
+    sc = hip_backend.upload(bp.base)
+    targets = hip_backend.render_diff_batch(sc, bp.views, bp.truth.desc())
+    fit = hip_backend.fit(sc, bp.views, bp.start.desc(), targets, lr=2e-2)
+    first = fit.step()
+    # The first step's gradient is the batched adjoint of dL/dI = 2 (I - T) / N at the start.
+    imgs = hip_backend.render_diff_batch(sc, bp.views, bp.start.desc())
+    n = sum(i.size for i in imgs)
+    dldis = [(2.0 / n) * (i.astype("<f8") - t) for i, t in zip(imgs, targets)]
+    want = hip_backend.adjoint_batch(sc, bp.views, bp.start.desc(), bp.start.n_grads, dldis)
+    _, _, g = fit.params()
+    assert np.abs(g - want).max() <= 1e-3 * np.abs(want).max()
+    loss0 = first["loss"]
+    assert abs(loss0 - sum(float(np.sum((i.astype("<f8") - t) ** 2)) for i, t in zip(imgs, targets)) / n) <= 1e-6 * loss0
+    for _ in range(40):
+        last = fit.step()
+    pal, inten, _ = fit.params()
+    fitted = bp.start.with_values(pal, inten)
+    e0 = diff.parameter_error(bp.start, bp.truth)
+    e1 = diff.parameter_error(fitted, bp.truth)
+    print("loss", loss0, "->", last["loss"], "error", e0, "->", e1)
+    assert last["loss"] < 0.5 * loss0
+    assert e1[0] < e0[0]
+    fit.free()
+    sc.free()
