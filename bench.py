@@ -90,7 +90,7 @@ def roofline(config, samples, trace_ms, launches):
     achieved = b_alg["bytes_per_sample"] * per_launch / mean_s / 1e9
     traffic = None
     traffic_path = ROOT / "profiles" / "traffic.json"
-    if traffic_path.exists() and kernel_form() == "direct, waves 5":
+    if traffic_path.exists() and kernel_form() == "direct, waves 6":
         t = json.loads(traffic_path.read_text()).get(f"config{config}")
         if t:
             traffic = t["hbm_bytes_per_sample"] * per_launch
@@ -104,7 +104,7 @@ def roofline(config, samples, trace_ms, launches):
 
 def kernel_form():
     form = os.environ.get("LW_KERNEL", "direct")
-    form += f", waves {os.environ.get('LW_WAVES', '5')}"
+    form += f", waves {os.environ.get('LW_WAVES', '6')}"
     if os.environ.get("LW_XCD_REMAP", "0") != "0":
         form += ", xcd remap"
     return form

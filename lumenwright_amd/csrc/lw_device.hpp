@@ -709,139 +709,166 @@ __device__ __forceinline__ uint32_t cam_width(const ViewCam &c) { return c.width
 __device__ __forceinline__ uint32_t cam_height(const ViewCam &c) { return c.height; }
 __device__ __forceinline__ uint32_t cam_seed(const ViewCam &c) { return c.seed; }
 
-// integrator.rs:539-709: the radiance of sample `sample` of pixel (px, py).
-// The camera ray comes from `c` (the Params themselves, or one view of a batch).
+// render_sample (integrator.rs:539-709) in two parts: sample_begin makes the camera ray
+// (:540-569), sample_event runs one iteration of the event loop (:571-707) and says whether
+// the path goes on. render_sample runs them to the end; a kernel can also interleave paths
+// (start the next sample of a lane as soon as its path ends) with the same arithmetic.
+struct SampleState {
+  V3 o, d, radiance, beta;
+  Rng rng;
+  float tmin;
+  uint32_t bounces, ev;
+  bool specular, in_water;
+};
+
 template <class Cam>
-__device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &p, const Cam &c,
-                                            const float *lut, uint32_t px, uint32_t py,
-                                            uint32_t sample) {
+__device__ __forceinline__ void sample_begin(SampleState &S, const Cam &c, uint32_t px, uint32_t py,
+                                             uint32_t sample) {
   const uint32_t width = cam_width(c);
   const uint32_t height = cam_height(c);
-  const uint32_t max_bounces = p.frame[3];
-  const float eps = p.misc_f[1];
-  Rng rng = Rng::make(py * width + px, sample, cam_seed(c));
-
+  S.rng = Rng::make(py * width + px, sample, cam_seed(c));
   const float tan_half = c.cam_pos[3];
   const float aspect = c.cam_fwd[3];
-  const float jx = rng.next();
-  const float jy = rng.next();
+  const float jx = S.rng.next();
+  const float jy = S.rng.next();
   const float sx = (2.0f * ((float)px + jx) / (float)width - 1.0f) * aspect * tan_half;
   const float sy = (1.0f - 2.0f * ((float)py + jy) / (float)height) * tan_half;
-  V3 d = normalize(V3{c.cam_fwd[0], c.cam_fwd[1], c.cam_fwd[2]} +
-                   V3{c.cam_right[0], c.cam_right[1], c.cam_right[2]} * sx +
-                   V3{c.cam_up[0], c.cam_up[1], c.cam_up[2]} * sy);
-  V3 o = V3{c.cam_pos[0], c.cam_pos[1], c.cam_pos[2]};
+  S.d = normalize(V3{c.cam_fwd[0], c.cam_fwd[1], c.cam_fwd[2]} +
+                  V3{c.cam_right[0], c.cam_right[1], c.cam_right[2]} * sx +
+                  V3{c.cam_up[0], c.cam_up[1], c.cam_up[2]} * sy);
+  S.o = V3{c.cam_pos[0], c.cam_pos[1], c.cam_pos[2]};
+  S.radiance = V3{0.0f, 0.0f, 0.0f};
+  S.beta = V3{1.0f, 1.0f, 1.0f};
+  S.bounces = 0;
+  S.ev = 0;
+  S.specular = true;
+  S.in_water = false;
+  S.tmin = 0.0f;
+}
 
-  V3 radiance{0.0f, 0.0f, 0.0f};
-  V3 beta{1.0f, 1.0f, 1.0f};
-  uint32_t bounces = 0;
-  bool specular = true;
-  bool in_water = false;
+// One iteration of the event loop; false once the path has ended (a `break` of the loop or
+// MAX_EVENTS iterations), true where the loop would go on (`continue` or the loop's end).
+__device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s, const lw_params &p,
+                                             const float *lut) {
+  if (S.ev >= MAX_EVENTS) return false;
+  S.ev++;
+  const uint32_t max_bounces = p.frame[3];
+  const float eps = p.misc_f[1];
   const uint32_t strategy = p.misc_u[3] > 0 ? p.emitter[0] : LW_EMITTER_SAMPLING_NONE;
-  float tmin = 0.0f;
   const V3 sigma = V3{p.water_color[0], p.water_color[1], p.water_color[2]};
   const V3 sun = V3{p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]};
   const V3 sun_color = V3{p.sun_color[0], p.sun_color[1], p.sun_color[2]};
   const float sin_r = f_sqrt(f_max(1.0f - p.sun_dir[3] * p.sun_dir[3], 0.0f));
   const bool lit = p.sun_color[0] > 0.0f || p.sun_color[1] > 0.0f || p.sun_color[2] > 0.0f;
 
-  for (uint32_t ev = 0; ev < MAX_EVENTS; ev++) {
-    Hit hit;
-    bool has_hit = trace(s, p, o, d, tmin, INFINITY, hit, nullptr);
-    bool forced = false;
-    if (bounces == 0 && p.see_through[1] > 0 && has_hit && !is_see_through(p, hit.quad)) {
-      Hit st;
-      if (see_through_hit(s, p, o, d, hit.t, st)) {
-        const uint32_t sflags = f_bits(s.quads[(size_t)st.quad * 5].w);
-        if ((sflags & LW_FLAG_ALPHA_MASK) == LW_ALPHA_TRANSLUCENT) {
-          if (rng.next() < (float)(st.texel >> 24) / 255.0f) {
-            hit = st;
-            forced = true;
-          }
-        } else {
+  Hit hit;
+  bool has_hit = trace(s, p, S.o, S.d, S.tmin, INFINITY, hit, nullptr);
+  bool forced = false;
+  if (S.bounces == 0 && p.see_through[1] > 0 && has_hit && !is_see_through(p, hit.quad)) {
+    Hit st;
+    if (see_through_hit(s, p, S.o, S.d, hit.t, st)) {
+      const uint32_t sflags = f_bits(s.quads[(size_t)st.quad * 5].w);
+      if ((sflags & LW_FLAG_ALPHA_MASK) == LW_ALPHA_TRANSLUCENT) {
+        if (S.rng.next() < (float)(st.texel >> 24) / 255.0f) {
           hit = st;
+          forced = true;
         }
+      } else {
+        hit = st;
       }
     }
-    float tp;
-    if (water_plane(p, o, d, tp) && (!has_hit || tp < hit.t)) {
-      if (in_water) beta = mul(beta, transmittance(sigma, tp));
-      const bool front = d.y < 0.0f;
-      const V3 n = v3(0.0f, front ? 1.0f : -1.0f, 0.0f);
-      water_interface(p, o + d * tp, d, n, front, in_water, eps, rng, d, o, in_water);
-      specular = true;
-      tmin = 0.0f;
-      continue;
-    }
-    if (!has_hit) {
-      if (!in_water && !under_water_plane(p, o, d)) {
-        if (bounces == 0) {
-          radiance = radiance + mul(beta, sky_lens(s, p, d));
-        } else {
-          radiance = radiance + mul(beta, sky_light(s, p, d));
-        }
-        if (specular && bounces == 0) radiance = radiance + mul(beta, sun_texture(s, p, lut, d));
-      }
-      break;
-    }
-    if (in_water) beta = mul(beta, transmittance(sigma, hit.t));
-    const float4 *q = s.quads + (size_t)hit.quad * 5;
-    const float4 q0 = q[0], q1 = q[1], q2 = q[2];
-    const uint32_t flags = f_bits(q0.w);
-    const V3 pos = o + d * hit.t;
-    const float alpha = (float)(hit.texel >> 24) / 255.0f;
-    const V3 albedo = mul(linear_rgb(lut, hit.texel), linear_rgb(lut, f_bits(q1.w)));
-    // integrator.rs:307-315 (shading_normal)
-    V3 n = normalize(cross(xyz(q2), xyz(q1)));
-    const bool front = dot(n, d) < 0.0f;
-    if (!front) n = -n;
-
-    if ((flags & LW_FLAG_WATER) != 0) {
-      water_interface(p, pos, d, n, front, in_water, eps, rng, d, o, in_water);
-      specular = true;
-      tmin = 0.0f;
-      continue;
-    }
-
-    if (!forced && (flags & LW_FLAG_ALPHA_MASK) == LW_ALPHA_TRANSLUCENT && rng.next() >= alpha) {
-      beta = mul(beta, lerp(V3{1.0f, 1.0f, 1.0f}, albedo, alpha));
-      o = pos + d * eps;
-      tmin = 0.0f;
-      continue;
-    }
-
-    if (bounces >= max_bounces) break;
-    const float emittance = q2.w * p.misc_f[0];
-    const bool emits = emittance > 0.0f &&
-                       (p.emitter[1] == 0 || strategy == LW_EMITTER_SAMPLING_NONE || bounces == 0);
-    if (emits) radiance = radiance + mul(beta, mul(albedo, albedo)) * emittance;
-    bounces += 1;
-
-    const V3 origin = pos + n * eps;
-    if (lit && dot(n, sun) > -sin_r && p.sun_dir[1] > -sin_r) {
-      const V3 l = sample_sun(p, rng);
-      const float cos_l = dot(n, l);
-      if (cos_l > 0.0f && l.y >= 0.0f) {
-        const V3 tr = shadow(s, p, lut, origin, l, INFINITY);
-        radiance = radiance + mul(mul(mul(beta, albedo), sun_color), tr) * cos_l;
-      }
-    }
-    if (!emits && strategy != LW_EMITTER_SAMPLING_NONE) {
-      radiance = radiance + mul(mul(beta, albedo), sample_emitters(s, p, lut, pos, origin, n, rng));
-    }
-
-    beta = mul(beta, albedo);
-    if (bounces >= 3) {
-      const float q_rr = f_clamp(max_elem(beta), 0.05f, 0.95f);
-      if (rng.next() > q_rr) break;
-      beta = beta / q_rr;
-    }
-    d = cosine_hemisphere(n, rng);
-    o = origin;
-    tmin = 0.0f;
-    specular = false;
   }
-  return radiance;
+  float tp;
+  if (water_plane(p, S.o, S.d, tp) && (!has_hit || tp < hit.t)) {
+    if (S.in_water) S.beta = mul(S.beta, transmittance(sigma, tp));
+    const bool front = S.d.y < 0.0f;
+    const V3 n = v3(0.0f, front ? 1.0f : -1.0f, 0.0f);
+    water_interface(p, S.o + S.d * tp, S.d, n, front, S.in_water, eps, S.rng, S.d, S.o, S.in_water);
+    S.specular = true;
+    S.tmin = 0.0f;
+    return true;
+  }
+  if (!has_hit) {
+    if (!S.in_water && !under_water_plane(p, S.o, S.d)) {
+      if (S.bounces == 0) {
+        S.radiance = S.radiance + mul(S.beta, sky_lens(s, p, S.d));
+      } else {
+        S.radiance = S.radiance + mul(S.beta, sky_light(s, p, S.d));
+      }
+      if (S.specular && S.bounces == 0) S.radiance = S.radiance + mul(S.beta, sun_texture(s, p, lut, S.d));
+    }
+    return false;
+  }
+  if (S.in_water) S.beta = mul(S.beta, transmittance(sigma, hit.t));
+  const float4 *q = s.quads + (size_t)hit.quad * 5;
+  const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+  const uint32_t flags = f_bits(q0.w);
+  const V3 pos = S.o + S.d * hit.t;
+  const float alpha = (float)(hit.texel >> 24) / 255.0f;
+  const V3 albedo = mul(linear_rgb(lut, hit.texel), linear_rgb(lut, f_bits(q1.w)));
+  // integrator.rs:307-315 (shading_normal)
+  V3 n = normalize(cross(xyz(q2), xyz(q1)));
+  const bool front = dot(n, S.d) < 0.0f;
+  if (!front) n = -n;
+
+  if ((flags & LW_FLAG_WATER) != 0) {
+    water_interface(p, pos, S.d, n, front, S.in_water, eps, S.rng, S.d, S.o, S.in_water);
+    S.specular = true;
+    S.tmin = 0.0f;
+    return true;
+  }
+
+  if (!forced && (flags & LW_FLAG_ALPHA_MASK) == LW_ALPHA_TRANSLUCENT && S.rng.next() >= alpha) {
+    S.beta = mul(S.beta, lerp(V3{1.0f, 1.0f, 1.0f}, albedo, alpha));
+    S.o = pos + S.d * eps;
+    S.tmin = 0.0f;
+    return true;
+  }
+
+  if (S.bounces >= max_bounces) return false;
+  const float emittance = q2.w * p.misc_f[0];
+  const bool emits = emittance > 0.0f &&
+                     (p.emitter[1] == 0 || strategy == LW_EMITTER_SAMPLING_NONE || S.bounces == 0);
+  if (emits) S.radiance = S.radiance + mul(S.beta, mul(albedo, albedo)) * emittance;
+  S.bounces += 1;
+
+  const V3 origin = pos + n * eps;
+  if (lit && dot(n, sun) > -sin_r && p.sun_dir[1] > -sin_r) {
+    const V3 l = sample_sun(p, S.rng);
+    const float cos_l = dot(n, l);
+    if (cos_l > 0.0f && l.y >= 0.0f) {
+      const V3 tr = shadow(s, p, lut, origin, l, INFINITY);
+      S.radiance = S.radiance + mul(mul(mul(S.beta, albedo), sun_color), tr) * cos_l;
+    }
+  }
+  if (!emits && strategy != LW_EMITTER_SAMPLING_NONE) {
+    S.radiance = S.radiance + mul(mul(S.beta, albedo), sample_emitters(s, p, lut, pos, origin, n, S.rng));
+  }
+
+  S.beta = mul(S.beta, albedo);
+  if (S.bounces >= 3) {
+    const float q_rr = f_clamp(max_elem(S.beta), 0.05f, 0.95f);
+    if (S.rng.next() > q_rr) return false;
+    S.beta = S.beta / q_rr;
+  }
+  S.d = cosine_hemisphere(n, S.rng);
+  S.o = origin;
+  S.tmin = 0.0f;
+  S.specular = false;
+  return true;
+}
+
+// integrator.rs:539-709: the radiance of sample `sample` of pixel (px, py); the camera ray
+// comes from `c` (the Params themselves, or one view of a batch).
+template <class Cam>
+__device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &p, const Cam &c,
+                                            const float *lut, uint32_t px, uint32_t py,
+                                            uint32_t sample) {
+  SampleState S;
+  sample_begin(S, c, px, py, sample);
+  while (sample_event(S, s, p, lut)) {
+  }
+  return S.radiance;
 }
 
 __device__ __forceinline__ V3 render_sample(const DevScene &s, const lw_params &p, const float *lut,

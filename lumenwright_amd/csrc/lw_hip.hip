@@ -580,6 +580,45 @@ __global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const 
   }
 }
 
+// The direct form with several samples per lane: lane (px, py) renders samples
+// first .. first + count - 1 of its pixel one after another in one loop, starting the next
+// sample as soon as its path ends, so a wave's lanes wait on one another's path lengths once
+// per launch instead of once per sample. Each sample's arithmetic is render_sample's.
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_render_multi(const lw_params p, const DevScene s,
+                                                         uint32_t sample_start, uint32_t nb,
+                                                         uint32_t per_lane,
+                                                         float *__restrict__ rad) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint32_t tiles_x = (w + 15u) / 16u;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  if (px >= w || py >= h) return;
+  const uint32_t first = blockIdx.y * per_lane;
+  const uint32_t count = min(per_lane, nb - first);
+  SampleState S;
+  uint32_t k = 0;
+  bool live = false;
+  for (;;) {
+    if (!live) {
+      if (k > 0) {
+        float *o = rad + ((size_t)(first + k - 1) * h * w + (size_t)py * w + px) * 3;
+        o[0] = S.radiance.x;
+        o[1] = S.radiance.y;
+        o[2] = S.radiance.z;
+      }
+      if (k == count) break;
+      sample_begin(S, p, px, py, sample_start + first + k);
+      k++;
+    }
+    live = sample_event(S, s, p, lut_s);
+  }
+}
+
 // A batch pass's view, tile and sample slice of workgroup `base` (k_render_batch's layout).
 struct BatchItem {
   uint32_t v, sl, px, py;
@@ -830,12 +869,16 @@ struct lw_hip_backend {
   // slots).
   int mode = 1;
   // LW_WAVES, LW_TRACE_WAVES: register budgets (see dispatch). The direct form measured
-  // fastest at 5 waves per SIMD (96 VGPRs): 500 Msamples/s at the 720p 256-spp frame
-  // against 451 at 4, 492 at 6 and 414 at 8. The persistent form, which has no 5-wave
-  // variant, then runs at its own best, 4.
-  int waves = 5, trace_waves = 1;
+  // fastest at 6 waves per SIMD (80 VGPRs): 564 Msamples/s at the 720p 256-spp frame
+  // against 528 at 5 and 412 at 8. The persistent form, which has no 6-wave variant, then
+  // runs at its own best, 4.
+  int waves = 6, trace_waves = 1;
   // LW_XCD_REMAP=1: the direct kernel gives each XCD a contiguous run of tiles.
   uint32_t xcd_remap = 0;
+  // LW_SPL: samples per lane of the direct form (k_render_multi when above 1). Measured
+  // slower at every budget (at 6 waves: 564, 509, 495, 475 Msamples/s for 1, 4, 8, 16): lanes
+  // whose paths drift apart in depth lose more to divergence than the wave gains back.
+  uint32_t per_lane = 1;
   uint32_t wf_slots = 1u << 20;
   DevBuf<PathState> wf_st;
   DevBuf<float4> wf_o, wf_d;
@@ -1179,6 +1222,11 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     if (diff) {
       LW_DIFF_DISPATCH(b->waves, (k_render_diff<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
                                      p, ds, *diff, done, b->rad.ptr)));
+    } else if (b->mode == MODE_DIRECT && b->per_lane > 1) {
+      const uint32_t per = std::min(b->per_lane, nb);
+      LW_DIRECT_DISPATCH(b->waves, (k_render_multi<W><<<dim3(tiles, (nb + per - 1) / per),
+                                                         dim3(kBlock), 0, b->stream>>>(
+                                       p, ds, done, nb, per, b->rad.ptr)));
     } else if (b->mode == MODE_DIRECT) {
       LW_DIRECT_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
                                        p, ds, done, b->xcd_remap, b->rad.ptr)));
@@ -1691,6 +1739,7 @@ int lw_hip_open(int device, lw_hip_backend **out) {
       if (std::strcmp(e, "wavefront") == 0) b->mode = MODE_WAVEFRONT;
     }
     if (const char *e = std::getenv("LW_XCD_REMAP")) b->xcd_remap = std::atoi(e) != 0;
+    if (const char *e = std::getenv("LW_SPL")) b->per_lane = (uint32_t)std::max(1, std::min(64, std::atoi(e)));
     if (const char *e = std::getenv("LW_WF_SLOTS")) {
       const long v = std::atol(e);
       if (v >= kBlock) b->wf_slots = (uint32_t)std::min<long>(v, 1l << 24) / kBlock * kBlock;

@@ -93,11 +93,14 @@ def test_batch_rejects_views_that_differ_beyond_the_camera(batch_setup):
         b.close()
 
 
-@pytest.mark.parametrize("env", [{"LW_XCD_REMAP": "1"}, {"LW_WAVES": "6"}, {"LW_WAVES": "8"}],
-                         ids=["xcd-remap", "6-waves", "8-waves"])
+@pytest.mark.parametrize("env", [{"LW_XCD_REMAP": "1"}, {"LW_WAVES": "6"}, {"LW_WAVES": "8"},
+                                 {"LW_SPL": "3"}, {"LW_SPL": "8", "LW_WAVES": "6"},
+                                 {"LW_SPL": "1"}],
+                         ids=["xcd-remap", "6-waves", "8-waves", "3-per-lane", "8-per-lane-6-waves",
+                              "1-per-lane"])
 def test_launch_variants_give_the_same_image(batch_setup, env):
-    # The XCD tile renumbering and the register budgets change only where and how a sample
-    # runs, never its value.
+    # The XCD tile renumbering, the register budgets and the samples per lane change only
+    # where and how a sample runs, never its value.
     base, views = batch_setup
     p, spp = views[3]
     ref = open_backend({})
