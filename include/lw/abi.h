@@ -337,6 +337,34 @@ int lw_hip_adjoint(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params
                    uint32_t spp, const lw_diff_desc *diff, const float *d_loss_d_image,
                    double *grads);
 
+/* ---- autotuning (no counterpart in the reference) ---- */
+
+#define LW_FORM_PERSISTENT 0u
+#define LW_FORM_DIRECT 1u
+#define LW_FORM_WAVEFRONT 2u
+
+typedef struct lw_tune_entry {
+  uint32_t form;      /* LW_FORM_* */
+  uint32_t waves;     /* the register budget: minimum waves per SIMD the kernel is built for */
+  uint32_t per_lane;  /* samples per lane (direct form) */
+  uint32_t pad;
+  double ms;          /* HIP-event time of the probe's path kernels */
+} lw_tune_entry;
+
+typedef struct lw_tune_result {
+  lw_tune_entry entries[16];
+  uint32_t n;         /* candidates timed */
+  uint32_t best;      /* index of the fastest, now the handle's setting */
+  uint32_t pad[2];
+} lw_tune_result;
+
+/* Times a probe of params' frame (its first probe_spp samples per pixel, one launch) with
+ * each launch shape the library has -- direct form at 4, 5, 6 and 8 waves per SIMD and 1 or
+ * 4 samples per lane, persistent form at 3 and 4, wavefront form -- and keeps the fastest
+ * for later renders and batches on this handle. Every shape gives the same images. */
+int lw_hip_autotune(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                    uint32_t probe_spp, lw_tune_result *out);
+
 /* Diagnostics (no counterpart in the reference): copies `bytes` between two device buffers
  * with a plain streaming kernel (k_copy) twice and reports the second copy's time -- the
  * known-size reference for hardware byte counters. */

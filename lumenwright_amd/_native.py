@@ -45,6 +45,7 @@ SIGNATURES = {
     "lw_hip_fit_step": (i32, [vp, vp, vp, vp]),
     "lw_hip_fit_params": (i32, [vp, vp, vp, vp, vp]),
     "lw_hip_fit_free": (None, [vp, vp]),
+    "lw_hip_autotune": (i32, [vp, vp, vp, u32, vp]),
     # host.h
     "lw_celestial_angle": (f32, [ctypes.c_int64]),
     "lw_sun_direction_from_angle": (None, [f32, vp]),

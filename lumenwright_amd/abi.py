@@ -127,6 +127,21 @@ class FitStatsC(ctypes.Structure):
                 ("samples", ctypes.c_uint64), ("step", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
+class TuneEntryC(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_uint32), ("waves", ctypes.c_uint32),
+                ("per_lane", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("ms", ctypes.c_double)]
+
+
+class TuneResultC(ctypes.Structure):
+    """lw_tune_result."""
+
+    _fields_ = [("entries", TuneEntryC * 16), ("n", ctypes.c_uint32), ("best", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32 * 2)]
+
+
+FORMS = {0: "persistent", 1: "direct", 2: "wavefront"}
+
+
 class TimingC(ctypes.Structure):
     """lw_timing."""
 

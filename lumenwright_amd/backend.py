@@ -14,7 +14,8 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _native
-from .abi import HIT, PARAMS, VIEW, BoundView, FitOptionsC, FitStatsC, HqFrame, TimingC
+from .abi import (FORMS, HIT, PARAMS, VIEW, BoundView, FitOptionsC, FitStatsC, HqFrame,
+                  TimingC, TuneResultC)
 
 LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
 
@@ -199,6 +200,17 @@ class HipBackend:
         self._check(_native.lib().lw_hip_adjoint(self.handle, scene.handle, _addr(p), spp,
                                                  ctypes.byref(desc), _addr(g), _addr(out)))
         return out
+
+    def autotune(self, scene: HipScene, params: np.ndarray, probe_spp: int = 4) -> dict:
+        """lw_hip_autotune: times each launch shape on a probe of the frame and keeps the
+        fastest on this handle; returns the timings and the choice."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        r = TuneResultC()
+        self._check(_native.lib().lw_hip_autotune(self.handle, scene.handle, _addr(p), probe_spp,
+                                                  ctypes.byref(r)))
+        rows = [{"form": FORMS[e.form], "waves": e.waves, "per_lane": e.per_lane, "ms": e.ms}
+                for e in r.entries[:r.n]]
+        return {"candidates": rows, "best": rows[r.best]}
 
     def copy_probe(self, nbytes: int) -> float:
         """Times a plain device-to-device copy of nbytes (ms); see lw_hip_copy_probe."""

@@ -1178,9 +1178,11 @@ constexpr uint32_t kMaxBatch = 64;         // render/cuda.rs:165
 // With `diff`, the samples are render_sample_diff's under its parameters (direct form).
 // With `out_f64`, it also receives the per-pixel means in f64 (sum / spp before the f32
 // rounding of out_rgb).
+// With `one_launch`, all spp samples go in one launch (up to kMaxBatch), for timing probes.
 void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
                   uint32_t spp_in, lw_progress_fn progress, void *user, float *out_rgb,
-                  const DiffDev *diff = nullptr, double *out_f64 = nullptr) {
+                  const DiffDev *diff = nullptr, double *out_f64 = nullptr,
+                  bool one_launch = false) {
   auto t_wall = std::chrono::steady_clock::now();
   if (!(layout_key(params_in) == sc.key))
     throw InvalidScene{"params do not match the layout the scene was validated with"};
@@ -1215,6 +1217,7 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   // power of two that gives it kFillBlocks workgroups (same samples, same summation order).
   uint32_t done = 0, batch = 1;
   while (batch < max_batch && (uint64_t)tiles * batch < kFillBlocks) batch *= 2;
+  if (one_launch) batch = max_batch;
   while (done < spp) {
     const uint32_t nb = std::min(batch, std::min(spp - done, max_batch));
     const auto start = std::chrono::steady_clock::now();
@@ -2004,6 +2007,54 @@ void lw_hip_fit_free(lw_hip_backend *b, lw_hip_fit *f) {
   if (!f) return;
   if (b) (void)hipSetDevice(b->device);
   delete f;
+}
+
+int lw_hip_autotune(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                    uint32_t probe_spp, lw_tune_result *out) {
+  if (!b || !scene || !params || !out) return LW_EINVAL;
+  const int mode0 = b->mode, waves0 = b->waves;
+  const uint32_t lane0 = b->per_lane;
+  const int rc = guarded(b, [&] {
+    struct Cand {
+      int mode, waves;
+      uint32_t per_lane;
+    };
+    static const Cand cands[] = {{MODE_DIRECT, 6, 1}, {MODE_DIRECT, 5, 1}, {MODE_DIRECT, 4, 1},
+                                 {MODE_DIRECT, 8, 1}, {MODE_DIRECT, 6, 4}, {MODE_DIRECT, 5, 4},
+                                 {MODE_PERSISTENT, 4, 1}, {MODE_PERSISTENT, 3, 1},
+                                 {MODE_WAVEFRONT, 4, 1}};
+    const uint32_t n = (uint32_t)(sizeof(cands) / sizeof(cands[0]));
+    lw_params p = *params;
+    const uint32_t spp = std::max(1u, std::min(probe_spp, kMaxBatch));
+    std::vector<float> img((size_t)p.frame[0] * p.frame[1] * 3);
+    std::memset(out, 0, sizeof(*out));
+    double best = 1e300;
+    for (uint32_t i = 0; i < n; i++) {
+      b->mode = cands[i].mode;
+      b->waves = cands[i].waves;
+      b->per_lane = cands[i].per_lane;
+      // warm (code object load), then time one launch of spp samples per pixel
+      render_frame(b, *scene, p, 1, nullptr, nullptr, img.data());
+      render_frame(b, *scene, p, spp, nullptr, nullptr, img.data(), nullptr, nullptr, true);
+      const double ms = b->timing.trace_ms;
+      out->entries[i] = lw_tune_entry{(uint32_t)cands[i].mode, (uint32_t)cands[i].waves,
+                                      cands[i].per_lane, 0u, ms};
+      if (ms < best) {
+        best = ms;
+        out->best = i;
+      }
+    }
+    out->n = n;
+    b->mode = cands[out->best].mode;
+    b->waves = cands[out->best].waves;
+    b->per_lane = cands[out->best].per_lane;
+  });
+  if (rc != LW_OK) {
+    b->mode = mode0;
+    b->waves = waves0;
+    b->per_lane = lane0;
+  }
+  return rc;
 }
 
 int lw_hip_copy_probe(lw_hip_backend *b, size_t bytes, double *ms) {

@@ -118,3 +118,24 @@ def test_launch_variants_give_the_same_image(batch_setup, env):
     finally:
         ref.close()
         alt.close()
+
+
+def test_autotune_picks_the_fastest_shape_and_keeps_the_images(batch_setup):
+    base, views = batch_setup
+    p, spp = views[3]
+    b = open_backend({})
+    try:
+        sc = b.upload(base)
+        before = b.render_hq_scene(sc, p, spp)
+        r = b.autotune(sc, p, probe_spp=2)
+        ms = [c["ms"] for c in r["candidates"]]
+        assert len(ms) >= 8 and all(m > 0 for m in ms)
+        assert r["best"]["ms"] == min(ms)
+        after = b.render_hq_scene(sc, p, spp)
+        assert np.array_equal(before.view("<u4"), after.view("<u4"))
+        outs = b.render_hq_batch(sc, views[:3])
+        for (pv, sv), img in zip(views[:3], outs):
+            assert np.array_equal(img.view("<u4"), b.render_hq_scene(sc, pv, sv).view("<u4"))
+        sc.free()
+    finally:
+        b.close()
