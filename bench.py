@@ -11,14 +11,25 @@ resident in HBM. W untimed warmup steps, then K timed steps; one JSON line.
 both ways. The images are identical (same samples, same summation order), so the two are
 compared at equal image error.
 
-Besides the throughput the line carries:
+--config 3: the differentiable render of the 8 x 8-chunk world with 16 emitter blocks
+(256 x 144, 64 spp, 4 bounces): a step is one forward and one adjoint (path replay) of an L2
+image loss over 115 parameters; the line carries the adjoint / forward kernel-time ratio.
+
+--config 4: the batched inverse fit -- 32 views of the config-1 world at 640 x 360, 128 spp,
+6 bounces, 192 albedo factors and 128 emitter-intensity groups -- a step is one fit step on
+the device (batched forward, loss, batched adjoint, Adam update); --fit-steps runs a longer
+fit and reports the parameter error against the ground truth.
+
+Besides the throughput the config-1 line carries:
   roofline      the path-tracing kernels' algorithmic scene bytes per launch (per-sample
                 bytes counted once by the instrumented oracle, profiles/b_alg.json, times
                 the samples of a launch) over their mean launch time (HIP events on the
-                library's stream), against the 8 TB/s HBM peak;
+                library's stream), against the 8 TB/s HBM peak, with the HBM bytes per launch
+                the kernel's counters measured (profiles/traffic.json);
   cpu_baseline  the oracle (the C++ restatement of the reference's CPU backend) on the
                 host's cores over a bounded slice of the same frame (the first S samples of
-                every pixel), timed in this run.
+                every pixel), timed in this run;
+  mrays_per_s   Msamples/s times the trace calls per sample the oracle counts.
 """
 
 import argparse
@@ -102,6 +113,16 @@ def roofline(config, samples, trace_ms, launches):
                     "(profiles/k_render_cfg1_720p_valu_l2.md)" if config == 1 else None}
 
 
+def mrays(config, msamples):
+    """Mrays/s: Msamples/s times the trace calls per sample (camera, extension, shadow and
+    emitter rays) the instrumented oracle counts for the workload (profiles/b_alg.json)."""
+    b_alg_path = ROOT / "profiles" / "b_alg.json"
+    if not b_alg_path.exists():
+        return None
+    b = json.loads(b_alg_path.read_text()).get(f"config{config}")
+    return msamples * b["traces_per_sample"] if b else None
+
+
 def kernel_form():
     form = os.environ.get("LW_KERNEL", "direct")
     form += f", waves {os.environ.get('LW_WAVES', '6')}"
@@ -120,6 +141,14 @@ def bench_plain(args, lw, frames):
     b = lw.HipBackend.open(0)
     scene = b.upload(frame)
     w, h, spp = cfg.width, cfg.height, cfg.spp
+    form = kernel_form()
+    tuned = None
+    if args.autotune:
+        tuned = b.autotune(scene, frame.params, probe_spp=4)
+        bt = tuned["best"]
+        form = f"{bt['form']}, waves {bt['waves']}" + (
+            f", {bt['per_lane']} samples per lane" if bt["per_lane"] > 1 else "")
+        log(f"autotune: {tuned}")
     for i in range(args.warmup):
         b.render_hq_scene(scene, frame.params, spp)
         log(f"warmup {i + 1}/{args.warmup}: {b.last_timing()}")
@@ -159,8 +188,10 @@ def bench_plain(args, lw, frames):
         "dtype": "f32 path arithmetic, f64 accumulation",
         "data": "synthetic (seeded value-noise voxel world, procedural sky)",
         "config": {"workload": cfg.name, "width": w, "height": h, "spp": spp,
-                   "max_bounces": cfg.max_bounces, "kernel": kernel_form(),
+                   "max_bounces": cfg.max_bounces, "kernel": form,
                    "parallelism": "1 GPU"},
+        "mrays_per_s": mrays(args.config, value),
+        "autotune": tuned,
         "kernel_ms_per_step": trace_ms / args.steps,
         "accumulate_ms_per_step": acc_ms / args.steps,
         "launches_per_step": launches / args.steps,
@@ -382,6 +413,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="also time a copy of this many bytes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch shape by timing probes first (lw_hip_autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the path "
