@@ -337,6 +337,30 @@ int lw_hip_adjoint(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params
                    uint32_t spp, const lw_diff_desc *diff, const float *d_loss_d_image,
                    double *grads);
 
+/* ---- post-processing on the device (crates/lumenwright/src/render/post.rs) ---- */
+
+/* tonemap (post.rs:148-170): n linear RGB pixels (n*3 floats) to packed display RGBA8,
+ * op one of LW_TONEMAP_* (include/lw/host.h), bytes identical to the reference's (the
+ * display encoding goes through byte thresholds made with libm's powf). */
+int lw_hip_tonemap(lw_hip_backend *b, const float *hdr, size_t n, float exposure, uint32_t op,
+                   uint32_t *out);
+
+/* tonemap_mix (post.rs:176-215): weight * op_a + (1 - weight) * op_b in display values. At
+ * weight >= 1 or <= 0 it is tonemap of one operator; inside, the display values are
+ * computed with the power in f64 rounded once, which is libm's powf except where powf is not
+ * correctly rounded (0.06% of the f32 inputs in [0, 1]), a byte differing only if that input
+ * also sits at a rounding boundary. */
+int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float exposure,
+                       uint32_t op_a, uint32_t op_b, float weight, uint32_t *out);
+
+/* downsample (post.rs:218-247): box-filters w x h RGBA8 by k into (w/k) x (h/k). */
+int lw_hip_downsample(lw_hip_backend *b, const uint32_t *px, uint32_t w, uint32_t h, uint32_t k,
+                      uint32_t *out);
+
+/* render_hq_scene followed by tonemap on the device: only the w*h RGBA8 words cross PCIe. */
+int lw_hip_render_hq_rgba8(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                           uint32_t spp, float exposure, uint32_t op, uint32_t *out_rgba8);
+
 /* ---- autotuning (no counterpart in the reference) ---- */
 
 #define LW_FORM_PERSISTENT 0u

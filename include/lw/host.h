@@ -109,6 +109,28 @@ void lw_hq_params(const lw_hq_options *o, float emitter_intensity, uint32_t widt
                   const lw_scene_info *scene, const float eye[3], const lw_camera_rays *rays,
                   const lw_hq_sky *sky, double turbidity, lw_params *out);
 
+/* ---- post-processing (crates/lumenwright/src/render/post.rs) ---- */
+
+/* Tone-mapping operators (post.rs:3-21). */
+#define LW_TONEMAP_GAMMA 0u
+#define LW_TONEMAP_ACES 1u
+#define LW_TONEMAP_FILMIC 2u
+#define LW_TONEMAP_FILMIC_MID_GREY 3u
+
+/* The Hable curve an operator uses (post.rs:37-138: Chunky's FILMIC_WORLDS preset, or it
+ * brightened by mid_grey_gain(0.18) for FilmicMidGrey): a, b, c, d, e, f, w, then
+ * white_scale() in out[7]. */
+void lw_hable_curve(uint32_t op, float out[8]);
+
+/* The byte thresholds of the display encoding (post.rs:141-144,165): out[k - 1] is the
+ * smallest f32 v with (clamp(v, 0, 1)^(1/2.2) * 255 + 0.5) as u32 >= k, with libm's powf
+ * (Rust std's f32::powf on Linux). The byte of v is then the number of thresholds <= v. */
+void lw_encode_thresholds(float out[255]);
+
+/* output.rs:19-30: writes hdr (w*h*3 floats, top row first) as a Portable Float Map,
+ * rows bottom to top. Returns 0, or -1 with lw_last_error set. */
+int lw_write_pfm(const char *path, const float *hdr, uint32_t w, uint32_t h);
+
 /* ---- synthetic worlds ---- */
 
 typedef struct lw_world_spec {

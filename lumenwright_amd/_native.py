@@ -46,6 +46,10 @@ SIGNATURES = {
     "lw_hip_fit_params": (i32, [vp, vp, vp, vp, vp]),
     "lw_hip_fit_free": (None, [vp, vp]),
     "lw_hip_autotune": (i32, [vp, vp, vp, u32, vp]),
+    "lw_hip_tonemap": (i32, [vp, vp, sz, f32, u32, vp]),
+    "lw_hip_tonemap_mix": (i32, [vp, vp, sz, f32, u32, u32, f32, vp]),
+    "lw_hip_downsample": (i32, [vp, vp, u32, u32, u32, vp]),
+    "lw_hip_render_hq_rgba8": (i32, [vp, vp, vp, u32, f32, u32, vp]),
     # host.h
     "lw_celestial_angle": (f32, [ctypes.c_int64]),
     "lw_sun_direction_from_angle": (None, [f32, vp]),
@@ -64,6 +68,9 @@ SIGNATURES = {
     "lw_world_build": (i32, [vp, ctypes.POINTER(vp)]),
     "lw_world_free": (None, [vp]),
     "lw_dusk_sky_maps": (ctypes.c_long, [vp, u32, u32, u32, u32, vp, sz]),
+    "lw_hable_curve": (None, [u32, vp]),
+    "lw_encode_thresholds": (None, [vp]),
+    "lw_write_pfm": (i32, [ctypes.c_char_p, vp, u32, u32]),
 }
 
 

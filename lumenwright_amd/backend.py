@@ -212,6 +212,40 @@ class HipBackend:
                 for e in r.entries[:r.n]]
         return {"candidates": rows, "best": rows[r.best]}
 
+    def tonemap(self, hdr: np.ndarray, exposure: float = 1.0, op: int = 0) -> np.ndarray:
+        """lw_hip_tonemap (render/post.rs:148-170): (n, 3) linear to n packed RGBA8."""
+        h = np.ascontiguousarray(hdr, dtype="<f4").reshape(-1, 3)
+        out = np.zeros(h.shape[0], dtype="<u4")
+        self._check(_native.lib().lw_hip_tonemap(self.handle, _addr(h), h.shape[0], exposure, op,
+                                                 _addr(out)))
+        return out
+
+    def tonemap_mix(self, hdr: np.ndarray, exposure: float, op_a: int, op_b: int,
+                    weight: float) -> np.ndarray:
+        """lw_hip_tonemap_mix (render/post.rs:176-215)."""
+        h = np.ascontiguousarray(hdr, dtype="<f4").reshape(-1, 3)
+        out = np.zeros(h.shape[0], dtype="<u4")
+        self._check(_native.lib().lw_hip_tonemap_mix(self.handle, _addr(h), h.shape[0], exposure,
+                                                     op_a, op_b, weight, _addr(out)))
+        return out
+
+    def downsample(self, px: np.ndarray, w: int, h: int, k: int) -> np.ndarray:
+        """lw_hip_downsample (render/post.rs:218-247)."""
+        a = np.ascontiguousarray(px, dtype="<u4").reshape(-1)
+        n = (w // k) * (h // k) if k > 1 else w * h
+        out = np.zeros(n, dtype="<u4")
+        self._check(_native.lib().lw_hip_downsample(self.handle, _addr(a), w, h, k, _addr(out)))
+        return out
+
+    def render_hq_rgba8(self, scene: HipScene, params: np.ndarray, spp: int,
+                        exposure: float = 1.0, op: int = 0) -> np.ndarray:
+        """lw_hip_render_hq_rgba8: the frame rendered and tone-mapped on the device."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        out = np.zeros(int(p["frame"][0]) * int(p["frame"][1]), dtype="<u4")
+        self._check(_native.lib().lw_hip_render_hq_rgba8(self.handle, scene.handle, _addr(p), spp,
+                                                         exposure, op, _addr(out)))
+        return out
+
     def copy_probe(self, nbytes: int) -> float:
         """Times a plain device-to-device copy of nbytes (ms); see lw_hip_copy_probe."""
         ms = ctypes.c_double()

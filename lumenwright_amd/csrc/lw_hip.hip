@@ -367,6 +367,123 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
   o[2] = c.z;
 }
 
+// ---- post-processing (render/post.rs) ----
+
+// A tone curve's inputs: the byte thresholds of the display encoding (host-made with libm's
+// powf: the byte of v is the number of thresholds <= v), and per operator its Hable curve
+// (a, b, c, d, e, f, w) and white scale.
+struct ToneDev {
+  float thr[255];
+  float exposure, weight;
+  uint32_t op_a, op_b;
+  float ca[7], cb[7];
+  double white_a, white_b;
+};
+
+// Rust std's f32::clamp: a NaN stays NaN (unlike the kernel crate's select-based clamp).
+__device__ __forceinline__ float clamp_std(float x, float lo, float hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// post.rs:23-26
+__device__ __forceinline__ float aces_f(float x) {
+  const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+  return clamp_std((x * (a * x + b)) / (x * (c * x + d) + e), 0.0f, 1.0f);
+}
+
+// post.rs:85-96: h(y) with the parameter products in f32 and the rest in f64.
+__device__ __forceinline__ double hable_h(const float *k, double y) {
+  const double a = (double)k[0], b = (double)k[1];
+  const double cb = (double)(k[2] * k[1]), de = (double)(k[3] * k[4]), df = (double)(k[3] * k[5]),
+               ef = (double)(k[4] / k[5]);
+  return (y * (a * y + cb) + de) / (y * (a * y + b) + df) - ef;
+}
+
+// The operator's value before the display gamma (post.rs:155-165).
+__device__ __forceinline__ float tone_linear(uint32_t op, const float *k, double white, float v,
+                                             float exposure) {
+  float x = v * exposure;
+  x = isnan(x) ? 0.0f : (x > 0.0f ? x : 0.0f);  // f32::max(x, 0.0): the non-NaN operand
+  if (op == LW_TONEMAP_GAMMA) return x;
+  if (op == LW_TONEMAP_ACES) return aces_f(x);
+  return (float)(hable_h(k, 2.0 * (double)x) * white);
+}
+
+// The display byte of v (post.rs:165): the number of thresholds <= v (NaN: none).
+__device__ __forceinline__ uint32_t tone_byte(const float *thr, float v) {
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t step = 128; step > 0; step >>= 1)
+    if (c + step <= 255u && thr[c + step - 1] <= v) c += step;
+  return c;
+}
+
+__device__ __forceinline__ void load_thr(float *thr_s, const ToneDev &t) {
+  for (int i = threadIdx.x; i < 255; i += blockDim.x) thr_s[i] = t.thr[i];
+  __syncthreads();
+}
+
+// tonemap (post.rs:148-170), one lane per pixel.
+__global__ void __launch_bounds__(kBlock) k_tonemap(const float *__restrict__ hdr, uint32_t n,
+                                                    const ToneDev t, uint32_t *__restrict__ out) {
+  __shared__ float thr_s[256];
+  load_thr(thr_s, t);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t px = 0xFF000000u;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    px |= tone_byte(thr_s, tone_linear(t.op_a, t.ca, t.white_a, hdr[3 * (size_t)i + c], t.exposure))
+          << (8 * c);
+  out[i] = px;
+}
+
+// The display encoding in f32 (post.rs:141-144) as the mix needs the value itself: the
+// power in f64, rounded once to f32.
+__device__ __forceinline__ float encode_f(float v) {
+  const float x = clamp_std(v, 0.0f, 1.0f);
+  return (float)pow((double)x, (double)(1.0f / 2.2f));
+}
+
+// tonemap_mix (post.rs:176-215) for 0 < weight < 1.
+__global__ void __launch_bounds__(kBlock) k_tonemap_mix(const float *__restrict__ hdr, uint32_t n,
+                                                        const ToneDev t,
+                                                        uint32_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t px = 0xFF000000u;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float v = hdr[3 * (size_t)i + c];
+    const float d = t.weight * encode_f(tone_linear(t.op_a, t.ca, t.white_a, v, t.exposure)) +
+                    (1.0f - t.weight) * encode_f(tone_linear(t.op_b, t.cb, t.white_b, v, t.exposure));
+    const float b = d * 255.0f + 0.5f;
+    px |= (b > 0.0f ? (uint32_t)b : 0u) << (8 * c);
+  }
+  out[i] = px;
+}
+
+// downsample (post.rs:218-247), one lane per output pixel.
+__global__ void __launch_bounds__(kBlock) k_downsample(const uint32_t *__restrict__ px, uint32_t w,
+                                                       uint32_t k, uint32_t ow, uint32_t oh,
+                                                       uint32_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ow * oh) return;
+  const uint32_t x = i % ow, y = i / ow;
+  uint32_t acc[4] = {0u, 0u, 0u, 0u};
+  for (uint32_t dy = 0; dy < k; dy++)
+    for (uint32_t dx = 0; dx < k; dx++) {
+      const uint32_t c = px[(size_t)(y * k + dy) * w + x * k + dx];
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[j] += (c >> (8 * j)) & 0xFFu;
+    }
+  const uint32_t nk = k * k;
+  uint32_t v = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) v += ((acc[j] + nk / 2) / nk) << (8 * j);
+  out[i] = v;
+}
+
 // A plain streaming copy, 16 B per lane per step: the reference point that turns hardware
 // byte counters into bytes for the path kernels.
 __global__ void __launch_bounds__(kBlock) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
@@ -892,6 +1009,8 @@ struct lw_hip_backend {
   uint32_t batch_passes = 0;
   bool use_graphs = true;
   DevBuf<BatchDev> batch_dev;
+  DevBuf<float> tone_in;
+  DevBuf<uint32_t> tone_out;
   BatchPlan plan;  // the last batch's layout (its sums stay in acc when it keeps them)
   // Differentiable rendering: the parameters and derivative inputs of the last call.
   DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
@@ -1271,8 +1390,9 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
         (uint32_t)npix, spp, b->acc.ptr, b->out.ptr);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipMemcpyAsync(out_rgb, b->out.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost,
-                         b->stream));
+  if (out_rgb)  // without it the image stays in b->out for the caller's next kernel
+    HIP_TRY(hipMemcpyAsync(out_rgb, b->out.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                           b->stream));
   if (out_f64)
     HIP_TRY(hipMemcpyAsync(out_f64, b->acc.ptr, npix * 3 * sizeof(double), hipMemcpyDeviceToHost,
                            b->stream));
@@ -1624,6 +1744,53 @@ void adjoint_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   tm.trace_ms = ms;
   tm.launches = plan.npass;
   tm.samples = plan.samples;
+}
+
+// The ToneDev of an operator pair (post.rs:148-215).
+ToneDev make_tone(float exposure, uint32_t op_a, uint32_t op_b, float weight) {
+  if (op_a > LW_TONEMAP_FILMIC_MID_GREY || op_b > LW_TONEMAP_FILMIC_MID_GREY)
+    throw InvalidScene{"unknown tone-mapping operator"};
+  ToneDev t{};
+  lw_encode_thresholds(t.thr);
+  float k[8];
+  lw_hable_curve(op_a, k);
+  std::memcpy(t.ca, k, sizeof(t.ca));
+  t.white_a = (double)k[7];
+  lw_hable_curve(op_b, k);
+  std::memcpy(t.cb, k, sizeof(t.cb));
+  t.white_b = (double)k[7];
+  t.exposure = exposure;
+  t.weight = weight;
+  t.op_a = op_a;
+  t.op_b = op_b;
+  return t;
+}
+
+// Tone-maps n pixels already on the device (in) into b->tone_out and copies them to out.
+void tonemap_device(lw_hip_backend *b, const float *in, size_t n, const ToneDev &t, bool mix,
+                    uint32_t *out) {
+  if (n > 0xFFFFFFFFull) throw InvalidScene{"image too large"};
+  b->tone_out.ensure(std::max<size_t>(n, 1));
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  if (n) {
+    if (mix)
+      k_tonemap_mix<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->tone_out.ptr);
+    else
+      k_tonemap<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->tone_out.ptr);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  if (n)
+    HIP_TRY(hipMemcpyAsync(out, b->tone_out.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  lw_timing tm{};
+  tm.trace_ms = ms;
+  tm.launches = 1;
+  tm.samples = n;
+  b->timing = tm;
 }
 
 template <typename F>
@@ -2055,6 +2222,70 @@ int lw_hip_autotune(lw_hip_backend *b, const lw_hip_scene *scene, const lw_param
     b->per_lane = lane0;
   }
   return rc;
+}
+
+int lw_hip_tonemap(lw_hip_backend *b, const float *hdr, size_t n, float exposure, uint32_t op,
+                   uint32_t *out) {
+  if (!b || (n && (!hdr || !out))) return LW_EINVAL;
+  return guarded(b, [&] {
+    HIP_TRY(hipSetDevice(b->device));
+    const ToneDev t = make_tone(exposure, op, op, 1.0f);
+    b->tone_in.upload(hdr, n * 3, b->stream);
+    tonemap_device(b, b->tone_in.ptr, n, t, false, out);
+  });
+}
+
+int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float exposure,
+                       uint32_t op_a, uint32_t op_b, float weight, uint32_t *out) {
+  if (!b || (n && (!hdr || !out))) return LW_EINVAL;
+  return guarded(b, [&] {
+    HIP_TRY(hipSetDevice(b->device));
+    // post.rs:183-188: the ends are each curve alone, bytes included
+    const bool ends = weight >= 1.0f || weight <= 0.0f;
+    const uint32_t op = weight >= 1.0f ? op_a : op_b;
+    const ToneDev t = ends ? make_tone(exposure, op, op, 1.0f) : make_tone(exposure, op_a, op_b, weight);
+    b->tone_in.upload(hdr, n * 3, b->stream);
+    tonemap_device(b, b->tone_in.ptr, n, t, !ends, out);
+  });
+}
+
+int lw_hip_downsample(lw_hip_backend *b, const uint32_t *px, uint32_t w, uint32_t h, uint32_t k,
+                      uint32_t *out) {
+  if (!b || !px || !out) return LW_EINVAL;
+  return guarded(b, [&] {
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t n = (size_t)w * h;
+    if (k <= 1) {
+      std::memcpy(out, px, n * sizeof(uint32_t));
+      return;
+    }
+    const uint32_t ow = w / k, oh = h / k;
+    const size_t m = (size_t)ow * oh;
+    DevBuf<uint32_t> in, res;
+    in.upload(px, n, b->stream);
+    res.alloc(std::max<size_t>(m, 1));
+    if (m) {
+      k_downsample<<<dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+          in.ptr, w, k, ow, oh, res.ptr);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(out, res.ptr, m * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  });
+}
+
+int lw_hip_render_hq_rgba8(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
+                           uint32_t spp, float exposure, uint32_t op, uint32_t *out_rgba8) {
+  if (!b || !scene || !params || !out_rgba8) return LW_EINVAL;
+  return guarded(b, [&] {
+    const ToneDev t = make_tone(exposure, op, op, 1.0f);
+    render_frame(b, *scene, *params, spp, nullptr, nullptr, nullptr);
+    const lw_timing render = b->timing;
+    tonemap_device(b, b->out.ptr, (size_t)params->frame[0] * params->frame[1], t, false, out_rgba8);
+    const double tone_ms = b->timing.trace_ms;
+    b->timing = render;
+    b->timing.accumulate_ms += tone_ms;
+  });
 }
 
 int lw_hip_copy_probe(lw_hip_backend *b, size_t bytes, double *ms) {

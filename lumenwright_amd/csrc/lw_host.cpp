@@ -8,6 +8,7 @@
 #include <array>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -834,3 +835,87 @@ long lw_dusk_sky_maps(const float sun_dir[3], uint32_t lw, uint32_t lh, uint32_t
 }
 
 }  // extern "C"
+
+// ---- post-processing (render/post.rs) ----
+
+namespace {
+
+struct Hable {
+  float a, b, c, d, e, f, w;
+  // post.rs:70-83: 1 / h(W) in float
+  float white_scale() const {
+    const float num = w * (a * w + c * b) + d * e;
+    const float den = w * (a * w + b) + d * f;
+    return 1.0f / (num / den - e / f);
+  }
+  // post.rs:113-121: A k^2, B k, W / k through f64
+  Hable brightened(double k) const {
+    Hable r = *this;
+    r.a = (float)((double)a * k * k);
+    r.b = (float)((double)b * k);
+    r.w = (float)((double)w / k);
+    return r;
+  }
+  // post.rs:122-131: the positive root of the quadratic in y = 2 k m
+  double mid_grey_gain(double m) const {
+    const double aa = a, bb = b, cb = (double)(c * b), de = (double)(d * e), df = (double)(d * f),
+                 ef = (double)(e / f);
+    const double q = m / (double)white_scale() + ef;
+    const double qa = aa * (1.0 - q), qb = cb - q * bb, qc = de - q * df;
+    const double y = (-qb + std::sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa);
+    return y / (2.0 * m);
+  }
+};
+
+constexpr Hable kFilmicWorlds = {0.15f, 0.5f, 0.1f, 0.2f, 0.02f, 0.3f, 11.2f};  // post.rs:55-63
+
+uint32_t display_byte(float v) {  // post.rs:141-144,165 with Rust's saturating `as u32`
+  float x = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  const float d = std::pow(x, 1.0f / 2.2f) * 255.0f + 0.5f;
+  if (!(d > 0.0f)) return 0;
+  return (uint32_t)d;
+}
+
+}  // namespace
+
+void lw_hable_curve(uint32_t op, float out[8]) {
+  Hable h = kFilmicWorlds;
+  if (op == LW_TONEMAP_FILMIC_MID_GREY) h = kFilmicWorlds.brightened(kFilmicWorlds.mid_grey_gain(0.18));
+  const float v[8] = {h.a, h.b, h.c, h.d, h.e, h.f, h.w, h.white_scale()};
+  std::memcpy(out, v, sizeof(v));
+}
+
+void lw_encode_thresholds(float out[255]) {
+  // display_byte is non-decreasing over [0, 1] (tests check every f32 there), so each
+  // threshold is found by bisection over the bit patterns of [0, 1].
+  for (uint32_t k = 1; k <= 255; k++) {
+    uint32_t lo = 0, hi = 0x3f800000u;  // display_byte(1) = 255 >= k
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      float v;
+      std::memcpy(&v, &mid, 4);
+      if (display_byte(v) >= k)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    std::memcpy(&out[k - 1], &lo, 4);
+  }
+}
+
+int lw_write_pfm(const char *path, const float *hdr, uint32_t w, uint32_t h) {
+  FILE *f = path ? std::fopen(path, "wb") : nullptr;
+  if (!f) {
+    g_host_error = std::string("cannot open ") + (path ? path : "(null)");
+    return -1;
+  }
+  const std::string head = "PF\n" + std::to_string(w) + " " + std::to_string(h) + "\n-1.0\n";
+  bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
+  for (uint32_t yy = 0; ok && yy < h; yy++) {
+    const float *row = hdr + (size_t)(h - 1 - yy) * w * 3;
+    ok = std::fwrite(row, 4, (size_t)w * 3, f) == (size_t)w * 3;  // little-endian host
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) g_host_error = std::string("writing ") + path;
+  return ok ? 0 : -1;
+}

@@ -189,6 +189,33 @@ def dusk_sky_maps(sun_dir, lens=(2048, 1024), light=(360, 180)) -> np.ndarray:
     return out
 
 
+# ---- post-processing host helpers (render/post.rs, lumenwright-cli output.rs) ----
+
+TONEMAP_GAMMA, TONEMAP_ACES, TONEMAP_FILMIC, TONEMAP_FILMIC_MID_GREY = 0, 1, 2, 3
+
+
+def hable_curve(op: int) -> np.ndarray:
+    """The operator's Hable curve (a, b, c, d, e, f, w) and its white scale (post.rs:37-138)."""
+    out = np.zeros(8, dtype="<f4")
+    _native.lib().lw_hable_curve(op, ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def encode_thresholds() -> np.ndarray:
+    """The 255 byte thresholds of the display encoding (libm powf)."""
+    out = np.zeros(255, dtype="<f4")
+    _native.lib().lw_encode_thresholds(ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def write_pfm(path: str, hdr: np.ndarray, w: int, h: int) -> None:
+    """output.rs:19-30: a Portable Float Map, rows bottom to top."""
+    a = np.ascontiguousarray(hdr, dtype="<f4").reshape(-1)
+    assert a.size == w * h * 3
+    if _native.lib().lw_write_pfm(str(path).encode(), ctypes.c_void_p(a.ctypes.data), w, h) != 0:
+        raise OSError(f"writing {path}")
+
+
 # ---- synthetic worlds ----
 
 @dataclass(frozen=True)

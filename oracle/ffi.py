@@ -78,6 +78,21 @@ def lib():
         L.lwo_adjoint.restype = i32
         L.lwo_render_diff_dot.argtypes = [vp, u32, vp, vp, ctypes.c_uint, vp]
         L.lwo_render_diff_dot.restype = i32
+        L.lwo_tonemap.argtypes = [vp, sz, f32, u32, vp]
+        L.lwo_tonemap.restype = None
+        L.lwo_tonemap_mix.argtypes = [vp, sz, f32, u32, u32, f32, vp]
+        L.lwo_tonemap_mix.restype = None
+        L.lwo_downsample.argtypes = [vp, u32, u32, u32, vp]
+        L.lwo_downsample.restype = None
+        L.lwo_hable.argtypes = [u32, vp, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                ctypes.POINTER(ctypes.c_double)]
+        L.lwo_hable.restype = None
+        L.lwo_encode.argtypes = [vp, sz, vp]
+        L.lwo_encode.restype = None
+        L.lwo_pfm.argtypes = [vp, u32, u32, vp, sz]
+        L.lwo_pfm.restype = sz
+        L.lwo_check_encode_thresholds.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(u32)]
+        L.lwo_check_encode_thresholds.restype = ctypes.c_uint64
         L.lwo_sky_tangent.argtypes = [vp, vp, i32, vp, sz, vp]
         L.lwo_sky_tangent.restype = i32
         L.lwo_preetham_radiance.argtypes = [vp, vp, sz, vp]
@@ -346,3 +361,59 @@ def water_interface(params, pos, d, n, front, in_water, eps, rng_state):
                                    float(eps), rng_state, _addr(d_out), _addr(o_out),
                                    ctypes.byref(w))
     return d_out, o_out, bool(w.value), st
+
+
+# ---- post-processing (crates/lumenwright/src/render/post.rs) ----
+
+def tonemap(hdr, exposure: float, op: int):
+    h = _f32(hdr).reshape(-1, 3)
+    out = np.zeros(h.shape[0], dtype="<u4")
+    lib().lwo_tonemap(_addr(h), h.shape[0], exposure, op, _addr(out))
+    return out
+
+
+def tonemap_mix(hdr, exposure: float, a: int, b: int, weight: float):
+    h = _f32(hdr).reshape(-1, 3)
+    out = np.zeros(h.shape[0], dtype="<u4")
+    lib().lwo_tonemap_mix(_addr(h), h.shape[0], exposure, a, b, weight, _addr(out))
+    return out
+
+
+def downsample(px, w: int, h: int, k: int):
+    a = np.ascontiguousarray(px, dtype="<u4").reshape(-1)
+    n = (w // k) * (h // k) if k > 1 else w * h
+    out = np.zeros(n, dtype="<u4")
+    lib().lwo_downsample(_addr(a), w, h, k, _addr(out))
+    return out
+
+
+def hable(op: int, x: float = 0.18):
+    """(a, b, c, d, e, f, w, white_scale), mid_grey_gain(0.18), before_gamma(x)."""
+    out = np.zeros(8, dtype="<f4")
+    g, bg = ctypes.c_double(), ctypes.c_double()
+    lib().lwo_hable(op, _addr(out), ctypes.byref(g), x, ctypes.byref(bg))
+    return out, g.value, bg.value
+
+
+def encode(v):
+    v = _f32(v)
+    out = np.empty_like(v)
+    lib().lwo_encode(_addr(v), v.size, _addr(out))
+    return out
+
+
+def pfm(hdr, w: int, h: int) -> bytes:
+    a = _f32(hdr).reshape(-1)
+    n = lib().lwo_pfm(_addr(a), w, h, None, 0)
+    buf = np.zeros(n, dtype=np.uint8)
+    lib().lwo_pfm(_addr(a), w, h, _addr(buf), n)
+    return buf.tobytes()
+
+
+def check_encode_thresholds(thr, threads: int = 0):
+    """Mismatches of the threshold byte against post.rs's byte over every f32 in [0, 1]."""
+    t = _f32(thr)
+    first = ctypes.c_uint32()
+    n = lib().lwo_check_encode_thresholds(_addr(t), threads or os.cpu_count() or 1,
+                                          ctypes.byref(first))
+    return n, first.value

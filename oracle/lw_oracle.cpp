@@ -6,7 +6,9 @@
 #define LWO_COUNTING 0
 #include "lw_oracle_impl.hpp"
 #include "lw_oracle_diff.hpp"
+#include "lw_oracle_post.hpp"
 
+#include <algorithm>
 #include <cstdio>
 
 using namespace lwo;
@@ -290,6 +292,88 @@ int lwo_adjoint(const View *view, uint32_t spp, const DiffDesc *diff, const floa
                 unsigned threads, double *grads) {
   if (threads == 0) threads = 1;
   return guarded([&] { adjoint(*view, spp, *diff, dldi, threads, grads); });
+}
+
+
+// ---- post-processing (render/post.rs), lw_oracle_post.hpp ----
+
+// tonemap (post.rs:148-170); op: 0 Gamma, 1 Aces, 2 Filmic, 3 FilmicMidGrey.
+void lwo_tonemap(const float *hdr, size_t n, float exposure, uint32_t op, uint32_t *out) {
+  lwo_post::tonemap(hdr, n, exposure, op, out);
+}
+
+// tonemap_mix (post.rs:176-215)
+void lwo_tonemap_mix(const float *hdr, size_t n, float exposure, uint32_t a, uint32_t b,
+                     float weight, uint32_t *out) {
+  lwo_post::tonemap_mix(hdr, n, exposure, a, b, weight, out);
+}
+
+// downsample (post.rs:218-247); out holds (w / k) * (h / k) words (w * h when k <= 1).
+void lwo_downsample(const uint32_t *px, uint32_t w, uint32_t h, uint32_t k, uint32_t *out) {
+  auto v = lwo_post::downsample(px, w, h, k);
+  std::memcpy(out, v.data(), v.size() * 4);
+}
+
+// The Hable curve of an op (post.rs:37-138): a, b, c, d, e, f, w, white_scale; and the
+// game curve's gain mid_grey_gain(0.18) and before_gamma(x) for x.
+void lwo_hable(uint32_t op, float out[8], double *gain, double x, double *before) {
+  const lwo_post::HableCurve c = lwo_post::curve_of(op);
+  const float v[8] = {c.a, c.b, c.c, c.d, c.e, c.f, c.w, c.white_scale()};
+  std::memcpy(out, v, sizeof(v));
+  *gain = lwo_post::HABLE_FILMIC_WORLDS.mid_grey_gain(lwo_post::MID_GREY);
+  *before = c.before_gamma(x);
+}
+
+// encode (post.rs:141-144) elementwise.
+void lwo_encode(const float *v, size_t n, float *out) {
+  for (size_t i = 0; i < n; i++) out[i] = lwo_post::encode(v[i]);
+}
+
+// The PFM file of an image (output.rs:19-30); returns its size, writing it when cap allows.
+size_t lwo_pfm(const float *hdr, uint32_t w, uint32_t h, uint8_t *out, size_t cap) {
+  std::string s = lwo_post::pfm(hdr, w, h);
+  if (out && cap >= s.size()) std::memcpy(out, s.data(), s.size());
+  return s.size();
+}
+
+// Every f32 v in [0, 1] (and 1.5, +inf): the count of thresholds <= v against post.rs's
+// byte (encode(v) * 255 + 0.5) as u32, on `threads` threads. Returns the number of
+// mismatches and the first mismatching input in *first.
+uint64_t lwo_check_encode_thresholds(const float *thr, unsigned threads, uint32_t *first) {
+  const uint32_t one = 0x3f800000u;
+  if (threads == 0) threads = 1;
+  for (int k = 1; k < 255; k++)
+    if (!(thr[k - 1] <= thr[k])) {
+      *first = (uint32_t)k;
+      return ~0ull;  // not sorted
+    }
+  std::vector<uint64_t> bad(threads, 0);
+  std::vector<uint32_t> fst(threads, 0xFFFFFFFFu);
+  auto work = [&](unsigned t) {
+    const uint32_t lo = (uint32_t)((uint64_t)(one + 1) * t / threads);
+    const uint32_t hi = (uint32_t)((uint64_t)(one + 1) * (t + 1) / threads);
+    for (uint32_t u = lo; u < hi; u++) {
+      float v;
+      std::memcpy(&v, &u, 4);
+      uint32_t want = lwo_post::as_u32_sat(lwo_post::encode(v) * 255.0f + 0.5f);
+      uint32_t got = (uint32_t)(std::upper_bound(thr, thr + 255, v) - thr);  // sorted: checked
+      if (got != want) {
+        if (!bad[t]) fst[t] = u;
+        bad[t]++;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; t++) pool.emplace_back(work, t);
+  work(0);
+  for (auto &t : pool) t.join();
+  uint64_t total = 0;
+  *first = 0xFFFFFFFFu;
+  for (unsigned t = 0; t < threads; t++) {
+    total += bad[t];
+    if (bad[t] && fst[t] < *first) *first = fst[t];
+  }
+  return total;
 }
 
 }  // extern "C"
