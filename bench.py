@@ -29,7 +29,8 @@ Besides the throughput the config-1 line carries:
   cpu_baseline  the oracle (the C++ restatement of the reference's CPU backend) on the
                 host's cores over a bounded slice of the same frame (the first S samples of
                 every pixel), timed in this run;
-  mrays_per_s   Msamples/s times the trace calls per sample the oracle counts.
+  mrays_per_s   Msamples/s times the trace calls per sample the oracle counts;
+  post          the device tonemap and tone mix of the last frame (kernel time, GB/s).
 """
 
 import argparse
@@ -131,6 +132,20 @@ def kernel_form():
     return form
 
 
+def post_timing(b, frames, img, reps: int = 5) -> dict:
+    """The device tonemap and tone mix of the rendered frame (post.rs:148-215): the best
+    kernel time of a few calls (HIP events around the one kernel) and its 16 B per pixel
+    (12 B of linear RGB read, 4 B of RGBA8 written) over that time."""
+    n = img.shape[0]
+    out = {"pixels": n}
+    for name, call in (("tonemap", lambda: b.tonemap(img, 1.0, frames.TONEMAP_FILMIC_MID_GREY)),
+                       ("tonemap_mix", lambda: b.tonemap_mix(img, 1.0, frames.TONEMAP_GAMMA,
+                                                             frames.TONEMAP_FILMIC_MID_GREY, 0.5))):
+        best = min((call(), b.last_timing()["trace_ms"])[1] for _ in range(reps))
+        out[name] = {"kernel_ms": best, "GBps": 16 * n / best / 1e6}
+    return out
+
+
 def bench_plain(args, lw, frames):
     cfg = frames.CONFIGS[args.config]
     if args.spp:
@@ -165,7 +180,8 @@ def bench_plain(args, lw, frames):
     samples = w * h * spp * args.steps
     value = samples / elapsed / 1e6
     assert np.isfinite(img).all()
-    extra = {}
+    extra = {"post": post_timing(b, frames, img)}
+    log(f"post-processing: {extra['post']}")
     if args.copy_probe:
         ms = b.copy_probe(args.copy_probe)
         extra["copy_probe"] = {"bytes": args.copy_probe, "ms": ms,

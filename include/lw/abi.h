@@ -346,12 +346,16 @@ int lw_hip_tonemap(lw_hip_backend *b, const float *hdr, size_t n, float exposure
                    uint32_t *out);
 
 /* tonemap_mix (post.rs:176-215): weight * op_a + (1 - weight) * op_b in display values. At
- * weight >= 1 or <= 0 it is tonemap of one operator; inside, the display values are
- * computed with the power in f64 rounded once, which is libm's powf except where powf is not
- * correctly rounded (0.06% of the f32 inputs in [0, 1]), a byte differing only if that input
- * also sits at a rounding boundary. */
+ * weight >= 1 or <= 0 it is tonemap of one operator. Inside, the display values are libm's
+ * powf(x, 1/2.2f) as in lw_hip_encode, so the bytes are the reference's. */
 int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float exposure,
                        uint32_t op_a, uint32_t op_b, float weight, uint32_t *out);
+
+/* The display encoding (post.rs:141-144): clamp(x, 0, 1)^(1/2.2f) for n values, equal to the
+ * reference's f32::powf (libm's powf) for every f32. The power is taken in f64 and rounded
+ * once, and the inputs where libm's powf rounds otherwise come from a table the backend makes
+ * on its first call with libm itself; setup_ms (may be null) receives that one-time cost. */
+int lw_hip_encode(lw_hip_backend *b, const float *x, size_t n, float *out, double *setup_ms);
 
 /* downsample (post.rs:218-247): box-filters w x h RGBA8 by k into (w/k) x (h/k). */
 int lw_hip_downsample(lw_hip_backend *b, const uint32_t *px, uint32_t w, uint32_t h, uint32_t k,

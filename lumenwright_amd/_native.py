@@ -48,6 +48,7 @@ SIGNATURES = {
     "lw_hip_autotune": (i32, [vp, vp, vp, u32, vp]),
     "lw_hip_tonemap": (i32, [vp, vp, sz, f32, u32, vp]),
     "lw_hip_tonemap_mix": (i32, [vp, vp, sz, f32, u32, u32, f32, vp]),
+    "lw_hip_encode": (i32, [vp, vp, sz, vp, vp]),
     "lw_hip_downsample": (i32, [vp, vp, u32, u32, u32, vp]),
     "lw_hip_render_hq_rgba8": (i32, [vp, vp, vp, u32, f32, u32, vp]),
     # host.h

@@ -229,6 +229,16 @@ class HipBackend:
                                                      op_a, op_b, weight, _addr(out)))
         return out
 
+    def encode(self, x: np.ndarray) -> np.ndarray:
+        """lw_hip_encode (render/post.rs:141-144): clamp(x, 0, 1)^(1/2.2) as libm's powf."""
+        a = np.ascontiguousarray(x, dtype="<f4").reshape(-1)
+        out = np.zeros_like(a)
+        ms = ctypes.c_double()
+        self._check(_native.lib().lw_hip_encode(self.handle, _addr(a), a.size, _addr(out),
+                                                ctypes.byref(ms)))
+        self.encode_setup_ms = ms.value
+        return out
+
     def downsample(self, px: np.ndarray, w: int, h: int, k: int) -> np.ndarray:
         """lw_hip_downsample (render/post.rs:218-247)."""
         a = np.ascontiguousarray(px, dtype="<u4").reshape(-1)

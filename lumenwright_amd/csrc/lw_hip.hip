@@ -438,29 +438,93 @@ __global__ void __launch_bounds__(kBlock) k_tonemap(const float *__restrict__ hd
   out[i] = px;
 }
 
-// The display encoding in f32 (post.rs:141-144) as the mix needs the value itself: the
-// power in f64, rounded once to f32.
-__device__ __forceinline__ float encode_f(float v) {
+// The display encoding as the mix needs the value itself (post.rs:141-144): libm's
+// powf(x, 1/2.2f), which stays within one ulp but is not always correctly rounded. Here the
+// power is taken in f64 (error near 2^-52) and rounded once to f32. Over every f32 in [0, 1],
+// libm's result differs from the correctly rounded one only where the exact power lies within
+// 2^-33.2 (relative) of an f32 rounding midpoint, so an input whose f64 power lies within
+// kPowBand = 2^-32 of one is "in the band". For those, and only those, the result comes from
+// a table the host makes once with libm itself (ensure_pow_fix): the sorted pairs (input bits,
+// libm result bits) of the band inputs where libm and the rounded f64 power differ.
+constexpr double kPowBand = 0x1p-32;
+
+// The rounded f64 power of x in [0, 1] (or NaN) and whether x is in the band.
+__device__ __forceinline__ float pow_display(float x, bool &in_band) {
+  const double p = pow((double)x, (double)(1.0f / 2.2f));
+  const float c = (float)p;
+  in_band = false;
+  if (c > 0.0f && c <= 1.0f) {
+    const uint32_t cb = __float_as_uint(c);
+    const float nb = __uint_as_float((double)c <= p ? cb + 1u : cb - 1u);
+    const double mid = ((double)c + (double)nb) * 0.5;  // exact: adjacent f32 values
+    in_band = fabs(p - mid) < kPowBand * p;
+  }
+  return c;
+}
+
+__device__ __forceinline__ float encode_f(float v, const uint2 *__restrict__ fix, uint32_t nfix) {
   const float x = clamp_std(v, 0.0f, 1.0f);
-  return (float)pow((double)x, (double)(1.0f / 2.2f));
+  bool in_band;
+  const float c = pow_display(x, in_band);
+  if (!in_band) return c;
+  const uint32_t u = __float_as_uint(x);
+  uint32_t lo = 0, hi = nfix;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (fix[mid].x < u) lo = mid + 1; else hi = mid;
+  }
+  return (lo < nfix && fix[lo].x == u) ? __uint_as_float(fix[lo].y) : c;
+}
+
+// The band inputs among the f32 bit patterns [0, n) with their rounded f64 powers, appended
+// to out (one atomic per wave; entries past cap are counted, not written).
+__global__ void __launch_bounds__(kBlock) k_pow_band(uint32_t n, uint2 *__restrict__ out,
+                                                     uint32_t *__restrict__ count, uint32_t cap) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // Every lane of a wave runs the same trip count, so the ballot sees the whole wave.
+  const uint32_t base0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  for (uint32_t w0 = base0; w0 < n; w0 += stride) {
+    const uint32_t u = w0 + lane;
+    bool in_band = false;
+    float c = 0.0f;
+    if (u < n) c = pow_display(__uint_as_float(u), in_band);
+    const uint64_t m = __ballot(in_band);
+    if (m == 0) continue;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (in_band) {
+      const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (slot < cap) out[slot] = make_uint2(u, __float_as_uint(c));
+    }
+  }
 }
 
 // tonemap_mix (post.rs:176-215) for 0 < weight < 1.
 __global__ void __launch_bounds__(kBlock) k_tonemap_mix(const float *__restrict__ hdr, uint32_t n,
-                                                        const ToneDev t,
-                                                        uint32_t *__restrict__ out) {
+                                                        const ToneDev t, const uint2 *__restrict__ fix,
+                                                        uint32_t nfix, uint32_t *__restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t px = 0xFF000000u;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const float v = hdr[3 * (size_t)i + c];
-    const float d = t.weight * encode_f(tone_linear(t.op_a, t.ca, t.white_a, v, t.exposure)) +
-                    (1.0f - t.weight) * encode_f(tone_linear(t.op_b, t.cb, t.white_b, v, t.exposure));
+    const float d = t.weight * encode_f(tone_linear(t.op_a, t.ca, t.white_a, v, t.exposure), fix, nfix) +
+                    (1.0f - t.weight) * encode_f(tone_linear(t.op_b, t.cb, t.white_b, v, t.exposure), fix, nfix);
     const float b = d * 255.0f + 0.5f;
     px |= (b > 0.0f ? (uint32_t)b : 0u) << (8 * c);
   }
   out[i] = px;
+}
+
+// The display encoding of n values (post.rs:141-144), as the mix computes it.
+__global__ void __launch_bounds__(kBlock) k_encode(const float *__restrict__ x, uint32_t n,
+                                                   const uint2 *__restrict__ fix, uint32_t nfix,
+                                                   float *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = encode_f(x[i], fix, nfix);
 }
 
 // downsample (post.rs:218-247), one lane per output pixel.
@@ -1011,6 +1075,12 @@ struct lw_hip_backend {
   DevBuf<BatchDev> batch_dev;
   DevBuf<float> tone_in;
   DevBuf<uint32_t> tone_out;
+  // The display encoding's corrections to the rounded f64 power (see encode_f), made on the
+  // first call that needs them.
+  DevBuf<uint2> pow_fix;
+  uint32_t pow_fix_n = 0;
+  bool pow_fix_ready = false;
+  double pow_fix_ms = 0.0;
   BatchPlan plan;  // the last batch's layout (its sums stay in acc when it keeps them)
   // Differentiable rendering: the parameters and derivative inputs of the last call.
   DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
@@ -1766,6 +1836,45 @@ ToneDev make_tone(float exposure, uint32_t op_a, uint32_t op_b, float weight) {
   return t;
 }
 
+// The table encode_f reads: every f32 in [0, 1] whose rounded f64 power lies in the band
+// (k_pow_band, one pass over the 2^30 bit patterns), then libm's powf on those inputs here,
+// keeping the ones where it differs from the device's rounding.
+void ensure_pow_fix(lw_hip_backend *b) {
+  if (b->pow_fix_ready) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t n = 0x3f800001u;  // the bit patterns of [0, 1]
+  const uint32_t cap = 1u << 24;   // about 6.0M inputs are in the band
+  DevBuf<uint2> cand;
+  DevBuf<uint32_t> count;
+  cand.alloc(cap);
+  count.alloc(1);
+  HIP_TRY(hipMemsetAsync(count.ptr, 0, sizeof(uint32_t), b->stream));
+  k_pow_band<<<dim3(8192), dim3(kBlock), 0, b->stream>>>(n, cand.ptr, count.ptr, cap);
+  HIP_TRY(hipGetLastError());
+  uint32_t found = 0;
+  HIP_TRY(hipMemcpyAsync(&found, count.ptr, sizeof(found), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (found > cap) throw InvalidScene{"display-encoding band holds more inputs than expected"};
+  std::vector<uint2> host(found);
+  if (found)
+    HIP_TRY(hipMemcpy(host.data(), cand.ptr, (size_t)found * sizeof(uint2), hipMemcpyDeviceToHost));
+  std::vector<uint2> fix;
+  for (const uint2 &e : host) {
+    float x;
+    std::memcpy(&x, &e.x, 4);
+    const float g = lw::display_pow(x);
+    uint32_t gb;
+    std::memcpy(&gb, &g, 4);
+    if (gb != e.y) fix.push_back(make_uint2(e.x, gb));
+  }
+  std::sort(fix.begin(), fix.end(), [](const uint2 &a, const uint2 &c) { return a.x < c.x; });
+  b->pow_fix.upload(fix.data(), fix.size(), b->stream);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->pow_fix_n = (uint32_t)fix.size();
+  b->pow_fix_ready = true;
+  b->pow_fix_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // Tone-maps n pixels already on the device (in) into b->tone_out and copies them to out.
 void tonemap_device(lw_hip_backend *b, const float *in, size_t n, const ToneDev &t, bool mix,
                     uint32_t *out) {
@@ -1775,7 +1884,8 @@ void tonemap_device(lw_hip_backend *b, const float *in, size_t n, const ToneDev 
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
   if (n) {
     if (mix)
-      k_tonemap_mix<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->tone_out.ptr);
+      k_tonemap_mix<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->pow_fix.ptr,
+                                                          b->pow_fix_n, b->tone_out.ptr);
     else
       k_tonemap<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->tone_out.ptr);
     HIP_TRY(hipGetLastError());
@@ -2244,8 +2354,29 @@ int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float expo
     const bool ends = weight >= 1.0f || weight <= 0.0f;
     const uint32_t op = weight >= 1.0f ? op_a : op_b;
     const ToneDev t = ends ? make_tone(exposure, op, op, 1.0f) : make_tone(exposure, op_a, op_b, weight);
+    if (!ends) ensure_pow_fix(b);
     b->tone_in.upload(hdr, n * 3, b->stream);
     tonemap_device(b, b->tone_in.ptr, n, t, !ends, out);
+  });
+}
+
+int lw_hip_encode(lw_hip_backend *b, const float *x, size_t n, float *out, double *setup_ms) {
+  if (!b || (n && (!x || !out))) return LW_EINVAL;
+  return guarded(b, [&] {
+    HIP_TRY(hipSetDevice(b->device));
+    if (n > 0xFFFFFFFFull) throw InvalidScene{"too many values"};
+    ensure_pow_fix(b);
+    if (setup_ms) *setup_ms = b->pow_fix_ms;
+    DevBuf<float> in, res;
+    in.upload(x, n, b->stream);
+    res.alloc(std::max<size_t>(n, 1));
+    if (n) {
+      k_encode<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+          in.ptr, (uint32_t)n, b->pow_fix.ptr, b->pow_fix_n, res.ptr);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(out, res.ptr, n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
   });
 }
 

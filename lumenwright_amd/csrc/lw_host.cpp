@@ -871,12 +871,14 @@ constexpr Hable kFilmicWorlds = {0.15f, 0.5f, 0.1f, 0.2f, 0.02f, 0.3f, 11.2f};  
 
 uint32_t display_byte(float v) {  // post.rs:141-144,165 with Rust's saturating `as u32`
   float x = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-  const float d = std::pow(x, 1.0f / 2.2f) * 255.0f + 0.5f;
+  const float d = lw::display_pow(x) * 255.0f + 0.5f;
   if (!(d > 0.0f)) return 0;
   return (uint32_t)d;
 }
 
 }  // namespace
+
+float lw::display_pow(float x) { return std::pow(x, 1.0f / 2.2f); }
 
 void lw_hable_curve(uint32_t op, float out[8]) {
   Hable h = kFilmicWorlds;

@@ -93,6 +93,8 @@ def lib():
         L.lwo_pfm.restype = sz
         L.lwo_check_encode_thresholds.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(u32)]
         L.lwo_check_encode_thresholds.restype = ctypes.c_uint64
+        L.lwo_powf_band.argtypes = [ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
+        L.lwo_powf_band.restype = None
         L.lwo_sky_tangent.argtypes = [vp, vp, i32, vp, sz, vp]
         L.lwo_sky_tangent.restype = i32
         L.lwo_preetham_radiance.argtypes = [vp, vp, sz, vp]
@@ -417,3 +419,17 @@ def check_encode_thresholds(thr, threads: int = 0):
     n = lib().lwo_check_encode_thresholds(_addr(t), threads or os.cpu_count() or 1,
                                           ctypes.byref(first))
     return n, first.value
+
+
+def powf_band(threads: int = 0, inputs: bool = False):
+    """libm's powf(x, 1/2.2f) against the rounded f64 power over every f32 in [0, 1]:
+    (deviations, deviations outside the 2^-32 band, inputs in the band, deviations that are not
+    neighbours, largest relative midpoint distance of a deviation[, the band inputs])."""
+    st = np.zeros(4, dtype=np.uint64)
+    mr = ctypes.c_double()
+    band = np.zeros(1 << 24, dtype="<u4") if inputs else None
+    lib().lwo_powf_band(threads or os.cpu_count() or 1, _addr(st),
+                        ctypes.byref(mr), _addr(band) if inputs else None,
+                        band.size if inputs else 0)
+    out = (int(st[0]), int(st[1]), int(st[2]), int(st[3]), mr.value)
+    return out + (band[: min(int(st[2]), band.size)],) if inputs else out

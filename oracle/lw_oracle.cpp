@@ -9,6 +9,7 @@
 #include "lw_oracle_post.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 
 using namespace lwo;
@@ -374,6 +375,75 @@ uint64_t lwo_check_encode_thresholds(const float *thr, unsigned threads, uint32_
     if (bad[t] && fst[t] < *first) *first = fst[t];
   }
   return total;
+}
+
+// The display encoding's power over every f32 x in [0, 1]: libm's powf(x, 1/2.2f) against the
+// f64 power p = pow(x, 1/2.2f) rounded once to f32. A deviation is an x where the two differ;
+// x is in the band when p lies within 2^-32 * p of the f32 rounding midpoint on p's side of
+// round(p). stats: [0] deviations, [1] deviations outside the band, [2] inputs in the band,
+// [3] deviations that are not a neighbour of round(p); *max_rel: the largest |p - mid| / p
+// among the deviations. With `band` non-null (room for `cap`), the band inputs are written
+// there in ascending order.
+static bool band_of(float x, double &rel) {
+  const double p = std::pow((double)x, (double)(1.0f / 2.2f));
+  const float c = (float)p;
+  rel = 1.0;
+  if (!(c > 0.0f && c <= 1.0f)) return false;
+  uint32_t cb;
+  std::memcpy(&cb, &c, 4);
+  const uint32_t nbb = (double)c <= p ? cb + 1u : cb - 1u;
+  float nb;
+  std::memcpy(&nb, &nbb, 4);
+  const double mid = ((double)c + (double)nb) * 0.5;
+  rel = std::fabs(p - mid) / p;
+  return std::fabs(p - mid) < 0x1p-32 * p;
+}
+
+void lwo_powf_band(unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band,
+                   uint64_t cap) {
+  const uint32_t one = 0x3f800000u;
+  if (threads == 0) threads = 1;
+  std::vector<std::array<uint64_t, 4>> st(threads, std::array<uint64_t, 4>{0, 0, 0, 0});
+  std::vector<double> mr(threads, 0.0);
+  std::vector<std::vector<uint32_t>> found(threads);
+  auto work = [&](unsigned t) {
+    const uint32_t lo = (uint32_t)((uint64_t)(one + 1) * t / threads);
+    const uint32_t hi = (uint32_t)((uint64_t)(one + 1) * (t + 1) / threads);
+    for (uint32_t u = lo; u < hi; u++) {
+      float x;
+      std::memcpy(&x, &u, 4);
+      double rel;
+      const bool in = band_of(x, rel);
+      if (in) {
+        if (band) found[t].push_back(u);
+        st[t][2]++;
+      }
+      const float g = lwo_post::encode(x);
+      const float c = (float)std::pow((double)x, (double)(1.0f / 2.2f));
+      uint32_t gb, cb;
+      std::memcpy(&gb, &g, 4);
+      std::memcpy(&cb, &c, 4);
+      if (gb != cb) {
+        st[t][0]++;
+        if (!in) st[t][1]++;
+        if (gb != cb + 1u && gb != cb - 1u) st[t][3]++;
+        mr[t] = std::max(mr[t], rel);
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; t++) pool.emplace_back(work, t);
+  work(0);
+  for (auto &t : pool) t.join();
+  for (int k = 0; k < 4; k++) stats[k] = 0;
+  *max_rel = 0.0;
+  uint64_t at = 0;
+  for (unsigned t = 0; t < threads; t++) {
+    for (int k = 0; k < 4; k++) stats[k] += st[t][k];
+    *max_rel = std::max(*max_rel, mr[t]);
+    for (uint32_t u : found[t])
+      if (at < cap) band[at++] = u;
+  }
 }
 
 }  // extern "C"

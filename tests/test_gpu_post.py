@@ -57,6 +57,24 @@ def test_tonemap_mix_equals_the_oracle(hip_backend, oracle, hdr, pair, weight):
     assert bad.size == 0, (bad.size, hdr[bad[:4]], got[bad[:4]], want[bad[:4]])
 
 
+def test_display_encoding_equals_libm_powf(hip_backend, oracle):
+    """lw_hip_encode against libm's powf on every input of the band (where the rounded f64
+    power and powf can differ), 8M random inputs of [0, 1] and the edges."""
+    *_, band = oracle.powf_band(inputs=True)
+    rng = np.random.default_rng(11)
+    rand = rng.integers(0, 0x3F800001, size=1 << 23, dtype=np.uint64).astype("<u4")
+    edges = np.array([0.0, -0.0, 1.0, 1.5, -2.0, np.inf, -np.inf, np.nan, 1e-45, 1e-38],
+                     dtype="<f4")
+    x = np.concatenate([band.view("<f4"), rand.view("<f4"), edges])
+    got = hip_backend.encode(x)
+    print(f"band inputs {band.size}, table set-up {hip_backend.encode_setup_ms:.0f} ms")
+    want = oracle.encode(x)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    bad = np.flatnonzero(got[~nan].view("<u4") != want[~nan].view("<u4"))
+    assert bad.size == 0, (bad.size, x[~nan][bad[:4]], got[~nan][bad[:4]], want[~nan][bad[:4]])
+
+
 @pytest.mark.parametrize("w,h,k", [(2, 2, 2), (1280, 720, 2), (1283, 721, 3), (97, 13, 4),
                                    (64, 64, 1), (5, 3, 8), (1920, 1080, 5)])
 def test_downsample_equals_the_oracle(hip_backend, oracle, w, h, k):
@@ -75,10 +93,9 @@ def test_render_then_tonemap_on_the_device_equals_the_oracle_tonemap(hip_backend
         params = fr.view().params
         hdr = hip_backend.render_hq_scene(scene, params, fr.spp)
         px = hip_backend.render_hq_rgba8(scene, params, fr.spp, 0.8, op)
+        assert hip_backend.last_timing()["samples"] == 160 * 90 * 4
         assert np.array_equal(px, oracle.tonemap(hdr, 0.8, op))
         assert np.array_equal(px, hip_backend.tonemap(hdr, 0.8, op))
-        t = hip_backend.last_timing()
-        assert t["samples"] == 160 * 90 * 4
     finally:
         scene.free()
 

@@ -103,3 +103,15 @@ def test_pfm_writer_matches_output_rs(oracle):
         path = os.path.join(d, "x.pfm")
         frames.write_pfm(path, hdr, w, h)
         assert open(path, "rb").read() == want
+
+
+def test_libm_powf_deviates_only_near_rounding_midpoints(oracle):
+    """The device's display encoding takes the power in f64 and rounds it once, and looks up
+    libm's powf only for inputs in the band (f64 power within 2^-32 of an f32 rounding
+    midpoint). That is exact if every f32 in [0, 1] where powf differs from the rounded power
+    is in the band, with room for the f64 power's own error (near 2^-52)."""
+    dev, outside, band, not_nb, max_rel = oracle.powf_band()
+    assert dev > 0  # libm's powf is not correctly rounded here: the band is needed
+    assert outside == 0 and not_nb == 0
+    assert max_rel < 2.0 ** -33
+    assert band < 1 << 24  # the device sweep's capacity
