@@ -170,6 +170,13 @@ typedef struct lw_world {
   uint32_t *quad_emitter;
   /* what the builder made (voxel.rs HqStats) */
   uint64_t bricks, uniform_bricks, buried_bricks, solid_cells, water_cells, cutout_cells;
+  /* the interned volume the layout was made from, as lw_hip_scene_build takes it (see
+   * lw_voxel_scene): voxels_len = 4096 per brick, brick_occludes one per brick,
+   * model_flags one per model */
+  uint16_t *voxels;
+  size_t voxels_len;
+  uint8_t *brick_occludes;
+  uint8_t *model_flags;
 } lw_world;
 
 /* The terrain height h(x, z) = floor(64 + 24 V(x, z)) of a spec, V the 4-octave value noise. */

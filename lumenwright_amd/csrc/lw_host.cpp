@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -631,7 +632,12 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
       by = (int32_t)(i / (dx * dz));
       bz = (int32_t)((i / dx) % dz);
     };
-    // Pass 1: bricks whose every block occludes (every cell solid).
+    lw_world *w = (lw_world *)std::calloc(1, sizeof(lw_world));
+    if (!w) throw std::bad_alloc();
+    std::unique_ptr<lw_world, void (*)(lw_world *)> guard(w, lw_world_free);
+
+    // The interned volume: per brick whether every block occludes (voxel.rs:119-130), and
+    // every voxel's model id; an emitter block gets a model of its own (its emittances).
     std::vector<uint8_t> all_occluding(nb, 0);
     for (uint64_t b = 0; b < nb; b++) {
       int32_t bx, by, bz;
@@ -642,22 +648,13 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
           if (by * 16 + 15 > H(bx * 16 + x, bz * 16 + z)) all = false;
       all_occluding[b] = all ? 1 : 0;
     }
-    auto brick_index = [&](int32_t bx, int32_t by, int32_t bz) -> int64_t {
-      if (bx < 0 || by < 0 || bz < 0 || bx >= (int32_t)dx || by >= (int32_t)dy || bz >= (int32_t)dz) return -1;
-      return ((int64_t)by * dz + bz) * dx + bx;
-    };
-
-    // Pass 2: voxel ids, emitters, top-level entries (voxel.rs:302-377).
-    std::vector<uint32_t> top(nb, LW_EMPTY);
-    std::vector<uint32_t> bricks;
-    std::vector<std::pair<std::array<int32_t, 3>, uint32_t>> sites;
-    lw_world *w = (lw_world *)std::calloc(1, sizeof(lw_world));
-    if (!w) throw std::bad_alloc();
-    std::vector<uint16_t> ids(4096);
+    std::vector<uint16_t> vox(nb * 4096);
+    std::vector<uint8_t> flags(models.size(), 0);
+    for (uint32_t m = 1; m <= n_cube; m++) flags[m] = LW_MODEL_OCCLUDES;
+    uint32_t n_emitters = 0;
     for (uint64_t b = 0; b < nb; b++) {
       int32_t bx, by, bz;
       brick_xyz(b, bx, by, bz);
-      bool any = false;
       for (uint32_t local = 0; local < 4096; local++) {
         const int32_t lx = local & 15, lz = (local >> 4) & 15, ly = local >> 8;
         const int32_t x = bx * 16 + lx, y = by * 16 + ly, z = bz * 16 + lz;
@@ -672,15 +669,15 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
             uint32_t sx, sy;
             sprite_xy(t, sx, sy);
             float em[6];
-            const uint32_t e_index = (uint32_t)sites.size();
+            const uint32_t e_index = n_emitters++;
             for (int f = 0; f < 6; f++)
               em[f] = column_emitter ? 1.0f
                                      : (float)(1.0 + 3.0 * u01(hash5(tseed, H_EMITTANCE, e_index, (uint64_t)f, 0)));
             id = (uint32_t)models.size();
             models.push_back(lw_model{(uint32_t)quads.size(), 6, 0, 0});
+            flags.push_back(LW_MODEL_OCCLUDES | LW_MODEL_EMITS);
             cube_faces(quads, LW_ALPHA_OPAQUE | LW_FLAG_FRONT_ONLY, sx, sy, em);
             tag(t, e_index);
-            sites.push_back({{x, y, z}, id});
           } else {
             id = 1 + (uint32_t)(hash5(seed, H_TYPE, (uint64_t)x, (uint64_t)y, (uint64_t)z) % n_cube);
           }
@@ -692,8 +689,30 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
           if (id) w->cutout_cells++;
         }
         if (id > 0xFFFF) throw std::runtime_error("more than 65535 distinct models");
-        ids[local] = (uint16_t)id;
-        any = any || id != 0;
+        vox[b * 4096 + local] = (uint16_t)id;
+      }
+    }
+
+    // The layout (voxel.rs:305-342): EMPTY, buried, UNIFORM or stored bricks, and the emitter
+    // sites in brick then voxel order.
+    auto brick_index = [&](int32_t bx, int32_t by, int32_t bz) -> int64_t {
+      if (bx < 0 || by < 0 || bz < 0 || bx >= (int32_t)dx || by >= (int32_t)dy || bz >= (int32_t)dz) return -1;
+      return ((int64_t)by * dz + bz) * dx + bx;
+    };
+    std::vector<uint32_t> top(nb, LW_EMPTY);
+    std::vector<uint32_t> bricks;
+    std::vector<std::pair<std::array<int32_t, 3>, uint32_t>> sites;
+    for (uint64_t b = 0; b < nb; b++) {
+      int32_t bx, by, bz;
+      brick_xyz(b, bx, by, bz);
+      const uint16_t *ids = vox.data() + b * 4096;
+      bool any = false;
+      for (uint32_t local = 0; local < 4096; local++) {
+        any = any || ids[local] != 0;
+        if (flags[ids[local]] & LW_MODEL_EMITS)
+          sites.push_back({{bx * 16 + (int32_t)(local & 15), by * 16 + (int32_t)(local >> 8),
+                            bz * 16 + (int32_t)((local >> 4) & 15)},
+                           ids[local]});
       }
       if (!any) continue;  // an empty section stays EMPTY
       bool buried = all_occluding[b] != 0;
@@ -705,20 +724,16 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
       if (buried) {
         w->buried_bricks++;
         uint32_t first = 1;
-        for (uint16_t v : ids)
-          if (v != 0) {
-            first = v;
+        for (uint32_t k = 0; k < 4096; k++)
+          if (ids[k] != 0) {
+            first = ids[k];
             break;
           }
         top[b] = LW_UNIFORM | first;
         continue;
       }
       bool uniform = true;
-      for (uint16_t v : ids)
-        if (v != ids[0]) {
-          uniform = false;
-          break;
-        }
+      for (uint32_t k = 1; k < 4096 && uniform; k++) uniform = ids[k] == ids[0];
       if (uniform) {
         w->uniform_bricks++;
         top[b] = LW_UNIFORM | ids[0];
@@ -759,6 +774,10 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
     w->emitter_grid_len = egrid.size();
     w->quad_type = to_heap(quad_type);
     w->quad_emitter = to_heap(quad_emitter);
+    w->voxels = to_heap(vox);
+    w->voxels_len = vox.size();
+    w->brick_occludes = to_heap(all_occluding);
+    w->model_flags = to_heap(flags);
     lw_scene_info &info = w->info;
     std::memset(&info, 0, sizeof(info));
     info.dims[0] = dx;
@@ -773,7 +792,7 @@ int lw_world_build(const lw_world_spec *spec, lw_world **out) {
     info.emitter_count = (uint32_t)sites.size();
     info.see_through[0] = (uint32_t)quads.size();
     info.see_through[1] = 0;
-    *out = w;
+    *out = guard.release();
     return 0;
   } catch (const std::exception &e) {
     g_host_error = e.what();
@@ -791,6 +810,9 @@ void lw_world_free(lw_world *w) {
   std::free(w->emitter_grid);
   std::free(w->quad_type);
   std::free(w->quad_emitter);
+  std::free(w->voxels);
+  std::free(w->brick_occludes);
+  std::free(w->model_flags);
   std::free(w);
 }
 
