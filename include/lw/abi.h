@@ -225,6 +225,85 @@ int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene,
 
 void lw_hip_scene_free(lw_hip_backend *b, lw_hip_scene *scene);
 
+/* ---- the scene builder on the device (crates/lumenwright/src/scene/voxel.rs) ---- */
+
+#define LW_MODEL_OCCLUDES 1u /* the model's block occludes (BlockInfo::occludes, voxel.rs:119-130) */
+#define LW_MODEL_EMITS 2u    /* the model has quads and its block emits (voxel.rs:521) */
+
+/* The HqScene facts hq_params reads (voxel.rs:19-49). */
+typedef struct lw_scene_info {
+  uint32_t dims[3];
+  uint32_t offsets[5];
+  uint32_t atlas_size[2];
+  uint32_t quads_len;
+  uint32_t models_len;
+  uint32_t emitter_count;
+  uint32_t see_through[2];
+  uint32_t has_sun_sprite;
+  uint32_t sun_sprite[4];
+} lw_scene_info;
+
+/* What build_hq (voxel.rs:111-491) holds once it has interned the view's block-model
+ * instances (voxel.rs:170-305): one model id per voxel, the models with their quads and
+ * flags, the entity overlay's quads, and the atlas and sky the scene carries. */
+typedef struct lw_voxel_scene {
+  uint32_t dims[3];               /* bricks along x, y (sections) and z; fewer than 2^24 bricks */
+  uint32_t see_through_count;     /* the last this-many entity quads are see-through text
+                                     (voxel.rs:361-376) */
+  const uint16_t *voxels;         /* 4096 model ids per brick: brick (bx, by, bz) at
+                                     ((by * dims[2] + bz) * dims[0] + bx) * 4096, its voxel
+                                     (lx, ly, lz) at (ly * 16 + lz) * 16 + lx */
+  const uint8_t *brick_occludes;  /* per brick: every block of its section occludes
+                                     (voxel.rs:119-130); null: every voxel's model has
+                                     LW_MODEL_OCCLUDES */
+  const lw_model *models;         /* model 0 is air */
+  size_t models_len;              /* 1 .. 65536 */
+  const uint8_t *model_flags;     /* LW_MODEL_* per model */
+  const lw_quad *quads;           /* the models' quads */
+  size_t quads_len;
+  const lw_quad *entity_quads;    /* the overlay's quads in grid space, flags set
+                                     (voxel.rs:355-372) */
+  size_t entity_quads_len;
+  const uint32_t *atlas;
+  size_t atlas_len;
+  uint32_t atlas_size[2];
+  uint32_t has_sun_sprite;
+  uint32_t sun_sprite[4];
+  const float *sky;               /* the sky buffer renders read (lw_scene_view.sky) */
+  size_t sky_len;
+} lw_voxel_scene;
+
+/* HqStats (voxel.rs:51-70) with the buffers' lengths and the build's time. */
+typedef struct lw_build_stats {
+  uint64_t bricks, uniform_bricks, buried_bricks, instances, overlay_bricks, entity_quads,
+      emitters;
+  uint64_t grid_len, quads_len, emitters_len, emitter_grid_len;
+  double kernel_ms; /* HIP-event time from the first layout kernel to the last */
+  double wall_ms;   /* the whole call, uploads included */
+} lw_build_stats;
+
+/* build_hq's layout on the device (voxel.rs:305-491 and emitter_list, voxel.rs:542-607):
+ * the grid buffer (top-level entries with UNIFORM and buried bricks collapsed, the stored
+ * bricks, the entity overlay), the emitter list and the emitter grid, made in HBM from the
+ * voxel volume and kept there as a resident scene. info receives the HqScene facts params
+ * are made from (lw_hq_params); renders then take params carrying that layout. The overlay's
+ * cell lists are the reference's; their order inside an overlay brick's list buffer, which
+ * the reference takes from a HashMap (voxel.rs:418-424), is cell order here. stats may be
+ * null. No counterpart in the reference, which builds on the CPU and uploads. */
+int lw_hip_scene_build(lw_hip_backend *b, const lw_voxel_scene *vs, lw_scene_info *info,
+                       lw_build_stats *stats, lw_hip_scene **out);
+
+/* Copies one buffer of a resident scene to the host: LW_SCENE_GRID and
+ * LW_SCENE_EMITTER_GRID (uint32), LW_SCENE_EMITTERS (lw_emitter), LW_SCENE_QUADS (lw_quad).
+ * *len receives the element count; with dst null only *len is written, otherwise cap (in
+ * elements) must hold the buffer. No counterpart in the reference. */
+#define LW_SCENE_GRID 0u
+#define LW_SCENE_EMITTERS 1u
+#define LW_SCENE_EMITTER_GRID 2u
+#define LW_SCENE_QUADS 3u
+int lw_hip_scene_read(lw_hip_backend *b, const lw_hip_scene *scene, uint32_t which, void *dst,
+                      size_t cap, size_t *len);
+
 /* One view of a batch: its Params and samples per pixel. The views of a batch may differ
  * only in the camera (cam_pos, cam_fwd, cam_right, cam_up), the frame size (frame[0..1])
  * and the seed (misc_u[0]). */

@@ -77,19 +77,6 @@ typedef struct lw_hq_options {
   uint32_t prevent_normal_emitter;
 } lw_hq_options;
 
-/* The HqScene facts hq_params reads (voxel.rs:19-49). */
-typedef struct lw_scene_info {
-  uint32_t dims[3];
-  uint32_t offsets[5];
-  uint32_t atlas_size[2];
-  uint32_t quads_len;
-  uint32_t models_len;
-  uint32_t emitter_count;
-  uint32_t see_through[2];
-  uint32_t has_sun_sprite;
-  uint32_t sun_sprite[4];
-} lw_scene_info;
-
 /* CameraRays (render/mod.rs:172-183). */
 typedef struct lw_camera_rays {
   float forward[3];

@@ -9,8 +9,9 @@ mod.rs:92-103), reached through the C ABI of include/lw/abi.h. The package holds
   build    the in-tree hipcc build of lumenwright_amd/_lib/liblumenwright_hip.so
 """
 
-from .abi import (EMITTER, HIT, MODEL, PARAMS, QUAD, BoundView, HqFrame, HqScene, new_params)
+from .abi import (EMITTER, HIT, MODEL, PARAMS, QUAD, BoundView, HqFrame, HqScene, VoxelVolume,
+                  new_params)
 from .backend import BackendError, HipBackend, HipScene, probe
 
 __all__ = ["EMITTER", "HIT", "MODEL", "PARAMS", "QUAD", "BoundView", "HqFrame", "HqScene",
-           "new_params", "BackendError", "HipBackend", "HipScene", "probe"]
+           "VoxelVolume", "new_params", "BackendError", "HipBackend", "HipScene", "probe"]

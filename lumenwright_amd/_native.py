@@ -34,6 +34,8 @@ SIGNATURES = {
     "lw_hip_scene_upload": (i32, [vp, vp, ctypes.POINTER(vp)]),
     "lw_hip_render_hq_scene": (i32, [vp, vp, vp, u32, PROGRESS_FN, vp, vp]),
     "lw_hip_scene_free": (None, [vp, vp]),
+    "lw_hip_scene_build": (i32, [vp, vp, vp, vp, ctypes.POINTER(vp)]),
+    "lw_hip_scene_read": (i32, [vp, vp, u32, vp, sz, ctypes.POINTER(sz)]),
     "lw_hip_last_timing": (i32, [vp, vp]),
     "lw_hip_render_hq_batch": (i32, [vp, vp, vp, sz, PROGRESS_FN, vp, vp]),
     "lw_hip_copy_probe": (i32, [vp, sz, ctypes.POINTER(f64)]),

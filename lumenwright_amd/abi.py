@@ -157,6 +157,80 @@ def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
     return a.ctypes.data
 
 
+MODEL_OCCLUDES, MODEL_EMITS = 1, 2  # lw_voxel_scene.model_flags
+SCENE_GRID, SCENE_EMITTERS, SCENE_EMITTER_GRID, SCENE_QUADS = 0, 1, 2, 3  # lw_hip_scene_read
+
+
+class VoxelSceneC(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_uint32 * 3), ("see_through_count", ctypes.c_uint32),
+                ("voxels", ctypes.c_void_p), ("brick_occludes", ctypes.c_void_p),
+                ("models", ctypes.c_void_p), ("models_len", ctypes.c_size_t),
+                ("model_flags", ctypes.c_void_p), ("quads", ctypes.c_void_p),
+                ("quads_len", ctypes.c_size_t), ("entity_quads", ctypes.c_void_p),
+                ("entity_quads_len", ctypes.c_size_t), ("atlas", ctypes.c_void_p),
+                ("atlas_len", ctypes.c_size_t), ("atlas_size", ctypes.c_uint32 * 2),
+                ("has_sun_sprite", ctypes.c_uint32), ("sun_sprite", ctypes.c_uint32 * 4),
+                ("sky", ctypes.c_void_p), ("sky_len", ctypes.c_size_t)]
+
+
+class BuildStatsC(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "bricks", "uniform_bricks", "buried_bricks", "instances", "overlay_bricks", "entity_quads",
+        "emitters", "grid_len", "quads_len", "emitters_len", "emitter_grid_len")] + [
+        ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+
+@dataclass
+class VoxelVolume:
+    """What build_hq (crates/lumenwright/src/scene/voxel.rs:111-491) holds once it has
+    interned the view's block-model instances: one model id per voxel (4096 per brick, bricks
+    in y, z, x order, voxels (ly * 16 + lz) * 16 + lx), the models with their quads and
+    LW_MODEL_* flags, the entity overlay's quads and the atlas (lw_voxel_scene)."""
+
+    dims: tuple
+    voxels: np.ndarray
+    models: np.ndarray
+    model_flags: np.ndarray
+    quads: np.ndarray
+    atlas: np.ndarray
+    atlas_size: tuple
+    brick_occludes: Optional[np.ndarray] = None
+    entity_quads: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=QUAD))
+    see_through_count: int = 0
+    sun_sprite: Optional[tuple] = None
+
+    def c(self, sky=None):
+        """The lw_voxel_scene and the arrays it points into (keep both alive)."""
+        keep = {
+            "voxels": np.ascontiguousarray(self.voxels, dtype="<u2").reshape(-1),
+            "models": np.ascontiguousarray(self.models, dtype=MODEL),
+            "flags": np.ascontiguousarray(self.model_flags, dtype=np.uint8),
+            "quads": np.ascontiguousarray(self.quads, dtype=QUAD),
+            "entity": np.ascontiguousarray(self.entity_quads, dtype=QUAD),
+            "atlas": np.ascontiguousarray(self.atlas, dtype="<u4"),
+            "sky": np.ascontiguousarray(sky if sky is not None else np.zeros(0), dtype="<f4"),
+        }
+        occ = None
+        if self.brick_occludes is not None:
+            occ = keep["occ"] = np.ascontiguousarray(self.brick_occludes, dtype=np.uint8)
+        c = VoxelSceneC()
+        c.dims[:] = [int(d) for d in self.dims]
+        c.see_through_count = self.see_through_count
+        c.voxels = _ptr(keep["voxels"])
+        c.brick_occludes = _ptr(occ) if occ is not None else None
+        c.models, c.models_len = _ptr(keep["models"]), keep["models"].size
+        c.model_flags = _ptr(keep["flags"])
+        c.quads, c.quads_len = _ptr(keep["quads"]), keep["quads"].size
+        c.entity_quads, c.entity_quads_len = _ptr(keep["entity"]), keep["entity"].size
+        c.atlas, c.atlas_len = _ptr(keep["atlas"]), keep["atlas"].size
+        c.atlas_size[:] = [int(a) for a in self.atlas_size]
+        if self.sun_sprite is not None:
+            c.has_sun_sprite = 1
+            c.sun_sprite[:] = [int(a) for a in self.sun_sprite]
+        c.sky, c.sky_len = _ptr(keep["sky"]), keep["sky"].size
+        return c, keep
+
+
 @dataclass
 class HqScene:
     """The HQ scene buffers (crates/lumenwright/src/scene/voxel.rs:17-49)."""

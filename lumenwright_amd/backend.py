@@ -14,8 +14,10 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _native
-from .abi import (FORMS, HIT, PARAMS, VIEW, BoundView, FitOptionsC, FitStatsC, HqFrame,
-                  TimingC, TuneResultC)
+from .abi import (EMITTER, FORMS, HIT, MODEL, PARAMS, QUAD, SCENE_EMITTER_GRID, SCENE_EMITTERS,
+                  SCENE_GRID, SCENE_QUADS, VIEW, BoundView, BuildStatsC, FitOptionsC, FitStatsC,
+                  HqFrame, HqScene, TimingC, TuneResultC, VoxelVolume)
+from .frames import SceneInfoC
 
 LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
 
@@ -44,10 +46,10 @@ def _addr(a: np.ndarray):
 class HipScene:
     """A scene validated and resident in device memory (lw_hip_scene)."""
 
-    def __init__(self, backend: "HipBackend", handle: int, view: BoundView):
+    def __init__(self, backend: "HipBackend", handle: int, view: Optional[BoundView] = None):
         self.backend = backend
         self.handle = handle
-        self.params = view.params.copy()
+        self.params = view.params.copy() if view is not None else None
 
     def free(self):
         if self.handle:
@@ -108,6 +110,37 @@ class HipBackend:
         h = ctypes.c_void_p()
         self._check(_native.lib().lw_hip_scene_upload(self.handle, view.ref, ctypes.byref(h)))
         return HipScene(self, h.value, view)
+
+    def build_scene(self, vol: VoxelVolume, sky: Optional[np.ndarray] = None):
+        """lw_hip_scene_build: build_hq's layout (scene/voxel.rs:305-491) made on the device
+        from an interned voxel volume. Returns (HipScene, HqScene-shaped info, stats): the
+        info is an HqScene whose buffers are empty and whose layout facts (dims, offsets,
+        see-through range, emitter count) are the built scene's, for hq_params."""
+        c, keep = vol.c(sky)
+        info, stats, h = SceneInfoC(), BuildStatsC(), ctypes.c_void_p()
+        self._check(_native.lib().lw_hip_scene_build(self.handle, ctypes.byref(c), ctypes.byref(info),
+                                                     ctypes.byref(stats), ctypes.byref(h)))
+        del keep
+        layout = HqScene(grid=np.zeros(0, "<u4"), models=np.zeros(int(info.models_len), MODEL),
+                         quads=np.zeros(int(info.quads_len), QUAD), atlas=np.zeros(0, "<u4"),
+                         atlas_size=tuple(info.atlas_size), dims=tuple(info.dims),
+                         offsets=tuple(info.offsets), see_through=tuple(info.see_through),
+                         emitter_count=int(info.emitter_count),
+                         sun_sprite=tuple(info.sun_sprite) if info.has_sun_sprite else None)
+        st = {k: getattr(stats, k) for k, _ in BuildStatsC._fields_}
+        return HipScene(self, h.value), layout, st
+
+    def scene_read(self, scene: HipScene, which: int) -> np.ndarray:
+        """lw_hip_scene_read: one of a resident scene's buffers (SCENE_*) on the host."""
+        L = _native.lib()
+        n = ctypes.c_size_t()
+        self._check(L.lw_hip_scene_read(self.handle, scene.handle, which, None, 0, ctypes.byref(n)))
+        dtype = {SCENE_GRID: "<u4", SCENE_EMITTER_GRID: "<u4", SCENE_EMITTERS: EMITTER,
+                 SCENE_QUADS: QUAD}[which]
+        out = np.zeros(n.value, dtype=dtype)
+        self._check(L.lw_hip_scene_read(self.handle, scene.handle, which, _addr(out), out.size,
+                                        ctypes.byref(n)))
+        return out
 
     def render_hq_scene(self, scene: HipScene, params: np.ndarray, spp: int,
                         progress: Optional[Callable[[int, int], None]] = None) -> np.ndarray:

@@ -19,8 +19,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_lib" / "liblumenwright_hip.so"
-SOURCES = [CSRC / "lw_hip.hip", CSRC / "lw_host.cpp"]
-HEADERS = [CSRC / "lw_math.hpp", CSRC / "lw_device.hpp", CSRC / "lw_host.hpp",
+SOURCES = [CSRC / "lw_hip.hip", CSRC / "lw_build.hip", CSRC / "lw_host.cpp"]
+HEADERS = [CSRC / "lw_math.hpp", CSRC / "lw_device.hpp", CSRC / "lw_host.hpp", CSRC / "lw_build.hpp",
            ROOT / "include" / "lw" / "abi.h", ROOT / "include" / "lw" / "host.h"]
 
 ARCH = os.environ.get("LW_OFFLOAD_ARCH", "gfx950")
@@ -42,16 +42,37 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
+    """Compiles each source to an object in parallel (lumenwright_amd/_lib/obj/), then links."""
     OUT.parent.mkdir(parents=True, exist_ok=True)
+    objdir = OUT.parent / "obj"
+    objdir.mkdir(exist_ok=True)
+    inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    jobs = []
+    for src in SOURCES:
+        obj = objdir / (src.name + ".o")
+        cmd = [hipcc(), *compile_flags, *inc, "-x", "hip", "-c", str(src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                text=True)))
+    failed = []
+    for src, obj, proc in jobs:
+        out, _ = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src.name)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-I", str(ROOT / "include"), "-I", str(CSRC), "-x", "hip",
-           *[str(s) for s in SOURCES], "-o", str(tmp)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for _, o, _ in jobs],
+           "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
+        raise RuntimeError(f"link failed ({r.returncode})")
     os.replace(tmp, OUT)
     return OUT
 

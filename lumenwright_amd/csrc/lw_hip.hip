@@ -28,6 +28,7 @@
 #include "lw_host.hpp"
 #include "lw_path.hpp"
 #include "lw_diff.hpp"
+#include "lw_build.hpp"
 
 using namespace lw;
 
@@ -970,6 +971,11 @@ struct DevBuf {
   void ensure(size_t count) {
     if (count > n) alloc(count);
   }
+  void adopt(T *p, size_t count) {  // takes ownership of a hipMalloc'ed buffer
+    free();
+    ptr = p;
+    n = count;
+  }
   void free() {
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
@@ -1013,7 +1019,13 @@ struct lw_hip_scene {
   DevBuf<float> lut;
   DevBuf<uint2> occupancy;
   std::vector<uint32_t> quad_flags;  // host copy of each quad's flags (p0.w)
-  LayoutKey key;
+  // The params layout the scene was validated with. A scene built on the device
+  // (lw_hip_scene_build) takes it from the first render's params, checked against `built`.
+  mutable LayoutKey key;
+  mutable bool key_set = false;
+  bool is_built = false;
+  lw_scene_info built{};
+  size_t atlas_len = 0, sky_len = 0;
   uint32_t max_emitter_rays = 1;  // emitter face rays one bounce can trace
   DevScene dev() const {
     return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
@@ -1266,6 +1278,7 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.occupancy.upload(occ.data(), occ.size(), b->stream);
   HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors above die with this scope
   sc.key = layout_key(*v.params);
+  sc.key_set = true;
   // The most face rays one bounce's emitter sampling can trace (integrator.rs:441-469).
   uint64_t max_faces = 0, max_count = 0;
   for (size_t e = 0; e < v.emitters_len; e++) max_faces = std::max<uint64_t>(max_faces, v.emitters[e].faces[0]);
@@ -1279,6 +1292,150 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.max_emitter_rays = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(rays, 1), 1u << 20);
 }
 
+
+// Params must repeat the layout a scene was validated with. A built scene is validated
+// against the first params it meets: the scene fields must be those the build reported, and
+// the params-only checks of validate() (atlas size, sky layout) are made here.
+void check_layout(const lw_hip_scene &sc, const lw_params &p) {
+  if (!sc.key_set) {
+    auto fail = [](const std::string &m) { throw InvalidScene{m}; };
+    const lw_scene_info &i = sc.built;
+    for (int a = 0; a < 3; a++)
+      if (p.grid_dims[a] != i.dims[a]) fail("params grid_dims differ from the built scene");
+    if (p.grid_dims[3] != i.dims[0] * i.dims[1] * i.dims[2]) fail("params grid_dims[3] is not the brick count");
+    for (int k = 0; k < 4; k++)
+      if (p.off[k] != i.offsets[k]) fail("params offsets differ from the built scene");
+    if (p.off2[0] != i.offsets[4] || p.off2[3] != i.quads_len) fail("params offsets differ from the built scene");
+    if (p.see_through[0] != i.see_through[0] || p.see_through[1] != i.see_through[1])
+      fail("params see-through range differs from the built scene");
+    if (p.misc_u[2] != i.models_len || p.misc_u[3] != i.emitter_count)
+      fail("params model or emitter count differs from the built scene");
+    const int64_t aw = (int32_t)p.off2[1], ah = (int32_t)p.off2[2];
+    if (aw < 1 || ah < 1 || (uint64_t)(aw * ah) > sc.atlas_len) fail("atlas size does not fit the atlas");
+    if (p.sky_map[0] >= 1.0f) {
+      for (int k = 1; k < 4; k++)
+        if (!(p.sky_map[k] >= 1.0f)) fail("sky map sizes must be at least 1");
+      const uint64_t w0 = as_u32(p.sky_map[0]), h0 = as_u32(p.sky_map[1]);
+      const uint64_t w1 = as_u32(p.sky_map[2]), h1 = as_u32(p.sky_map[3]);
+      const uint64_t base = (uint64_t)as_u32(p.sky_map[0] * p.sky_map[1]) * 3;
+      if (w0 * h0 * 3 > sc.sky_len || base + w1 * h1 * 3 > sc.sky_len) fail("sky maps past the sky buffer");
+    } else if (p.misc_f[2] >= 1.0f) {
+      const uint64_t m = (uint64_t)as_u32(p.misc_f[2]) + 1;
+      if (m * m * 3 > sc.sky_len) fail("sky cache past the sky buffer");
+    }
+    sc.key = layout_key(p);
+    sc.key_set = true;
+  }
+  if (!(layout_key(p) == sc.key))
+    throw InvalidScene{"params do not match the layout the scene was validated with"};
+}
+
+// lw_hip_scene_build: the inputs checked as validate() checks an uploaded view, the volume
+// and the models' buffers uploaded, the layout made on the device (lw_build.hip).
+void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, lw_scene_info &info,
+                 lw_build_stats *stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  auto fail = [](const std::string &m) { throw InvalidScene{m}; };
+  const uint64_t dx = v.dims[0], dy = v.dims[1], dz = v.dims[2], nb = dx * dy * dz;
+  if (dx == 0 || dy == 0 || dz == 0 || dx * 16 > 0x7FFFFFF || dy * 16 > 0x7FFFFFF || dz * 16 > 0x7FFFFFF)
+    fail("dims out of range");
+  if (nb >= (1ull << 24)) fail("a built scene has fewer than 2^24 bricks");
+  if (!v.voxels || !v.models || !v.model_flags || !v.atlas) fail("a scene buffer is null");
+  if (v.models_len == 0 || v.models_len > 65536) fail("models_len out of range (1 .. 65536)");
+  if (v.model_flags[0] != 0) fail("model 0 is air: its flags must be 0");
+  if ((v.quads_len && !v.quads) || (v.entity_quads_len && !v.entity_quads)) fail("a quad buffer is null");
+  if (v.see_through_count > v.entity_quads_len) fail("more see-through quads than entity quads");
+  const uint64_t nq = (uint64_t)v.quads_len + v.entity_quads_len;
+  if (nq >= 0xFFFFFFFFull) fail("too many quads");
+  for (size_t m = 0; m < v.models_len; m++)
+    if ((uint64_t)v.models[m].quad_offset + v.models[m].quad_count > v.quads_len)
+      fail("model " + std::to_string(m) + " names quads past the quad buffer");
+  if (v.atlas_len == 0 || v.atlas_size[0] == 0 || v.atlas_size[1] == 0 ||
+      (uint64_t)v.atlas_size[0] * v.atlas_size[1] > v.atlas_len)
+    fail("atlas size does not fit the atlas");
+  HIP_TRY(hipSetDevice(b->device));
+  DevBuf<uint16_t> vox;
+  DevBuf<uint8_t> occ, flags;
+  vox.upload(v.voxels, nb * 4096, b->stream);
+  if (v.brick_occludes) occ.upload(v.brick_occludes, nb, b->stream);
+  flags.upload(v.model_flags, v.models_len, b->stream);
+  sc.models.upload(v.models, v.models_len, b->stream);
+  std::vector<lw_quad> quads(v.quads, v.quads + v.quads_len);
+  quads.insert(quads.end(), v.entity_quads, v.entity_quads + v.entity_quads_len);
+  if (quads.empty()) quads.push_back(lw_quad{});  // voxel.rs:441-443
+  sc.quads.upload(quads.data(), quads.size(), b->stream);
+  std::vector<uint32_t> qi(quads.size());
+  sc.quad_flags.resize(quads.size());
+  for (size_t q = 0; q < quads.size(); q++) {
+    qi[q] = quad_info(quads[q]);
+    std::memcpy(&sc.quad_flags[q], &quads[q].p0[3], 4);
+  }
+  sc.qinfo.upload(qi.data(), qi.size(), b->stream);
+  sc.atlas.upload(v.atlas, v.atlas_len, b->stream);
+  sc.sky.upload(v.sky, v.sky ? v.sky_len : 0, b->stream);
+  float lut[256];
+  gamma_lut(lut);
+  sc.lut.upload(lut, 256, b->stream);
+
+  const lwb::Input in{{v.dims[0], v.dims[1], v.dims[2]},
+                      vox.ptr,
+                      v.brick_occludes ? occ.ptr : nullptr,
+                      flags.ptr,
+                      sc.models.ptr,
+                      (uint32_t)v.models_len,
+                      v.entity_quads_len ? sc.quads.ptr + v.quads_len : nullptr,
+                      (uint32_t)v.entity_quads_len,
+                      (uint32_t)v.quads_len};
+  lwb::Output o;
+  try {
+    lwb::build(b->stream, in, o);  // waits for the stream: the host vectors above may go
+  } catch (const lwb::Error &e) {
+    if (e.invalid) throw InvalidScene{e.what};
+    throw HipError{e.code, e.what};
+  }
+  sc.grid.adopt(o.grid, o.grid_len);
+  sc.emitters.adopt(o.emitters, o.emitters_len);
+  sc.egrid.adopt(o.egrid, o.egrid_len);
+  sc.occupancy.adopt(o.occupancy, o.occupancy_len);
+  uint64_t max_faces = 0;
+  for (size_t m = 0; m < v.models_len; m++)
+    if (v.model_flags[m] & LW_MODEL_EMITS) max_faces = std::max<uint64_t>(max_faces, v.models[m].quad_count);
+  sc.max_emitter_rays = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(max_faces * o.max_cell_list, 1), 1u << 20);
+
+  std::memset(&info, 0, sizeof(info));
+  for (int a = 0; a < 3; a++) info.dims[a] = v.dims[a];
+  std::memcpy(info.offsets, o.offsets, sizeof(info.offsets));
+  info.atlas_size[0] = v.atlas_size[0];
+  info.atlas_size[1] = v.atlas_size[1];
+  info.quads_len = (uint32_t)quads.size();
+  info.models_len = (uint32_t)v.models_len;
+  info.emitter_count = (uint32_t)o.emitters_count;
+  info.see_through[0] = (uint32_t)(v.quads_len + v.entity_quads_len - v.see_through_count);
+  info.see_through[1] = v.see_through_count;
+  info.has_sun_sprite = v.has_sun_sprite;
+  std::memcpy(info.sun_sprite, v.sun_sprite, sizeof(info.sun_sprite));
+  sc.is_built = true;
+  sc.built = info;
+  sc.atlas_len = v.atlas_len;
+  sc.sky_len = v.sky ? v.sky_len : 0;
+  sc.key_set = false;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->bricks = o.bricks;
+    stats->uniform_bricks = o.uniform_bricks;
+    stats->buried_bricks = o.buried_bricks;
+    stats->instances = v.models_len - 1;
+    stats->overlay_bricks = o.overlay_bricks;
+    stats->entity_quads = v.entity_quads_len;
+    stats->emitters = o.emitters_count;
+    stats->grid_len = o.grid_len;
+    stats->quads_len = quads.size();
+    stats->emitters_len = o.emitters_len;
+    stats->emitter_grid_len = o.egrid_len;
+    stats->kernel_ms = o.kernel_ms;
+    stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+}
 
 // Launches with the register budget b->waves selects (LW_WAVES: the minimum waves per SIMD
 // the compiler is held to; 1 leaves it free).
@@ -1373,8 +1530,7 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
                   const DiffDev *diff = nullptr, double *out_f64 = nullptr,
                   bool one_launch = false) {
   auto t_wall = std::chrono::steady_clock::now();
-  if (!(layout_key(params_in) == sc.key))
-    throw InvalidScene{"params do not match the layout the scene was validated with"};
+  check_layout(sc, params_in);
   lw_params p = params_in;
   p.frame[2] = 0;
   const uint32_t w = p.frame[0], h = p.frame[1];
@@ -1494,8 +1650,7 @@ void check_batch_view(const lw_params &a, const lw_params &v) {
 BatchPlan plan_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *views, size_t nv) {
   if (nv == 0 || nv > kMaxViews) throw InvalidScene{"a batch holds 1 to 256 views"};
   const lw_params &p0 = views[0].params;
-  if (!(layout_key(p0) == sc.key))
-    throw InvalidScene{"params do not match the layout the scene was validated with"};
+  check_layout(sc, p0);
   BatchPlan plan;
   BatchDev &bd = plan.bd();
   std::memset(&bd, 0, sizeof(bd));
@@ -1740,8 +1895,7 @@ DiffDev prepare_diff(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params 
 void adjoint_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
                    uint32_t spp_in, const lw_diff_desc &dd, const float *dldi, double *grads) {
   auto t_wall = std::chrono::steady_clock::now();
-  if (!(layout_key(params_in) == sc.key))
-    throw InvalidScene{"params do not match the layout the scene was validated with"};
+  check_layout(sc, params_in);
   lw_params p = params_in;
   p.frame[2] = 0;
   const uint32_t w = p.frame[0], h = p.frame[1];
@@ -2081,6 +2235,41 @@ int lw_hip_scene_upload(lw_hip_backend *b, const lw_scene_view *view, lw_hip_sce
 void lw_hip_scene_free(lw_hip_backend *b, lw_hip_scene *scene) {
   if (b) (void)hipSetDevice(b->device);
   delete scene;
+}
+
+int lw_hip_scene_build(lw_hip_backend *b, const lw_voxel_scene *vs, lw_scene_info *info,
+                       lw_build_stats *stats, lw_hip_scene **out) {
+  if (!b || !vs || !info || !out) return LW_EINVAL;
+  *out = nullptr;
+  auto *sc = new lw_hip_scene();
+  int rc = guarded(b, [&] { build_scene(b, *vs, *sc, *info, stats); });
+  if (rc != LW_OK) {
+    delete sc;
+    return rc;
+  }
+  *out = sc;
+  return LW_OK;
+}
+
+int lw_hip_scene_read(lw_hip_backend *b, const lw_hip_scene *scene, uint32_t which, void *dst,
+                      size_t cap, size_t *len) {
+  if (!b || !scene || !len) return LW_EINVAL;
+  return guarded(b, [&] {
+    const void *src = nullptr;
+    size_t n = 0, size = 0;
+    switch (which) {
+      case LW_SCENE_GRID: src = scene->grid.ptr; n = scene->grid.n; size = 4; break;
+      case LW_SCENE_EMITTERS: src = scene->emitters.ptr; n = scene->emitters.n; size = sizeof(lw_emitter); break;
+      case LW_SCENE_EMITTER_GRID: src = scene->egrid.ptr; n = scene->egrid.n; size = 4; break;
+      case LW_SCENE_QUADS: src = scene->quads.ptr; n = scene->quads.n; size = sizeof(lw_quad); break;
+      default: throw InvalidScene{"unknown scene buffer"};
+    }
+    *len = n;
+    if (!dst) return;
+    if (cap < n) throw InvalidScene{"destination smaller than the buffer"};
+    HIP_TRY(hipSetDevice(b->device));
+    if (n) HIP_TRY(hipMemcpy(dst, src, n * size, hipMemcpyDeviceToHost));
+  });
 }
 
 int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _native
 from .abi import (EMITTER, EMITTER_SAMPLING_NONE, EMITTER_SAMPLING_ONE, MODEL, PARAMS, QUAD,
-                  HqFrame, HqScene)
+                  HqFrame, HqScene, VoxelVolume)
 
 f3 = ctypes.c_float * 3
 
@@ -78,7 +78,9 @@ class WorldC(ctypes.Structure):
                 ("quad_type", ctypes.c_void_p), ("quad_emitter", ctypes.c_void_p),
                 ("bricks", ctypes.c_uint64), ("uniform_bricks", ctypes.c_uint64),
                 ("buried_bricks", ctypes.c_uint64), ("solid_cells", ctypes.c_uint64),
-                ("water_cells", ctypes.c_uint64), ("cutout_cells", ctypes.c_uint64)]
+                ("water_cells", ctypes.c_uint64), ("cutout_cells", ctypes.c_uint64),
+                ("voxels", ctypes.c_void_p), ("voxels_len", ctypes.c_size_t),
+                ("brick_occludes", ctypes.c_void_p), ("model_flags", ctypes.c_void_p)]
 
 
 def _v3(a) -> "ctypes.Array":
@@ -250,7 +252,10 @@ def _copy(ptr, n, dtype):
     return np.frombuffer(bytes(buf), dtype=dtype).copy()
 
 
-def build_world(spec: WorldSpec) -> HqScene:
+def build_world(spec: WorldSpec, volume: bool = False) -> HqScene:
+    """The synthetic world's HQ scene (lw_world_build). With volume=True it also carries
+    `.volume`, the interned voxel volume the layout was made from (a VoxelVolume, what
+    lw_hip_scene_build takes)."""
     L = _native.lib()
     h = ctypes.c_void_p()
     rc = L.lw_world_build(ctypes.byref(spec.c()), ctypes.byref(h))
@@ -279,6 +284,13 @@ def build_world(spec: WorldSpec) -> HqScene:
         # rendering's parameter groups.
         scene.quad_type = _copy(w.quad_type, w.quads_len, "<u4")
         scene.quad_emitter = _copy(w.quad_emitter, w.quads_len, "<u4")
+        if volume:
+            nb = int(info.dims[0]) * int(info.dims[1]) * int(info.dims[2])
+            scene.volume = VoxelVolume(
+                dims=tuple(info.dims), voxels=_copy(w.voxels, w.voxels_len, "<u2"),
+                models=scene.models, model_flags=_copy(w.model_flags, w.models_len, np.uint8),
+                quads=scene.quads, atlas=scene.atlas, atlas_size=scene.atlas_size,
+                brick_occludes=_copy(w.brick_occludes, nb, np.uint8))
         return scene
     finally:
         L.lw_world_free(h)

@@ -95,6 +95,10 @@ def lib():
         L.lwo_check_encode_thresholds.restype = ctypes.c_uint64
         L.lwo_powf_band.argtypes = [ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
         L.lwo_powf_band.restype = None
+        L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
+        L.lwo_build_layout.restype = i32
+        L.lwo_build_fetch.argtypes = [vp, vp, vp]
+        L.lwo_build_fetch.restype = None
         L.lwo_sky_tangent.argtypes = [vp, vp, i32, vp, sz, vp]
         L.lwo_sky_tangent.restype = i32
         L.lwo_preetham_radiance.argtypes = [vp, vp, sz, vp]
@@ -433,3 +437,32 @@ def powf_band(threads: int = 0, inputs: bool = False):
                         band.size if inputs else 0)
     out = (int(st[0]), int(st[1]), int(st[2]), int(st[3]), mr.value)
     return out + (band[: min(int(st[2]), band.size)],) if inputs else out
+
+
+def build_layout(vol):
+    """build_hq's layout of a VoxelVolume (lumenwright_amd.abi): (grid, emitters, emitter grid,
+    dict of offsets, see_through, emitter_count and the brick counts)."""
+    from lumenwright_amd.abi import EMITTER, MODEL, QUAD
+    dims = np.array(vol.dims, dtype="<u4")
+    vox = np.ascontiguousarray(vol.voxels, dtype="<u2").reshape(-1)
+    occ = None if vol.brick_occludes is None else np.ascontiguousarray(vol.brick_occludes, np.uint8)
+    models = np.ascontiguousarray(vol.models, dtype=MODEL)
+    flags = np.ascontiguousarray(vol.model_flags, dtype=np.uint8)
+    ent = np.ascontiguousarray(vol.entity_quads, dtype=QUAD)
+    sizes = np.zeros(3, dtype=np.uint64)
+    meta = np.zeros(8, dtype="<u4")
+    stats = np.zeros(4, dtype=np.uint64)
+    rc = lib().lwo_build_layout(_addr(dims), _addr(vox), _addr(occ) if occ is not None else None,
+                                _addr(models), models.size, _addr(flags), len(vol.quads),
+                                _addr(ent) if ent.size else None, ent.size, vol.see_through_count,
+                                _addr(sizes), _addr(meta), _addr(stats))
+    if rc != 0:
+        raise RuntimeError(lib().lwo_last_error().decode())
+    grid = np.zeros(int(sizes[0]), dtype="<u4")
+    em = np.zeros(int(sizes[1]), dtype=EMITTER)
+    eg = np.zeros(int(sizes[2]), dtype="<u4")
+    lib().lwo_build_fetch(_addr(grid), _addr(em), _addr(eg))
+    info = {"offsets": tuple(int(x) for x in meta[:5]), "see_through": (int(meta[5]), int(meta[6])),
+            "emitter_count": int(meta[7]), "bricks": int(stats[0]), "uniform_bricks": int(stats[1]),
+            "buried_bricks": int(stats[2]), "overlay_bricks": int(stats[3])}
+    return grid, em, eg, info

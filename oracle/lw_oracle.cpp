@@ -7,6 +7,7 @@
 #include "lw_oracle_impl.hpp"
 #include "lw_oracle_diff.hpp"
 #include "lw_oracle_post.hpp"
+#include "lw_oracle_build.hpp"
 
 #include <algorithm>
 #include <array>
@@ -444,6 +445,40 @@ void lwo_powf_band(unsigned threads, uint64_t stats[4], double *max_rel, uint32_
     for (uint32_t u : found[t])
       if (at < cap) band[at++] = u;
   }
+}
+
+// build_hq's layout (voxel.rs:305-491, emitter_list :542-607) of an interned volume. sizes:
+// grid, emitters, emitter grid lengths; meta: offsets[5], see_through[2], emitter_count;
+// stats: bricks, uniform, buried, overlay bricks. The buffers are fetched with
+// lwo_build_fetch (the last build on this thread).
+static thread_local lwo_build::Built g_built;
+
+int lwo_build_layout(const uint32_t *dims, const uint16_t *voxels, const uint8_t *brick_occludes,
+                     const lwo_build::Model *models, size_t models_len, const uint8_t *flags,
+                     size_t quads_len, const lwo_build::Quad *entity, size_t n_entity,
+                     uint32_t see_through_count, uint64_t *sizes, uint32_t *meta, uint64_t *stats) {
+  return guarded([&] {
+    lwo_build::Volume v{{dims[0], dims[1], dims[2]}, voxels, brick_occludes, models, models_len, flags,
+                        quads_len, entity, n_entity, see_through_count};
+    g_built = lwo_build::build(v);
+    sizes[0] = g_built.grid.size();
+    sizes[1] = g_built.emitters.size();
+    sizes[2] = g_built.emitter_grid.size();
+    for (int k = 0; k < 5; k++) meta[k] = g_built.offsets[k];
+    meta[5] = g_built.see_through[0];
+    meta[6] = g_built.see_through[1];
+    meta[7] = g_built.emitter_count;
+    stats[0] = g_built.bricks;
+    stats[1] = g_built.uniform_bricks;
+    stats[2] = g_built.buried_bricks;
+    stats[3] = g_built.overlay_bricks;
+  });
+}
+
+void lwo_build_fetch(uint32_t *grid, lwo_build::Emitter *emitters, uint32_t *egrid) {
+  std::copy(g_built.grid.begin(), g_built.grid.end(), grid);
+  std::copy(g_built.emitters.begin(), g_built.emitters.end(), emitters);
+  std::copy(g_built.emitter_grid.begin(), g_built.emitter_grid.end(), egrid);
 }
 
 }  // extern "C"
