@@ -30,7 +30,9 @@ Besides the throughput the config-1 line carries:
                 host's cores over a bounded slice of the same frame (the first S samples of
                 every pixel), timed in this run;
   mrays_per_s   Msamples/s times the trace calls per sample the oracle counts;
-  post          the device tonemap and tone mix of the last frame (kernel time, GB/s).
+  post          the device tonemap and tone mix of the last frame (kernel time, GB/s);
+  scene_build   the world's layout made on the device from its voxel volume
+                (lw_hip_scene_build), and the oracle's CPU restatement of it.
 """
 
 import argparse
@@ -146,12 +148,38 @@ def post_timing(b, frames, img, reps: int = 5) -> dict:
     return out
 
 
+def build_timing(b, scene, sky, cpu: bool, reps: int = 3) -> dict:
+    """lw_hip_scene_build of the frame's world from its interned voxel volume: the best of a
+    few builds -- the device span (HIP events from the first layout kernel to the last) and the
+    whole call with the volume's upload -- and the volume's bytes over the device span. With
+    `cpu`, the oracle's single-threaded restatement of the same layout, timed once."""
+    vol = scene.volume
+    best = None
+    for _ in range(reps):
+        sc, _, st = b.build_scene(vol, sky)
+        sc.free()
+        if best is None or st["kernel_ms"] < best["kernel_ms"]:
+            best = st
+    out = {"bricks": int(np.prod(vol.dims)), "stored_bricks": best["bricks"],
+           "uniform_bricks": best["uniform_bricks"], "buried_bricks": best["buried_bricks"],
+           "emitters": best["emitters"], "device_ms": best["kernel_ms"], "call_ms": best["wall_ms"],
+           "voxel_GBps": vol.voxels.nbytes / best["kernel_ms"] / 1e6}
+    if cpu:
+        from oracle import ffi
+        ffi.build()
+        t = time.perf_counter()
+        ffi.build_layout(vol)
+        out["cpu_port_ms"] = (time.perf_counter() - t) * 1e3
+    return out
+
+
 def bench_plain(args, lw, frames):
     cfg = frames.CONFIGS[args.config]
     if args.spp:
         cfg = frames.config(args.config, spp=args.spp)
     t0 = time.perf_counter()
-    frame = frames.make_frame(cfg)
+    world = frames.build_world(cfg.world, volume=True)
+    frame = frames.make_frame(cfg, world)
     log(f"frame built in {time.perf_counter() - t0:.1f} s: {cfg.name}")
     b = lw.HipBackend.open(0)
     scene = b.upload(frame)
@@ -180,8 +208,10 @@ def bench_plain(args, lw, frames):
     samples = w * h * spp * args.steps
     value = samples / elapsed / 1e6
     assert np.isfinite(img).all()
-    extra = {"post": post_timing(b, frames, img)}
+    extra = {"post": post_timing(b, frames, img),
+             "scene_build": build_timing(b, world, frame.sky, not args.no_cpu_baseline)}
     log(f"post-processing: {extra['post']}")
+    log(f"scene build: {extra['scene_build']}")
     if args.copy_probe:
         ms = b.copy_probe(args.copy_probe)
         extra["copy_probe"] = {"bytes": args.copy_probe, "ms": ms,
