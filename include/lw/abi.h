@@ -444,6 +444,22 @@ int lw_hip_downsample(lw_hip_backend *b, const uint32_t *px, uint32_t w, uint32_
 int lw_hip_render_hq_rgba8(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
                            uint32_t spp, float exposure, uint32_t op, uint32_t *out_rgba8);
 
+/* ---- sky preparation on the device (crates/lumenwright/src/render/sky.rs) ---- */
+
+/* build_sky_cache (crates/lumenwright-kernel/src/integrator.rs:133-148, called by
+ * sky::sky_cache, sky.rs:593-611) for params.misc_f[2] as the resolution n: (n + 1)^2 * 3
+ * floats into out. *len receives the count; with out null only *len is written. */
+int lw_hip_sky_cache(lw_hip_backend *b, const lw_params *params, float *out, size_t cap,
+                     size_t *len);
+
+/* blend_chunky_sky (sky.rs:836-851) in place: every texel of the lens's map, then the
+ * light's (sizes params.sky_map, maps_len = (w0 h0 + w1 h1) * 3 floats) becomes
+ * weight * chunky(d) + (1 - weight) * texel for its direction d, chunky the kernel's
+ * sky_radiance through the sky cache of params (Preetham sky, misc_f[2] its resolution), as
+ * request.rs:891-911 builds it. */
+int lw_hip_blend_chunky_sky(lw_hip_backend *b, const lw_params *params, float weight, float *maps,
+                            size_t maps_len);
+
 /* ---- autotuning (no counterpart in the reference) ---- */
 
 #define LW_FORM_PERSISTENT 0u

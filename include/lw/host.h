@@ -171,6 +171,12 @@ int32_t lw_world_height(const lw_world_spec *spec, int32_t x, int32_t z);
 int lw_world_build(const lw_world_spec *spec, lw_world **out);
 void lw_world_free(lw_world *w);
 
+/* The directions of a w x h equirectangular map's texels (map_direction, sky.rs:856-860)
+ * as tables: az[2i], az[2i+1] = cos, sin of column i's azimuth (i + 1/2) / w * tau and
+ * el[2j], el[2j+1] = cos, sin of row j's height (j + 1/2) / h * pi - pi/2, with libm's cosf
+ * and sinf; texel (i, j) looks along (el_cos * az_cos, el_sin, el_cos * az_sin). */
+void lw_map_trig(uint32_t w, uint32_t h, float *az, float *el);
+
 /* A procedural dusk sky as the game look's two equirectangular maps (the lens's, then the
  * light's; layout of integrator.rs:190-243): writes lw*lh*3 + gw*gh*3 floats. */
 long lw_dusk_sky_maps(const float sun_dir[3], uint32_t lw, uint32_t lh, uint32_t gw, uint32_t gh,

@@ -272,6 +272,27 @@ class HipBackend:
         self.encode_setup_ms = ms.value
         return out
 
+    def sky_cache(self, params: np.ndarray) -> np.ndarray:
+        """lw_hip_sky_cache: build_sky_cache (integrator.rs:133-148) on the device at the
+        resolution params.misc_f[2]."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        n = ctypes.c_size_t()
+        L = _native.lib()
+        self._check(L.lw_hip_sky_cache(self.handle, _addr(p), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype="<f4")
+        if n.value:
+            self._check(L.lw_hip_sky_cache(self.handle, _addr(p), _addr(out), out.size,
+                                           ctypes.byref(n)))
+        return out
+
+    def blend_chunky_sky(self, params: np.ndarray, weight: float, maps: np.ndarray) -> np.ndarray:
+        """lw_hip_blend_chunky_sky (sky.rs:836-851): a blended copy of the two maps."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        out = np.ascontiguousarray(maps, dtype="<f4").copy()
+        self._check(_native.lib().lw_hip_blend_chunky_sky(self.handle, _addr(p), weight, _addr(out),
+                                                          out.size))
+        return out
+
     def downsample(self, px: np.ndarray, w: int, h: int, k: int) -> np.ndarray:
         """lw_hip_downsample (render/post.rs:218-247)."""
         a = np.ascontiguousarray(px, dtype="<u4").reshape(-1)

@@ -29,6 +29,7 @@
 #include "lw_path.hpp"
 #include "lw_diff.hpp"
 #include "lw_build.hpp"
+#include "lw_sky.hpp"
 
 using namespace lw;
 
@@ -366,6 +367,38 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
   o[0] = c.x;
   o[1] = c.y;
   o[2] = c.z;
+}
+
+// ---- sky preparation (render/sky.rs, integrator.rs:133-148) ----
+
+// build_sky_cache (integrator.rs:133-148), one lane per entry.
+__global__ void __launch_bounds__(kBlock) k_sky_cache(const lw_params p, uint32_t n, float *__restrict__ out) {
+  const uint32_t m = n + 1u;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m * m) return;
+  lw::sky_cache_entry(p, n, k / m, k % m, out + (size_t)k * 3);
+}
+
+// blend_chunky_sky (sky.rs:836-851) over one w x h map, one lane per texel: the texel's
+// direction (map_direction, sky.rs:856-860) from the per-column (cos, sin) of its azimuth and
+// the per-row (cos, sin) of its height, and Chunky's sky there as the kernel reads it through
+// the cache (sky_radiance).
+__global__ void __launch_bounds__(kBlock) k_blend_sky(float *__restrict__ map, uint32_t w, uint32_t h,
+                                                      const float2 *__restrict__ az,
+                                                      const float2 *__restrict__ el,
+                                                      const float *__restrict__ cache, const lw_params p,
+                                                      float weight) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= w * h) return;
+  const float2 a = az[k % w], e = el[k / w];
+  const V3 d = v3(e.x * a.x, e.y, e.x * a.y);
+  DevScene s{};
+  s.sky = cache;
+  const V3 c = sky_radiance(s, p, d);
+  float *t = map + (size_t)k * 3;
+  t[0] = weight * c.x + (1.0f - weight) * t[0];
+  t[1] = weight * c.y + (1.0f - weight) * t[1];
+  t[2] = weight * c.z + (1.0f - weight) * t[2];
 }
 
 // ---- post-processing (render/post.rs) ----
@@ -2546,6 +2579,83 @@ int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float expo
     if (!ends) ensure_pow_fix(b);
     b->tone_in.upload(hdr, n * 3, b->stream);
     tonemap_device(b, b->tone_in.ptr, n, t, !ends, out);
+  });
+}
+
+int lw_hip_sky_cache(lw_hip_backend *b, const lw_params *params, float *out, size_t cap, size_t *len) {
+  if (!b || !params || !len) return LW_EINVAL;
+  return guarded(b, [&] {
+    const float res = params->misc_f[2];
+    if (!(res >= 0.0f) || res > 4096.0f) throw InvalidScene{"sky cache resolution out of range (0 .. 4096)"};
+    const uint32_t n = as_u32(res);
+    const size_t need = n ? (size_t)(n + 1) * (n + 1) * 3 : 0;
+    *len = need;
+    if (!out || !need) return;
+    if (cap < need) throw InvalidScene{"destination smaller than the sky cache"};
+    HIP_TRY(hipSetDevice(b->device));
+    DevBuf<float> dst;
+    dst.alloc(need);
+    const uint32_t entries = (n + 1) * (n + 1);
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    k_sky_cache<<<dim3((entries + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream>>>(*params, n, dst.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipMemcpyAsync(out, dst.ptr, need * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    lw_timing tm{};
+    tm.trace_ms = ms;
+    tm.launches = 1;
+    tm.samples = entries;
+    b->timing = tm;
+  });
+}
+
+int lw_hip_blend_chunky_sky(lw_hip_backend *b, const lw_params *params, float weight, float *maps,
+                            size_t maps_len) {
+  if (!b || !params || !maps) return LW_EINVAL;
+  return guarded(b, [&] {
+    const float *sz = params->sky_map;
+    for (int k = 0; k < 4; k++)
+      if (!(sz[k] >= 1.0f) || sz[k] > 65536.0f) throw InvalidScene{"sky map sizes out of range (1 .. 65536)"};
+    const uint32_t lw_ = as_u32(sz[0]), lh = as_u32(sz[1]), gw = as_u32(sz[2]), gh = as_u32(sz[3]);
+    const size_t lens_n = (size_t)lw_ * lh, light_n = (size_t)gw * gh;
+    if (maps_len != (lens_n + light_n) * 3) throw InvalidScene{"maps_len is not the two maps' size"};
+    if (lens_n > 0xFFFFFFFFull || light_n > 0xFFFFFFFFull) throw InvalidScene{"sky map too large"};
+    const float res = params->misc_f[2];
+    if (!(res >= 1.0f) || res > 4096.0f) throw InvalidScene{"the blend needs a sky cache resolution (1 .. 4096)"};
+    const uint32_t n = as_u32(res);
+    HIP_TRY(hipSetDevice(b->device));
+    std::vector<float> tab(2 * ((size_t)lw_ + lh + gw + gh));
+    lw_map_trig(lw_, lh, tab.data(), tab.data() + 2 * lw_);
+    float *t2 = tab.data() + 2 * ((size_t)lw_ + lh);
+    lw_map_trig(gw, gh, t2, t2 + 2 * gw);
+    DevBuf<float> cache, dmaps, dtab;
+    const uint32_t entries = (n + 1) * (n + 1);
+    cache.alloc((size_t)entries * 3);
+    dmaps.upload(maps, maps_len, b->stream);
+    dtab.upload(tab.data(), tab.size(), b->stream);
+    const float2 *ft = reinterpret_cast<const float2 *>(dtab.ptr);
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    k_sky_cache<<<dim3((entries + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream>>>(*params, n, cache.ptr);
+    HIP_TRY(hipGetLastError());
+    k_blend_sky<<<dim3((unsigned)((lens_n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        dmaps.ptr, lw_, lh, ft, ft + lw_, cache.ptr, *params, weight);
+    HIP_TRY(hipGetLastError());
+    k_blend_sky<<<dim3((unsigned)((light_n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        dmaps.ptr + lens_n * 3, gw, gh, ft + lw_ + lh, ft + lw_ + lh + gw, cache.ptr, *params, weight);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipMemcpyAsync(maps, dmaps.ptr, maps_len * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    lw_timing tm{};
+    tm.trace_ms = ms;
+    tm.launches = 3;
+    tm.samples = lens_n + light_n;
+    b->timing = tm;
   });
 }
 

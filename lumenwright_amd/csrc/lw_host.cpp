@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "lw_math.hpp"
+#include "lw_sky.hpp"
 
 using namespace lw;
 
@@ -47,53 +48,6 @@ inline void normalize3(float v[3]) {
 inline double rem_euclid(double a, double b) {
   double r = std::fmod(a, b);
   return r < 0.0 ? r + std::fabs(b) : r;
-}
-
-// integrator.rs:44-74, on the host (libm expf).
-V3 preetham_radiance(const lw_params &p, V3 d) {
-  V3 sun = v3(p.sky_sun[0], p.sky_sun[1], p.sky_sun[2]);
-  float cos_theta = f_max(d.y + p.sky_zenith[3], 0.0f);
-  float cos_gamma = f_clamp(dot(d, sun), -1.0f, 1.0f);
-  float gamma = f_acos(cos_gamma);
-  float cos2_gamma = cos_gamma * cos_gamma;
-  const float *A = p.perez_a, *B = p.perez_b, *C = p.perez_c, *D = p.perez_d, *E = p.perez_e;
-  auto perez = [&](int k) -> float {
-    float a = cos_theta > 0.0f ? A[k] * ::expf(B[k] / cos_theta) : 0.0f;
-    return (1.0f + a) * (1.0f + C[k] * ::expf(D[k] * gamma) + E[k] * cos2_gamma);
-  };
-  float x = p.sky_zenith[0] * perez(0);
-  float y = p.sky_zenith[1] * perez(1);
-  float lum = p.sky_zenith[2] * perez(2);
-  if (y <= 5e-8f) return v3(0.0f, 0.0f, 0.0f);
-  float f = lum / y;
-  float cx = x * f, cy = lum, cz = (1.0f - x - y) * f;
-  return v3(2.3706743f * cx - 0.9000405f * cy - 0.4706338f * cz,
-            -0.513885f * cx + 1.4253036f * cy + 0.0885814f * cz,
-            0.0052982f * cx - 0.0146949f * cy + 1.0093968f * cz) *
-         0.045f;
-}
-
-// integrator.rs:79-104
-void rgb_to_hsl(V3 c, float out[3]) {
-  float r = f_clamp(c.x, 0.0f, 1.0f), g = f_clamp(c.y, 0.0f, 1.0f), b = f_clamp(c.z, 0.0f, 1.0f);
-  float cmax = f_max(f_max(r, g), b);
-  float cmin = f_min(f_min(r, g), b);
-  float delta = cmax - cmin;
-  float lightness = (cmax + cmin) / 2.0f;
-  float hue;
-  if (delta == 0.0f) {
-    hue = 0.0f;
-  } else if (cmax == r) {
-    hue = std::fmod((g - b) / delta, 6.0f) / 6.0f;
-  } else if (cmax == g) {
-    hue = (((b - r) / delta) + 2.0f) / 6.0f;
-  } else {
-    hue = (((r - g) / delta) + 4.0f) / 6.0f;
-  }
-  float saturation = delta < 5e-8f ? 0.0f : delta / (1.0f - f_abs(2.0f * lightness - 1.0f));
-  out[0] = hue;
-  out[1] = saturation;
-  out[2] = lightness;
 }
 
 // ---- synthetic worlds ----
@@ -389,25 +343,26 @@ long lw_sky_cache(const lw_params *params, float *out, size_t cap) {
   const uint32_t m = n + 1;
   const size_t need = (size_t)m * m * 3;
   if (cap < need) return -1;
-  for (uint32_t i = 0; i < m; i++) {
-    float sin_t, cos_t;
-    sin_cos_turns((float)i / (float)n, sin_t, cos_t);
-    for (uint32_t j = 0; j < m; j++) {
-      float sin_p, cos_p;
-      sin_cos_turns((float)j / (float)(2u * n) - 0.25f, sin_p, cos_p);
-      float r = f_abs(cos_p);
-      float hsl[3];
-      rgb_to_hsl(preetham_radiance(*params, v3(cos_t * r, sin_p, sin_t * r)), hsl);
-      size_t a = (size_t)(i * m + j) * 3;
-      out[a] = hsl[0];
-      out[a + 1] = hsl[1];
-      out[a + 2] = hsl[2];
-    }
-  }
+  for (uint32_t i = 0; i < m; i++)
+    for (uint32_t j = 0; j < m; j++) lw::sky_cache_entry(*params, n, i, j, out + (size_t)(i * m + j) * 3);
   return (long)need;
 }
 
 void lw_gamma_lut(float out[256]) { lw::gamma_lut(out); }
+
+void lw_map_trig(uint32_t w, uint32_t h, float *az, float *el) {
+  const float TAU = 6.28318530717958647692f, FRAC_PI_2 = 1.57079632679489661923f;
+  for (uint32_t i = 0; i < w; i++) {
+    const float a = ((float)i + 0.5f) / (float)w * TAU;
+    az[2 * i] = std::cos(a);
+    az[2 * i + 1] = std::sin(a);
+  }
+  for (uint32_t j = 0; j < h; j++) {
+    const float e = ((float)j + 0.5f) / (float)h * PI_F - FRAC_PI_2;
+    el[2 * j] = std::cos(e);
+    el[2 * j + 1] = std::sin(e);
+  }
+}
 
 void lw_expf_tabulated(const float *x, size_t n, float *out) {
   for (size_t i = 0; i < n; i++) out[i] = expf_tabulated(x[i]);

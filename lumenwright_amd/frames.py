@@ -139,6 +139,14 @@ def sky_cache(params: np.ndarray) -> np.ndarray:
     return out
 
 
+def map_trig(w: int, h: int):
+    """lw_map_trig: (cos, sin) per column azimuth and per row height of a w x h map."""
+    az = np.zeros((w, 2), dtype="<f4")
+    el = np.zeros((h, 2), dtype="<f4")
+    _native.lib().lw_map_trig(w, h, ctypes.c_void_p(az.ctypes.data), ctypes.c_void_p(el.ctypes.data))
+    return az, el
+
+
 def gamma_lut() -> np.ndarray:
     out = np.zeros(256, dtype="<f4")
     _native.lib().lw_gamma_lut(ctypes.c_void_p(out.ctypes.data))

@@ -95,6 +95,8 @@ def lib():
         L.lwo_check_encode_thresholds.restype = ctypes.c_uint64
         L.lwo_powf_band.argtypes = [ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
         L.lwo_powf_band.restype = None
+        L.lwo_blend_chunky_sky.argtypes = [vp, ctypes.c_float, vp, vp]
+        L.lwo_blend_chunky_sky.restype = i32
         L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
         L.lwo_build_layout.restype = i32
         L.lwo_build_fetch.argtypes = [vp, vp, vp]
@@ -466,3 +468,11 @@ def build_layout(vol):
             "emitter_count": int(meta[7]), "bricks": int(stats[0]), "uniform_bricks": int(stats[1]),
             "buried_bricks": int(stats[2]), "overlay_bricks": int(stats[3])}
     return grid, em, eg, info
+
+
+def blend_chunky_sky(view, weight: float, sizes, maps):
+    """blend_chunky_sky (sky.rs:836-851) of a copy of `maps` with the view's sky cache."""
+    out = np.ascontiguousarray(maps, dtype="<f4").copy()
+    sz = np.ascontiguousarray(sizes, dtype="<f4")
+    _check(lib().lwo_blend_chunky_sky(view.ref, weight, _addr(sz), _addr(out)))
+    return out

@@ -447,6 +447,33 @@ void lwo_powf_band(unsigned threads, uint64_t stats[4], double *max_rel, uint32_
   }
 }
 
+// map_direction (sky.rs:856-860) and blend_chunky_sky (sky.rs:836-851) with the kernel's
+// sky_radiance through the view's sky cache as Chunky's sky (request.rs:891-911): maps holds
+// the lens's then the light's map, sizes = view params' sky_map.
+int lwo_blend_chunky_sky(const View *view, float weight, const float *sizes, float *maps) {
+  return guarded([&] {
+    const float TAU = 6.28318530717958647692f, PI = 3.14159265358979323846f,
+                FRAC_PI_2 = 1.57079632679489661923f;
+    const uint32_t dims[2][2] = {{(uint32_t)sizes[0], (uint32_t)sizes[1]},
+                                 {(uint32_t)sizes[2], (uint32_t)sizes[3]}};
+    float *buf = maps;
+    for (auto &wh : dims) {
+      const uint32_t w = wh[0], h = wh[1];
+      for (uint32_t k = 0; k < w * h; k++) {
+        const float az = ((float)(k % w) + 0.5f) / (float)w * TAU;
+        const float el = ((float)(k / w) + 0.5f) / (float)h * PI - FRAC_PI_2;
+        const V3 d = v3(std::cos(el) * std::cos(az), std::sin(el), std::cos(el) * std::sin(az));
+        const V3 c = sky_radiance(*view, d);
+        float *t = buf + (size_t)k * 3;
+        t[0] = weight * c.x + (1.0f - weight) * t[0];
+        t[1] = weight * c.y + (1.0f - weight) * t[1];
+        t[2] = weight * c.z + (1.0f - weight) * t[2];
+      }
+      buf += (size_t)w * h * 3;
+    }
+  });
+}
+
 // build_hq's layout (voxel.rs:305-491, emitter_list :542-607) of an interned volume. sizes:
 // grid, emitters, emitter grid lengths; meta: offsets[5], see_through[2], emitter_count;
 // stats: bricks, uniform, buried, overlay bricks. The buffers are fetched with

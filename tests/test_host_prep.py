@@ -342,3 +342,43 @@ def test_device_exp_is_glibcs_expf(oracle):
     _native.lib().lw_expf_tabulated(ctypes.c_void_p(nan_in.ctypes.data), 1,
                                     ctypes.c_void_p(nan_out.ctypes.data))
     assert np.isnan(nan_out[0])
+
+
+def test_map_directions_are_libms_within_an_ulp():
+    # map_direction (sky.rs:856-860): (i + 1/2) / w * tau and (j + 1/2) / h * pi - pi/2, their
+    # cos and sin in f32 (libm); against f64 of the same f32 angles, within one f32 ulp.
+    w, h = 360, 180
+    az, el = frames.map_trig(w, h)
+    a = ((np.arange(w, dtype="<f4") + np.float32(0.5)) / np.float32(w)) * np.float32(2 * np.pi)
+    e = ((np.arange(h, dtype="<f4") + np.float32(0.5)) / np.float32(h)) * np.float32(np.pi) \
+        - np.float32(np.pi / 2)
+    for got, ang in ((az, a), (el, e)):
+        for k, f in ((0, np.cos), (1, np.sin)):
+            exact = f(ang.astype(np.float64))
+            ulp = np.spacing(np.abs(exact).astype("<f4")).astype(np.float64)
+            assert np.all(np.abs(got[:, k] - exact) <= ulp), f
+
+
+def test_the_blend_is_chunkys_sky_at_weight_one_and_the_map_at_zero(oracle):
+    # blend_chunky_sky (sky.rs:836-851): weight 0 leaves every texel, weight 1 makes each
+    # the kernel's sky_radiance along the texel's direction (lens map first, i fastest).
+    frame = frames.make_frame(frames.config(0, width=8, height=8, spp=1))
+    p = frame.params.copy()
+    sizes = [40, 20, 36, 18]
+    p["sky_map"] = sizes
+    s = frame.scene
+    v = BoundView(p, s.grid, s.models, s.quads, s.atlas, s.emitters, s.emitter_grid, frame.sky)
+    maps = np.random.default_rng(3).random((40 * 20 + 36 * 18) * 3).astype("<f4")
+    assert np.array_equal(oracle.blend_chunky_sky(v, 0.0, sizes, maps), maps)
+    one = oracle.blend_chunky_sky(v, 1.0, sizes, maps)
+    dirs = []
+    for w, h in ((40, 20), (36, 18)):
+        az, el = frames.map_trig(w, h)
+        j, i = np.divmod(np.arange(w * h), w)
+        dirs.append(np.stack([el[j, 0] * az[i, 0], el[j, 1], el[j, 0] * az[i, 1]], -1))
+    d = np.concatenate(dirs).astype("<f4")
+    want = oracle.sky_radiance(v, d)
+    assert np.array_equal(one.reshape(-1, 3).view("<u4"), want.reshape(-1, 3).view("<u4"))
+    half = oracle.blend_chunky_sky(v, 0.25, sizes, maps)
+    lin = np.float32(0.25) * want.reshape(-1) + np.float32(0.75) * maps
+    assert np.array_equal(half.view("<u4"), lin.astype("<f4").view("<u4"))
