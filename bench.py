@@ -32,7 +32,9 @@ Besides the throughput the config-1 line carries:
   mrays_per_s   Msamples/s times the trace calls per sample the oracle counts;
   post          the device tonemap and tone mix of the last frame (kernel time, GB/s);
   scene_build   the world's layout made on the device from its voxel volume
-                (lw_hip_scene_build), and the oracle's CPU restatement of it.
+                (lw_hip_scene_build), and the oracle's CPU restatement of it;
+  sky_prep      the game's sky maps, Chunky's sky cache and their blend on the device, and
+                the oracle's CPU game_sky_maps.
 """
 
 import argparse
@@ -173,6 +175,50 @@ def build_timing(b, scene, sky, cpu: bool, reps: int = 3) -> dict:
     return out
 
 
+def sky_timing(b, frames, cpu: bool) -> dict:
+    """Sky preparation on the device: the game's two maps (lw_hip_game_sky_maps, the dusk sky
+    of the reference's own test, sky.rs:1462-1491), Chunky's sky cache and its blend into
+    the maps; the best device time of a few calls each. With `cpu`, the oracle's
+    single-threaded game_sky_maps timed once."""
+    cam = frames.CameraRaysC()
+    cam.forward[:] = [-0.83153975, -0.3506493, 0.43079773]
+    cam.right[:] = [-0.45915133, 0.0, -0.888358]
+    cam.up[:] = [0.0, 1.0, 0.0]
+    cam.tan_half_fov, cam.aspect = 0.577, 16.0 / 9.0
+    sun = frames.celestial_direction(90.29)
+
+    def desc(bodies):
+        return frames.sky_description(
+            sky=(0.18, 0.24, 0.37), fog=(0.62, 0.40, 0.33), sky_fog_end=192.0,
+            render_fog=(172.8, 192.0), env_fog=(0.0, 1024.0), sky_disc=True,
+            sunrise=(0.98, 0.57, 0.25, 0.62), sun_angle=90.29,
+            sun=(sun, (0.0, 0.0, 2.0, 2.0)) if bodies else None, star_angle=90.29,
+            star_brightness=0.38 if bodies else 0.0)
+    lens, light = frames.quick_sky_new(cam, 16, 9, desc(True)), frames.quick_sky_new(cam, 16, 9, desc(False))
+    # the sun's 2 x 2 white sprite; one more row and column, since a texel coordinate can round
+    # up to the rect's far edge
+    atlas = np.full(9, 0xFFFFFFFF, "<u4")
+    colour = np.array([0x92, 0x92, 0xFF], "<f4") / np.float32(255.0)
+    stars = frames.star_table()
+    maps, info = b.game_sky_maps(lens, light, 0.378182, colour, atlas, 2, stars)  # makes the table
+    game_ms = min(b.game_sky_maps(lens, light, 0.378182, colour, atlas, 2, stars)[1]["kernel_ms"]
+                  for _ in range(3))
+    frame = frames.make_frame(frames.config(0, width=8, height=8, spp=1))
+    p = frame.params.copy()
+    p["sky_map"] = info["sizes"]
+    cache_ms = min((b.sky_cache(p), b.last_timing()["trace_ms"])[1] for _ in range(3))
+    blend_ms = min((b.blend_chunky_sky(p, 0.5, maps), b.last_timing()["trace_ms"])[1] for _ in range(3))
+    out = {"texels": int(maps.size // 3), "game_sky_maps_ms": game_ms, "pow_table_setup_ms": info["setup_ms"],
+           "sky_cache_ms": cache_ms, "blend_ms": blend_ms}
+    if cpu:
+        from oracle import ffi
+        ffi.build()
+        t = time.perf_counter()
+        ffi.game_sky_maps(lens, light, 0.378182, colour, atlas, 2)
+        out["cpu_port_game_sky_maps_ms"] = (time.perf_counter() - t) * 1e3
+    return out
+
+
 def bench_plain(args, lw, frames):
     cfg = frames.CONFIGS[args.config]
     if args.spp:
@@ -209,8 +255,10 @@ def bench_plain(args, lw, frames):
     value = samples / elapsed / 1e6
     assert np.isfinite(img).all()
     extra = {"post": post_timing(b, frames, img),
-             "scene_build": build_timing(b, world, frame.sky, not args.no_cpu_baseline)}
+             "scene_build": build_timing(b, world, frame.sky, not args.no_cpu_baseline),
+             "sky_prep": sky_timing(b, frames, not args.no_cpu_baseline)}
     log(f"post-processing: {extra['post']}")
+    log(f"sky preparation: {extra['sky_prep']}")
     log(f"scene build: {extra['scene_build']}")
     if args.copy_probe:
         ms = b.copy_probe(args.copy_probe)

@@ -460,6 +460,30 @@ int lw_hip_sky_cache(lw_hip_backend *b, const lw_params *params, float *out, siz
 int lw_hip_blend_chunky_sky(lw_hip_backend *b, const lw_params *params, float weight, float *maps,
                             size_t maps_len);
 
+/* GameSkyMaps (sky.rs:885-905) without its buffer, and the call's device time. */
+typedef struct lw_game_sky {
+  float sizes[4]; /* Params::sky_map: the lens's map, then the light's */
+  float light[3]; /* the lightmap's colour decoded */
+  float mean[3];  /* the cosine-weighted mean of the light's sky over the upper hemisphere */
+  float scale;
+  float veil[3];
+  double kernel_ms; /* HIP-event time of the drawing and veil kernels */
+  double setup_ms;  /* the one-time libm power table made by this call, 0 if it existed */
+} lw_game_sky;
+
+/* game_sky_maps (sky.rs:906-953) on the device: the client's sky (QuickSky::background,
+ * sky.rs:509-532) drawn into the lens's 2048 x 1024 map from `lens` and the light's 360 x 180
+ * map from `light` (lw_quick_sky_new), decoded to linear, the light's scaled so that a face
+ * open to the whole sky takes the lightmap's colour. out receives the two maps (the lens's,
+ * then the light's, linear RGB, (2048 * 1024 + 360 * 180) * 3 floats), info the rest. atlas
+ * holds the sprites the sun and moon rects name (atlas_w texels a row); stars is
+ * lw_star_table's. */
+int lw_hip_game_sky_maps(lw_hip_backend *b, const struct lw_quick_sky *lens,
+                         const struct lw_quick_sky *light, float sky_light_factor,
+                         const float sky_light_color[3], const uint32_t *atlas, size_t atlas_len,
+                         uint32_t atlas_w, const uint32_t *stars, size_t stars_len, float *out,
+                         size_t cap, lw_game_sky *info);
+
 /* ---- autotuning (no counterpart in the reference) ---- */
 
 #define LW_FORM_PERSISTENT 0u

@@ -177,6 +177,73 @@ void lw_world_free(lw_world *w);
  * and sinf; texel (i, j) looks along (el_cos * az_cos, el_sin, el_cos * az_sin). */
 void lw_map_trig(uint32_t w, uint32_t h, float *az, float *el);
 
+/* ---- the game's sky (render/sky.rs): the client's quick-mode sky drawn into maps ---- */
+
+#define LW_LENS_MAP_W 2048u /* sky.rs:776 */
+#define LW_LENS_MAP_H 1024u
+#define LW_LIGHT_MAP_W 360u /* sky.rs:781 */
+#define LW_LIGHT_MAP_H 180u
+#define LW_MAX_SKY_LIGHT 50.0f   /* sky.rs:786 */
+#define LW_DEFAULT_BRIGHTNESS 0.5f /* sky.rs:771 */
+#define LW_STAR_BINS_AZ 128u     /* sky.rs:67-70 */
+#define LW_STAR_BINS_EL 64u
+#define LW_STAR_WORDS 9u
+
+/* SkyDescription (sky.rs:225-253). */
+typedef struct lw_sky_description {
+  float sky[3];          /* visual/sky_color, sRGB 0..1 */
+  float fog[3];          /* the fog colour, sRGB 0..1 */
+  float sky_fog_end;
+  float render_fog[2];   /* start, end */
+  float env_fog[2];
+  uint32_t sky_disc;
+  float sunrise[4];      /* sRGB and alpha */
+  float sun_angle;       /* degrees */
+  uint32_t has_sun;
+  float sun_dir[3];
+  float sun_rect[4];     /* atlas x, y, w, h in texels */
+  uint32_t has_moon;
+  float moon_dir[3];
+  float moon_rect[4];
+  float star_angle;      /* degrees */
+  float star_brightness;
+} lw_sky_description;
+
+/* QuickSky (sky.rs:256-283), the same layout. */
+typedef struct lw_quick_sky {
+  float sky_color[4];
+  float fog_color[4];
+  float fog_dist[4];
+  float sunrise[4];
+  float flags[4];
+  float sun[4][4];
+  float moon[4][4];
+  float star_rot[3][4];
+  float fan[18][4];
+  float cam_forward[4];
+  float cam_right[4];
+  float cam_up[4];
+  float cam_misc[4];
+} lw_quick_sky;
+
+/* QuickSky::new (sky.rs:320-373) for a camera and an output size. */
+void lw_quick_sky_new(const lw_camera_rays *cam, uint32_t width, uint32_t height,
+                      const lw_sky_description *d, lw_quick_sky *out);
+
+/* celestial_direction (sky.rs:64-66): the unit vector towards a body at a celestial angle. */
+void lw_celestial_direction(float angle_deg, float out[3]);
+
+/* star_table (sky.rs:152-222): the client's stars binned by direction; returns the word
+ * count and writes them to out when cap holds them (out may be null). */
+long lw_star_table(uint32_t *out, size_t cap);
+
+/* The sine and cosine (libm) of each sky-disc sector's edges: th0 = -pi + k pi/4 and
+ * th0 + pi/4 for k = 0..7 (QuickSky::disc, sky.rs:408-410). */
+void lw_disc_sector_trig(float s0[8], float c0[8], float s1[8], float c1[8]);
+
+/* lightmap_sky (sky.rs:788-815): the client's lightmap colour for a cell open to the sky. */
+void lw_lightmap_sky(float factor, const float colour[3], float brightness, float out[3]);
+
 /* A procedural dusk sky as the game look's two equirectangular maps (the lens's, then the
  * light's; layout of integrator.rs:190-243): writes lw*lh*3 + gw*gh*3 floats. */
 long lw_dusk_sky_maps(const float sun_dir[3], uint32_t lw, uint32_t lh, uint32_t gw, uint32_t gh,

@@ -53,6 +53,7 @@ SIGNATURES = {
     "lw_hip_encode": (i32, [vp, vp, sz, vp, vp]),
     "lw_hip_sky_cache": (i32, [vp, vp, vp, sz, ctypes.POINTER(sz)]),
     "lw_hip_blend_chunky_sky": (i32, [vp, vp, f32, vp, sz]),
+    "lw_hip_game_sky_maps": (i32, [vp, vp, vp, f32, vp, vp, sz, u32, vp, sz, vp, sz, vp]),
     "lw_hip_downsample": (i32, [vp, vp, u32, u32, u32, vp]),
     "lw_hip_render_hq_rgba8": (i32, [vp, vp, vp, u32, f32, u32, vp]),
     # host.h
@@ -66,6 +67,11 @@ SIGNATURES = {
     "lw_sky_cache": (ctypes.c_long, [vp, vp, sz]),
     "lw_gamma_lut": (None, [vp]),
     "lw_map_trig": (None, [u32, u32, vp, vp]),
+    "lw_quick_sky_new": (None, [vp, u32, u32, vp, vp]),
+    "lw_celestial_direction": (None, [f32, vp]),
+    "lw_star_table": (ctypes.c_long, [vp, sz]),
+    "lw_disc_sector_trig": (None, [vp, vp, vp, vp]),
+    "lw_lightmap_sky": (None, [f32, vp, f32, vp]),
     "lw_expf_tabulated": (None, [vp, sz, vp]),
     "lw_camera_basis": (None, [f32, f32, vp, vp, vp]),
     "lw_camera_rays_new": (None, [f32, f32, f32, u32, u32, vp]),

@@ -173,6 +173,13 @@ class VoxelSceneC(ctypes.Structure):
                 ("sky", ctypes.c_void_p), ("sky_len", ctypes.c_size_t)]
 
 
+class GameSkyC(ctypes.Structure):
+    """lw_game_sky: GameSkyMaps (sky.rs:885-905) without its buffer."""
+    _fields_ = [("sizes", ctypes.c_float * 4), ("light", ctypes.c_float * 3),
+                ("mean", ctypes.c_float * 3), ("scale", ctypes.c_float), ("veil", ctypes.c_float * 3),
+                ("kernel_ms", ctypes.c_double), ("setup_ms", ctypes.c_double)]
+
+
 class BuildStatsC(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in (
         "bricks", "uniform_bricks", "buried_bricks", "instances", "overlay_bricks", "entity_quads",

@@ -16,7 +16,7 @@ import numpy as np
 from . import _native
 from .abi import (EMITTER, FORMS, HIT, MODEL, PARAMS, QUAD, SCENE_EMITTER_GRID, SCENE_EMITTERS,
                   SCENE_GRID, SCENE_QUADS, VIEW, BoundView, BuildStatsC, FitOptionsC, FitStatsC,
-                  HqFrame, HqScene, TimingC, TuneResultC, VoxelVolume)
+                  GameSkyC, HqFrame, HqScene, TimingC, TuneResultC, VoxelVolume)
 from .frames import SceneInfoC
 
 LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
@@ -292,6 +292,22 @@ class HipBackend:
         self._check(_native.lib().lw_hip_blend_chunky_sky(self.handle, _addr(p), weight, _addr(out),
                                                           out.size))
         return out
+
+    def game_sky_maps(self, lens, light, sky_light_factor: float, sky_light_color, atlas,
+                      atlas_w: int, stars: np.ndarray):
+        """lw_hip_game_sky_maps (sky.rs:906-953): (the two maps as one float32 buffer, the
+        lens's then the light's, and a dict of sizes, light, mean, scale, veil and times)."""
+        at = np.ascontiguousarray(atlas, dtype="<u4")
+        st = np.ascontiguousarray(stars, dtype="<u4")
+        col = np.ascontiguousarray(sky_light_color, dtype="<f4")
+        out = np.zeros((2048 * 1024 + 360 * 180) * 3, dtype="<f4")
+        info = GameSkyC()
+        self._check(_native.lib().lw_hip_game_sky_maps(
+            self.handle, ctypes.byref(lens), ctypes.byref(light), sky_light_factor, _addr(col),
+            _addr(at), at.size, atlas_w, _addr(st), st.size, _addr(out), out.size, ctypes.byref(info)))
+        return out, {"sizes": list(info.sizes), "light": list(info.light), "mean": list(info.mean),
+                     "scale": info.scale, "veil": list(info.veil), "kernel_ms": info.kernel_ms,
+                     "setup_ms": info.setup_ms}
 
     def downsample(self, px: np.ndarray, w: int, h: int, k: int) -> np.ndarray:
         """lw_hip_downsample (render/post.rs:218-247)."""

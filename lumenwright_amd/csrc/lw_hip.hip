@@ -30,6 +30,7 @@
 #include "lw_diff.hpp"
 #include "lw_build.hpp"
 #include "lw_sky.hpp"
+#include "lw_quick_sky.hpp"
 
 using namespace lw;
 
@@ -472,19 +473,24 @@ __global__ void __launch_bounds__(kBlock) k_tonemap(const float *__restrict__ hd
   out[i] = px;
 }
 
-// The display encoding as the mix needs the value itself (post.rs:141-144): libm's
-// powf(x, 1/2.2f), which stays within one ulp but is not always correctly rounded. Here the
-// power is taken in f64 (error near 2^-52) and rounded once to f32. Over every f32 in [0, 1],
-// libm's result differs from the correctly rounded one only where the exact power lies within
-// 2^-33.2 (relative) of an f32 rounding midpoint, so an input whose f64 power lies within
-// kPowBand = 2^-32 of one is "in the band". For those, and only those, the result comes from
+// libm's powf(x, y) for x in [0, 1] and the two exponents the post and sky code raise to:
+// 1/2.2f (the display encoding, post.rs:141-144) and 2.2f (the texel decoding, sky.rs:872).
+// powf stays within one ulp but is not always correctly rounded. Here the power is taken in
+// f64 (error near 2^-52) and rounded once to f32. Over every f32 in [0, 1], libm's result
+// differs from the correctly rounded one only where the exact power lies within 2^-33.2
+// (exponent 1/2.2) or 2^-32.9 (exponent 2.2), relative, of an f32 rounding midpoint, so an
+// input whose f64 power lies within kPowBand = 2^-32 of one is "in the band". For those, and only those, the result comes from
 // a table the host makes once with libm itself (ensure_pow_fix): the sorted pairs (input bits,
 // libm result bits) of the band inputs where libm and the rounded f64 power differ.
 constexpr double kPowBand = 0x1p-32;
+constexpr float kPowExponent[2] = {1.0f / 2.2f, 2.2f};
+enum { POW_ENCODE = 0, POW_DECODE = 1 };
 
-// The rounded f64 power of x in [0, 1] (or NaN) and whether x is in the band.
-__device__ __forceinline__ float pow_display(float x, bool &in_band) {
-  const double p = pow((double)x, (double)(1.0f / 2.2f));
+// The rounded f64 power of x in [0, 1] (or NaN) and whether x is in the band. A power that
+// rounds to 0 is in the band when it lies near 2^-150, the midpoint below the least
+// subnormal.
+__device__ __forceinline__ float pow_band(float x, float y, bool &in_band) {
+  const double p = pow((double)x, (double)y);
   const float c = (float)p;
   in_band = false;
   if (c > 0.0f && c <= 1.0f) {
@@ -492,14 +498,15 @@ __device__ __forceinline__ float pow_display(float x, bool &in_band) {
     const float nb = __uint_as_float((double)c <= p ? cb + 1u : cb - 1u);
     const double mid = ((double)c + (double)nb) * 0.5;  // exact: adjacent f32 values
     in_band = fabs(p - mid) < kPowBand * p;
+  } else if (c == 0.0f && p > 0.0) {
+    in_band = fabs(p - 0x1p-150) < kPowBand * p;
   }
   return c;
 }
 
-__device__ __forceinline__ float encode_f(float v, const uint2 *__restrict__ fix, uint32_t nfix) {
-  const float x = clamp_std(v, 0.0f, 1.0f);
+__device__ __forceinline__ float pow_libm(float x, float y, const uint2 *__restrict__ fix, uint32_t nfix) {
   bool in_band;
-  const float c = pow_display(x, in_band);
+  const float c = pow_band(x, y, in_band);
   if (!in_band) return c;
   const uint32_t u = __float_as_uint(x);
   uint32_t lo = 0, hi = nfix;
@@ -510,9 +517,13 @@ __device__ __forceinline__ float encode_f(float v, const uint2 *__restrict__ fix
   return (lo < nfix && fix[lo].x == u) ? __uint_as_float(fix[lo].y) : c;
 }
 
+__device__ __forceinline__ float encode_f(float v, const uint2 *__restrict__ fix, uint32_t nfix) {
+  return pow_libm(clamp_std(v, 0.0f, 1.0f), kPowExponent[POW_ENCODE], fix, nfix);
+}
+
 // The band inputs among the f32 bit patterns [0, n) with their rounded f64 powers, appended
 // to out (one atomic per wave; entries past cap are counted, not written).
-__global__ void __launch_bounds__(kBlock) k_pow_band(uint32_t n, uint2 *__restrict__ out,
+__global__ void __launch_bounds__(kBlock) k_pow_band(uint32_t n, float y, uint2 *__restrict__ out,
                                                      uint32_t *__restrict__ count, uint32_t cap) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -522,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_pow_band(uint32_t n, uint2 *__restri
     const uint32_t u = w0 + lane;
     bool in_band = false;
     float c = 0.0f;
-    if (u < n) c = pow_display(__uint_as_float(u), in_band);
+    if (u < n) c = pow_band(__uint_as_float(u), y, in_band);
     const uint64_t m = __ballot(in_band);
     if (m == 0) continue;
     uint32_t base = 0;
@@ -559,6 +570,38 @@ __global__ void __launch_bounds__(kBlock) k_encode(const float *__restrict__ x, 
                                                    float *__restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = encode_f(x[i], fix, nfix);
+}
+
+// draw_map (sky.rs:864-881) for the game's maps, one lane per texel: the client's sky along
+// the texel's direction, decoded to linear as c^2.2 with libm's powf (pow_libm).
+__global__ void __launch_bounds__(kBlock) k_quick_sky_map(float *__restrict__ out, uint32_t w, uint32_t h,
+                                                          const float2 *__restrict__ az,
+                                                          const float2 *__restrict__ el,
+                                                          const lw_quick_sky q, const lw::DiscSectors sec,
+                                                          const uint32_t *__restrict__ atlas, uint32_t atlas_w,
+                                                          uint32_t atlas_len, const uint32_t *__restrict__ stars,
+                                                          const uint2 *__restrict__ fix, uint32_t nfix) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= w * h) return;
+  const float2 a = az[k % w], e = el[k / w];
+  const V3 d = v3(e.x * a.x, e.y, e.x * a.y);
+  const V3 c = lw::qs_background(q, sec, d, atlas, atlas_w, atlas_len, stars);
+  const float y = kPowExponent[POW_DECODE];
+  float *t = out + (size_t)k * 3;
+  t[0] = pow_libm(c.x, y, fix, nfix);
+  t[1] = pow_libm(c.y, y, fix, nfix);
+  t[2] = pow_libm(c.z, y, fix, nfix);
+}
+
+// The light's map scaled with its veil (sky.rs:950): (c + veil) * scale per texel.
+__global__ void __launch_bounds__(kBlock) k_sky_veil(float *__restrict__ map, uint32_t n, float3 veil,
+                                                     float scale) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float *t = map + (size_t)k * 3;
+  t[0] = (t[0] + veil.x) * scale;
+  t[1] = (t[1] + veil.y) * scale;
+  t[2] = (t[2] + veil.z) * scale;
 }
 
 // downsample (post.rs:218-247), one lane per output pixel.
@@ -1120,12 +1163,14 @@ struct lw_hip_backend {
   DevBuf<BatchDev> batch_dev;
   DevBuf<float> tone_in;
   DevBuf<uint32_t> tone_out;
-  // The display encoding's corrections to the rounded f64 power (see encode_f), made on the
+  // libm's corrections to the rounded f64 power per exponent (see pow_libm), made on the
   // first call that needs them.
-  DevBuf<uint2> pow_fix;
-  uint32_t pow_fix_n = 0;
-  bool pow_fix_ready = false;
-  double pow_fix_ms = 0.0;
+  struct PowFix {
+    DevBuf<uint2> fix;
+    uint32_t n = 0;
+    bool ready = false;
+    double ms = 0.0;
+  } pow_fix[2];
   BatchPlan plan;  // the last batch's layout (its sums stay in acc when it keeps them)
   // Differentiable rendering: the parameters and derivative inputs of the last call.
   DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
@@ -2023,12 +2068,14 @@ ToneDev make_tone(float exposure, uint32_t op_a, uint32_t op_b, float weight) {
   return t;
 }
 
-// The table encode_f reads: every f32 in [0, 1] whose rounded f64 power lies in the band
-// (k_pow_band, one pass over the 2^30 bit patterns), then libm's powf on those inputs here,
-// keeping the ones where it differs from the device's rounding.
-void ensure_pow_fix(lw_hip_backend *b) {
-  if (b->pow_fix_ready) return;
+// The table pow_libm reads for one exponent: every f32 in [0, 1] whose rounded f64 power
+// lies in the band (k_pow_band, one pass over the 2^30 bit patterns), then libm's powf on
+// those inputs here, keeping the ones where it differs from the device's rounding.
+const lw_hip_backend::PowFix &ensure_pow_fix(lw_hip_backend *b, int which) {
+  lw_hip_backend::PowFix &pf = b->pow_fix[which];
+  if (pf.ready) return pf;
   const auto t0 = std::chrono::steady_clock::now();
+  const float y = kPowExponent[which];
   const uint32_t n = 0x3f800001u;  // the bit patterns of [0, 1]
   const uint32_t cap = 1u << 24;   // about 6.0M inputs are in the band
   DevBuf<uint2> cand;
@@ -2036,12 +2083,12 @@ void ensure_pow_fix(lw_hip_backend *b) {
   cand.alloc(cap);
   count.alloc(1);
   HIP_TRY(hipMemsetAsync(count.ptr, 0, sizeof(uint32_t), b->stream));
-  k_pow_band<<<dim3(8192), dim3(kBlock), 0, b->stream>>>(n, cand.ptr, count.ptr, cap);
+  k_pow_band<<<dim3(8192), dim3(kBlock), 0, b->stream>>>(n, y, cand.ptr, count.ptr, cap);
   HIP_TRY(hipGetLastError());
   uint32_t found = 0;
   HIP_TRY(hipMemcpyAsync(&found, count.ptr, sizeof(found), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  if (found > cap) throw InvalidScene{"display-encoding band holds more inputs than expected"};
+  if (found > cap) throw InvalidScene{"libm power band holds more inputs than expected"};
   std::vector<uint2> host(found);
   if (found)
     HIP_TRY(hipMemcpy(host.data(), cand.ptr, (size_t)found * sizeof(uint2), hipMemcpyDeviceToHost));
@@ -2049,17 +2096,18 @@ void ensure_pow_fix(lw_hip_backend *b) {
   for (const uint2 &e : host) {
     float x;
     std::memcpy(&x, &e.x, 4);
-    const float g = lw::display_pow(x);
+    const float g = lw::libm_powf(x, y);
     uint32_t gb;
     std::memcpy(&gb, &g, 4);
     if (gb != e.y) fix.push_back(make_uint2(e.x, gb));
   }
   std::sort(fix.begin(), fix.end(), [](const uint2 &a, const uint2 &c) { return a.x < c.x; });
-  b->pow_fix.upload(fix.data(), fix.size(), b->stream);
+  pf.fix.upload(fix.data(), fix.size(), b->stream);
   HIP_TRY(hipStreamSynchronize(b->stream));
-  b->pow_fix_n = (uint32_t)fix.size();
-  b->pow_fix_ready = true;
-  b->pow_fix_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  pf.n = (uint32_t)fix.size();
+  pf.ready = true;
+  pf.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return pf;
 }
 
 // Tone-maps n pixels already on the device (in) into b->tone_out and copies them to out.
@@ -2071,8 +2119,8 @@ void tonemap_device(lw_hip_backend *b, const float *in, size_t n, const ToneDev 
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
   if (n) {
     if (mix)
-      k_tonemap_mix<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->pow_fix.ptr,
-                                                          b->pow_fix_n, b->tone_out.ptr);
+      k_tonemap_mix<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->pow_fix[POW_ENCODE].fix.ptr,
+                                                          b->pow_fix[POW_ENCODE].n, b->tone_out.ptr);
     else
       k_tonemap<<<grid, dim3(kBlock), 0, b->stream>>>(in, (uint32_t)n, t, b->tone_out.ptr);
     HIP_TRY(hipGetLastError());
@@ -2576,7 +2624,7 @@ int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float expo
     const bool ends = weight >= 1.0f || weight <= 0.0f;
     const uint32_t op = weight >= 1.0f ? op_a : op_b;
     const ToneDev t = ends ? make_tone(exposure, op, op, 1.0f) : make_tone(exposure, op_a, op_b, weight);
-    if (!ends) ensure_pow_fix(b);
+    if (!ends) ensure_pow_fix(b, POW_ENCODE);
     b->tone_in.upload(hdr, n * 3, b->stream);
     tonemap_device(b, b->tone_in.ptr, n, t, !ends, out);
   });
@@ -2659,19 +2707,121 @@ int lw_hip_blend_chunky_sky(lw_hip_backend *b, const lw_params *params, float we
   });
 }
 
+int lw_hip_game_sky_maps(lw_hip_backend *b, const lw_quick_sky *lens, const lw_quick_sky *light,
+                         float sky_light_factor, const float sky_light_color[3], const uint32_t *atlas,
+                         size_t atlas_len, uint32_t atlas_w, const uint32_t *stars, size_t stars_len,
+                         float *out, size_t cap, lw_game_sky *info) {
+  if (!b || !lens || !light || !sky_light_color || !atlas || !stars || !out || !info) return LW_EINVAL;
+  return guarded(b, [&] {
+    const uint32_t lw_ = LW_LENS_MAP_W, lh = LW_LENS_MAP_H, gw = LW_LIGHT_MAP_W, gh = LW_LIGHT_MAP_H;
+    const size_t lens_n = (size_t)lw_ * lh, light_n = (size_t)gw * gh, total = (lens_n + light_n) * 3;
+    if (cap < total) throw InvalidScene{"destination smaller than the two maps"};
+    if (atlas_len == 0 || atlas_w == 0 || atlas_len > 0xFFFFFFFFull) throw InvalidScene{"empty atlas"};
+    // the star table: offsets per bin that stay inside its entries
+    const size_t nbins = (size_t)LW_STAR_BINS_AZ * LW_STAR_BINS_EL;
+    if (stars_len < nbins + 1) throw InvalidScene{"star table shorter than its bins"};
+    for (size_t k = 0; k < nbins; k++)
+      if (stars[k] > stars[k + 1]) throw InvalidScene{"star table bins out of order"};
+    if (nbins + 1 + (size_t)stars[nbins] * LW_STAR_WORDS > stars_len) throw InvalidScene{"star table entries past its end"};
+    HIP_TRY(hipSetDevice(b->device));
+    const double setup0 = b->pow_fix[POW_DECODE].ready ? 0.0 : -1.0;
+    const auto &pf = ensure_pow_fix(b, POW_DECODE);
+    std::vector<float> tab(2 * ((size_t)lw_ + lh + gw + gh));
+    lw_map_trig(lw_, lh, tab.data(), tab.data() + 2 * lw_);
+    float *t2 = tab.data() + 2 * ((size_t)lw_ + lh);
+    lw_map_trig(gw, gh, t2, t2 + 2 * gw);
+    lw::DiscSectors sec;
+    lw_disc_sector_trig(sec.s0, sec.c0, sec.s1, sec.c1);
+    DevBuf<float> dmaps, dtab;
+    DevBuf<uint32_t> datlas, dstars;
+    dmaps.alloc(total);
+    dtab.upload(tab.data(), tab.size(), b->stream);
+    datlas.upload(atlas, atlas_len, b->stream);
+    dstars.upload(stars, stars_len, b->stream);
+    const float2 *ft = reinterpret_cast<const float2 *>(dtab.ptr);
+    float *light_map = dmaps.ptr + lens_n * 3;
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    k_quick_sky_map<<<dim3((unsigned)((lens_n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        dmaps.ptr, lw_, lh, ft, ft + lw_, *lens, sec, datlas.ptr, atlas_w, (uint32_t)atlas_len, dstars.ptr,
+        pf.fix.ptr, pf.n);
+    HIP_TRY(hipGetLastError());
+    k_quick_sky_map<<<dim3((unsigned)((light_n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        light_map, gw, gh, ft + lw_ + lh, ft + lw_ + lh + gw, *light, sec, datlas.ptr, atlas_w,
+        (uint32_t)atlas_len, dstars.ptr, pf.fix.ptr, pf.n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    // The light's cosine-weighted mean over the upper hemisphere, summed in f64 on the host in
+    // the reference's order (sky.rs:922-931), then the scale and the veil (sky.rs:932-942).
+    std::vector<float> game(light_n * 3);
+    HIP_TRY(hipMemcpyAsync(game.data(), light_map, light_n * 3 * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    double acc[3] = {0.0, 0.0, 0.0}, weight = 0.0;
+    for (size_t k = 0; k < light_n; k++) {
+      const float *e = t2 + 2 * gw + 2 * (k / gw);
+      const float dy = e[1];
+      if (dy > 0.0f) {
+        const float r = 1.0f - dy * dy;
+        const double w = (double)(dy * std::sqrt(r > 0.0f ? r : 0.0f));
+        for (int c = 0; c < 3; c++) acc[c] += (double)game[k * 3 + c] * w;
+        weight += w;
+      }
+    }
+    float mean[3], target[3];
+    for (int c = 0; c < 3; c++) mean[c] = (float)(acc[c] / weight);
+    lw_lightmap_sky(sky_light_factor, sky_light_color, LW_DEFAULT_BRIGHTNESS, target);
+    for (int c = 0; c < 3; c++) target[c] = lw::libm_powf(target[c], kPowExponent[POW_DECODE]);
+    float scale = LW_MAX_SKY_LIGHT;
+    for (int c = 0; c < 3; c++)
+      if (mean[c] > 0.0f) scale = std::min(scale, target[c] / mean[c]);
+    float veil[3];
+    for (int c = 0; c < 3; c++) {
+      const float v = target[c] / scale - mean[c];
+      veil[c] = std::isnan(v) ? 0.0f : (v > 0.0f ? v : 0.0f);
+    }
+    HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    k_sky_veil<<<dim3((unsigned)((light_n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        light_map, (uint32_t)light_n, make_float3(veil[0], veil[1], veil[2]), scale);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+    HIP_TRY(hipMemcpyAsync(out, dmaps.ptr, total * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms_draw = 0.0f, ms_veil = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms_draw, b->ev[0], b->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_veil, b->ev[2], b->ev[3]));
+    std::memset(info, 0, sizeof(*info));
+    info->sizes[0] = (float)lw_;
+    info->sizes[1] = (float)lh;
+    info->sizes[2] = (float)gw;
+    info->sizes[3] = (float)gh;
+    for (int c = 0; c < 3; c++) {
+      info->light[c] = target[c];
+      info->mean[c] = mean[c];
+      info->veil[c] = veil[c];
+    }
+    info->scale = scale;
+    info->kernel_ms = (double)ms_draw + (double)ms_veil;
+    info->setup_ms = setup0 < 0.0 ? pf.ms : 0.0;
+    lw_timing tm{};
+    tm.trace_ms = info->kernel_ms;
+    tm.launches = 3;
+    tm.samples = lens_n + light_n;
+    b->timing = tm;
+  });
+}
+
 int lw_hip_encode(lw_hip_backend *b, const float *x, size_t n, float *out, double *setup_ms) {
   if (!b || (n && (!x || !out))) return LW_EINVAL;
   return guarded(b, [&] {
     HIP_TRY(hipSetDevice(b->device));
     if (n > 0xFFFFFFFFull) throw InvalidScene{"too many values"};
-    ensure_pow_fix(b);
-    if (setup_ms) *setup_ms = b->pow_fix_ms;
+    const auto &pf = ensure_pow_fix(b, POW_ENCODE);
+    if (setup_ms) *setup_ms = pf.ms;
     DevBuf<float> in, res;
     in.upload(x, n, b->stream);
     res.alloc(std::max<size_t>(n, 1));
     if (n) {
       k_encode<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
-          in.ptr, (uint32_t)n, b->pow_fix.ptr, b->pow_fix_n, res.ptr);
+          in.ptr, (uint32_t)n, pf.fix.ptr, pf.n, res.ptr);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(out, res.ptr, n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     }

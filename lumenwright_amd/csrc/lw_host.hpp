@@ -9,4 +9,6 @@ void gamma_lut(float out[256]);
 // The display encoding's power as the reference's CPU build computes it: libm's powf(x, 1/2.2f)
 // (post.rs:141-144, Rust's f32::powf).
 float display_pow(float x);
+// libm's powf (Rust's f32::powf).
+float libm_powf(float x, float y);
 }  // namespace lw

@@ -58,6 +58,29 @@ class CameraRaysC(ctypes.Structure):
                 ("aspect", ctypes.c_float)]
 
 
+class SkyDescriptionC(ctypes.Structure):
+    """SkyDescription (sky.rs:225-253), lw_sky_description."""
+    _fields_ = [("sky", ctypes.c_float * 3), ("fog", ctypes.c_float * 3),
+                ("sky_fog_end", ctypes.c_float), ("render_fog", ctypes.c_float * 2),
+                ("env_fog", ctypes.c_float * 2), ("sky_disc", ctypes.c_uint32),
+                ("sunrise", ctypes.c_float * 4), ("sun_angle", ctypes.c_float),
+                ("has_sun", ctypes.c_uint32), ("sun_dir", ctypes.c_float * 3),
+                ("sun_rect", ctypes.c_float * 4), ("has_moon", ctypes.c_uint32),
+                ("moon_dir", ctypes.c_float * 3), ("moon_rect", ctypes.c_float * 4),
+                ("star_angle", ctypes.c_float), ("star_brightness", ctypes.c_float)]
+
+
+class QuickSkyC(ctypes.Structure):
+    """QuickSky (sky.rs:256-283), lw_quick_sky."""
+    _fields_ = [("sky_color", ctypes.c_float * 4), ("fog_color", ctypes.c_float * 4),
+                ("fog_dist", ctypes.c_float * 4), ("sunrise", ctypes.c_float * 4),
+                ("flags", ctypes.c_float * 4), ("sun", (ctypes.c_float * 4) * 4),
+                ("moon", (ctypes.c_float * 4) * 4), ("star_rot", (ctypes.c_float * 4) * 3),
+                ("fan", (ctypes.c_float * 4) * 18), ("cam_forward", ctypes.c_float * 4),
+                ("cam_right", ctypes.c_float * 4), ("cam_up", ctypes.c_float * 4),
+                ("cam_misc", ctypes.c_float * 4)]
+
+
 class WorldSpecC(ctypes.Structure):
     _fields_ = [("chunks_x", ctypes.c_uint32), ("chunks_z", ctypes.c_uint32),
                 ("sections", ctypes.c_uint32), ("seed", ctypes.c_uint64),
@@ -137,6 +160,60 @@ def sky_cache(params: np.ndarray) -> np.ndarray:
                                    out.size)
     assert r == out.size
     return out
+
+
+def celestial_direction(angle_deg: float) -> np.ndarray:
+    out = f3()
+    _native.lib().lw_celestial_direction(angle_deg, out)
+    return np.array(out, dtype="<f4")
+
+
+def camera_rays(yaw: float, pitch: float, fov_y: float, width: int, height: int) -> "CameraRaysC":
+    rays = CameraRaysC()
+    _native.lib().lw_camera_rays_new(yaw, pitch, fov_y, width, height, ctypes.byref(rays))
+    return rays
+
+
+def sky_description(sky, fog, sky_fog_end, render_fog, env_fog, sky_disc, sunrise, sun_angle,
+                    sun=None, moon=None, star_angle=0.0, star_brightness=0.0) -> SkyDescriptionC:
+    """A SkyDescription; `sun` and `moon` are (direction, atlas rect) or None."""
+    d = SkyDescriptionC()
+    d.sky[:], d.fog[:] = list(sky), list(fog)
+    d.sky_fog_end = sky_fog_end
+    d.render_fog[:], d.env_fog[:] = list(render_fog), list(env_fog)
+    d.sky_disc = int(sky_disc)
+    d.sunrise[:] = list(sunrise)
+    d.sun_angle = sun_angle
+    if sun is not None:
+        d.has_sun = 1
+        d.sun_dir[:], d.sun_rect[:] = [float(x) for x in sun[0]], list(sun[1])
+    if moon is not None:
+        d.has_moon = 1
+        d.moon_dir[:], d.moon_rect[:] = [float(x) for x in moon[0]], list(moon[1])
+    d.star_angle, d.star_brightness = star_angle, star_brightness
+    return d
+
+
+def quick_sky_new(cam: "CameraRaysC", width: int, height: int, desc: SkyDescriptionC) -> QuickSkyC:
+    """QuickSky::new (sky.rs:320-373)."""
+    q = QuickSkyC()
+    _native.lib().lw_quick_sky_new(ctypes.byref(cam), width, height, ctypes.byref(desc), ctypes.byref(q))
+    return q
+
+
+def star_table() -> np.ndarray:
+    """star_table (sky.rs:152-222) as uint32 words."""
+    L = _native.lib()
+    n = L.lw_star_table(None, 0)
+    out = np.zeros(n, dtype="<u4")
+    L.lw_star_table(ctypes.c_void_p(out.ctypes.data), n)
+    return out
+
+
+def lightmap_sky(factor: float, colour, brightness: float = 0.5) -> np.ndarray:
+    out = f3()
+    _native.lib().lw_lightmap_sky(factor, _v3(colour), brightness, out)
+    return np.array(out, dtype="<f4")
 
 
 def map_trig(w: int, h: int):

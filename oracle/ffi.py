@@ -93,8 +93,24 @@ def lib():
         L.lwo_pfm.restype = sz
         L.lwo_check_encode_thresholds.argtypes = [vp, ctypes.c_uint, ctypes.POINTER(u32)]
         L.lwo_check_encode_thresholds.restype = ctypes.c_uint64
-        L.lwo_powf_band.argtypes = [ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
+        L.lwo_powf_band.argtypes = [ctypes.c_float, ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
         L.lwo_powf_band.restype = None
+        L.lwo_java_random.argtypes = [ctypes.c_int64, i32, sz, vp]
+        L.lwo_java_random.restype = None
+        L.lwo_stars.argtypes = [vp, sz]
+        L.lwo_stars.restype = ctypes.c_long
+        L.lwo_star_table.argtypes = [vp, sz]
+        L.lwo_star_table.restype = ctypes.c_long
+        L.lwo_quick_sky_new.argtypes = [vp, u32, u32, vp, vp]
+        L.lwo_quick_sky_new.restype = None
+        L.lwo_celestial_direction.argtypes = [ctypes.c_float, vp]
+        L.lwo_celestial_direction.restype = None
+        L.lwo_lightmap_sky.argtypes = [ctypes.c_float, vp, ctypes.c_float, vp]
+        L.lwo_lightmap_sky.restype = None
+        L.lwo_quick_background.argtypes = [vp, vp, sz, vp, u32, vp]
+        L.lwo_quick_background.restype = None
+        L.lwo_game_sky_maps.argtypes = [vp, vp, ctypes.c_float, vp, vp, u32, vp, vp]
+        L.lwo_game_sky_maps.restype = None
         L.lwo_blend_chunky_sky.argtypes = [vp, ctypes.c_float, vp, vp]
         L.lwo_blend_chunky_sky.restype = i32
         L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
@@ -427,14 +443,16 @@ def check_encode_thresholds(thr, threads: int = 0):
     return n, first.value
 
 
-def powf_band(threads: int = 0, inputs: bool = False):
-    """libm's powf(x, 1/2.2f) against the rounded f64 power over every f32 in [0, 1]:
+def powf_band(threads: int = 0, inputs: bool = False, exponent: float = 1.0 / 2.2):
+    """libm's powf(x, y) (y the f32 of `exponent`, 1/2.2 by default) against the rounded f64
+    power over every f32 in [0, 1]:
     (deviations, deviations outside the 2^-32 band, inputs in the band, deviations that are not
     neighbours, largest relative midpoint distance of a deviation[, the band inputs])."""
     st = np.zeros(4, dtype=np.uint64)
     mr = ctypes.c_double()
     band = np.zeros(1 << 24, dtype="<u4") if inputs else None
-    lib().lwo_powf_band(threads or os.cpu_count() or 1, _addr(st),
+    y = float(np.float32(1.0) / np.float32(2.2)) if exponent == 1.0 / 2.2 else float(np.float32(exponent))
+    lib().lwo_powf_band(y, threads or os.cpu_count() or 1, _addr(st),
                         ctypes.byref(mr), _addr(band) if inputs else None,
                         band.size if inputs else 0)
     out = (int(st[0]), int(st[1]), int(st[2]), int(st[3]), mr.value)
@@ -476,3 +494,66 @@ def blend_chunky_sky(view, weight: float, sizes, maps):
     sz = np.ascontiguousarray(sizes, dtype="<f4")
     _check(lib().lwo_blend_chunky_sky(view.ref, weight, _addr(sz), _addr(out)))
     return out
+
+
+# ---- the game's sky (render/sky.rs) ----
+
+def java_random(seed: int, bits: int, n: int):
+    out = np.zeros(n, dtype=np.int32)
+    lib().lwo_java_random(seed, bits, n, _addr(out))
+    return out
+
+
+def stars():
+    n = lib().lwo_stars(None, 0)
+    out = np.zeros((n, 9), dtype="<f4")
+    lib().lwo_stars(_addr(out), out.size)
+    return out
+
+
+def star_table():
+    n = lib().lwo_star_table(None, 0)
+    out = np.zeros(n, dtype="<u4")
+    lib().lwo_star_table(_addr(out), n)
+    return out
+
+
+def quick_sky_new(cam, w: int, h: int, desc):
+    """QuickSky::new; cam is a CameraRaysC, desc a SkyDescriptionC (lumenwright_amd.frames)."""
+    from lumenwright_amd.frames import QuickSkyC
+    c = np.array(list(cam.forward) + list(cam.right) + list(cam.up) + [cam.tan_half_fov, cam.aspect],
+                 dtype="<f4")
+    q = QuickSkyC()
+    lib().lwo_quick_sky_new(_addr(c), w, h, ctypes.byref(desc), ctypes.byref(q))
+    return q
+
+
+def celestial_direction(deg: float):
+    out = np.zeros(3, dtype="<f4")
+    lib().lwo_celestial_direction(deg, _addr(out))
+    return out
+
+
+def lightmap_sky(factor: float, colour, brightness: float = 0.5):
+    out = np.zeros(3, dtype="<f4")
+    lib().lwo_lightmap_sky(factor, _addr(_f32(colour)), brightness, _addr(out))
+    return out
+
+
+def quick_background(q, d, atlas, atlas_w: int):
+    d = _f32(d).reshape(-1, 3)
+    out = np.zeros_like(d)
+    at = np.ascontiguousarray(atlas, dtype="<u4")
+    lib().lwo_quick_background(ctypes.byref(q), _addr(d), d.shape[0], _addr(at), atlas_w, _addr(out))
+    return out
+
+
+def game_sky_maps(lens, light, factor: float, colour, atlas, atlas_w: int):
+    """game_sky_maps: (the two maps, dict of light, mean, scale, veil)."""
+    out = np.zeros((2048 * 1024 + 360 * 180) * 3, dtype="<f4")
+    info = np.zeros(10, dtype="<f4")
+    at = np.ascontiguousarray(atlas, dtype="<u4")
+    lib().lwo_game_sky_maps(ctypes.byref(lens), ctypes.byref(light), factor, _addr(_f32(colour)),
+                            _addr(at), atlas_w, _addr(out), _addr(info))
+    return out, {"light": list(info[0:3]), "mean": list(info[3:6]), "scale": float(info[6]),
+                 "veil": list(info[7:10])}

@@ -8,6 +8,7 @@
 #include "lw_oracle_diff.hpp"
 #include "lw_oracle_post.hpp"
 #include "lw_oracle_build.hpp"
+#include "lw_oracle_sky.hpp"
 
 #include <algorithm>
 #include <array>
@@ -378,17 +379,21 @@ uint64_t lwo_check_encode_thresholds(const float *thr, unsigned threads, uint32_
   return total;
 }
 
-// The display encoding's power over every f32 x in [0, 1]: libm's powf(x, 1/2.2f) against the
-// f64 power p = pow(x, 1/2.2f) rounded once to f32. A deviation is an x where the two differ;
-// x is in the band when p lies within 2^-32 * p of the f32 rounding midpoint on p's side of
-// round(p). stats: [0] deviations, [1] deviations outside the band, [2] inputs in the band,
+// libm's powf(x, y) over every f32 x in [0, 1] (y = 1/2.2f, the display encoding, or 2.2f,
+// the texel decoding) against the f64 power p = pow(x, y) rounded once to f32. A deviation is
+// an x where the two differ; x is in the band when p lies within 2^-32 * p of the f32 rounding
+// midpoint on p's side of round(p) (2^-150 when p rounds to 0). stats: [0] deviations, [1] deviations outside the band, [2] inputs in the band,
 // [3] deviations that are not a neighbour of round(p); *max_rel: the largest |p - mid| / p
 // among the deviations. With `band` non-null (room for `cap`), the band inputs are written
 // there in ascending order.
-static bool band_of(float x, double &rel) {
-  const double p = std::pow((double)x, (double)(1.0f / 2.2f));
+static bool band_of(float x, float y, double &rel) {
+  const double p = std::pow((double)x, (double)y);
   const float c = (float)p;
   rel = 1.0;
+  if (c == 0.0f && p > 0.0) {
+    rel = std::fabs(p - 0x1p-150) / p;
+    return std::fabs(p - 0x1p-150) < 0x1p-32 * p;
+  }
   if (!(c > 0.0f && c <= 1.0f)) return false;
   uint32_t cb;
   std::memcpy(&cb, &c, 4);
@@ -400,7 +405,7 @@ static bool band_of(float x, double &rel) {
   return std::fabs(p - mid) < 0x1p-32 * p;
 }
 
-void lwo_powf_band(unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band,
+void lwo_powf_band(float y, unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band,
                    uint64_t cap) {
   const uint32_t one = 0x3f800000u;
   if (threads == 0) threads = 1;
@@ -414,13 +419,13 @@ void lwo_powf_band(unsigned threads, uint64_t stats[4], double *max_rel, uint32_
       float x;
       std::memcpy(&x, &u, 4);
       double rel;
-      const bool in = band_of(x, rel);
+      const bool in = band_of(x, y, rel);
       if (in) {
         if (band) found[t].push_back(u);
         st[t][2]++;
       }
-      const float g = lwo_post::encode(x);
-      const float c = (float)std::pow((double)x, (double)(1.0f / 2.2f));
+      const float g = std::pow(x, y);
+      const float c = (float)std::pow((double)x, (double)y);
       uint32_t gb, cb;
       std::memcpy(&gb, &g, 4);
       std::memcpy(&cb, &c, 4);
@@ -472,6 +477,75 @@ int lwo_blend_chunky_sky(const View *view, float weight, const float *sizes, flo
       buf += (size_t)w * h * 3;
     }
   });
+}
+
+// ---- the game's sky (render/sky.rs) ----
+
+// java.util.Random (sky.rs:72-98): n values of next(bits) from seed.
+void lwo_java_random(int64_t seed, int bits, size_t n, int32_t *out) {
+  lwo_gsky::Random r(seed);
+  for (size_t i = 0; i < n; i++) out[i] = r.next(bits);
+}
+
+// stars (sky.rs:109-137) as 9 floats each (center, a1, a2); returns the count.
+long lwo_stars(float *out, size_t cap) {
+  const auto st = lwo_gsky::stars();
+  if (out && cap >= st.size() * 9)
+    for (size_t i = 0; i < st.size(); i++) {
+      const lwo_gsky::V v[3] = {st[i].center, st[i].a1, st[i].a2};
+      for (int k = 0; k < 3; k++) {
+        out[9 * i + 3 * k] = v[k].x;
+        out[9 * i + 3 * k + 1] = v[k].y;
+        out[9 * i + 3 * k + 2] = v[k].z;
+      }
+    }
+  return (long)st.size();
+}
+
+// star_table (sky.rs:152-222); returns the word count.
+long lwo_star_table(uint32_t *out, size_t cap) {
+  static const std::vector<uint32_t> t = lwo_gsky::star_table();
+  if (out && cap >= t.size()) std::copy(t.begin(), t.end(), out);
+  return (long)t.size();
+}
+
+// QuickSky::new (sky.rs:320-373); cam = forward, right, up, tan(fov/2), aspect.
+void lwo_quick_sky_new(const float *cam, uint32_t w, uint32_t h, const lwo_gsky::Description *d,
+                       lwo_gsky::Quick *out) {
+  *out = lwo_gsky::quick_new(cam, w, h, *d);
+}
+
+void lwo_celestial_direction(float deg, float *out) {
+  const lwo_gsky::V v = lwo_gsky::celestial_direction(deg);
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+}
+
+void lwo_lightmap_sky(float factor, const float *colour, float brightness, float *out) {
+  const lwo_gsky::V v = lwo_gsky::lightmap_sky(factor, {colour[0], colour[1], colour[2]}, brightness);
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+}
+
+// QuickSky::background (sky.rs:509-532) along n directions.
+void lwo_quick_background(const lwo_gsky::Quick *q, const float *d, size_t n, const uint32_t *atlas,
+                          uint32_t atlas_w, float *out) {
+  static const std::vector<uint32_t> t = lwo_gsky::star_table();
+  for (size_t i = 0; i < n; i++) {
+    const lwo_gsky::V c = lwo_gsky::background(*q, {d[3 * i], d[3 * i + 1], d[3 * i + 2]}, atlas, atlas_w, t);
+    out[3 * i] = c.x;
+    out[3 * i + 1] = c.y;
+    out[3 * i + 2] = c.z;
+  }
+}
+
+// game_sky_maps (sky.rs:906-953): out gets both maps; info light[3], mean[3], scale, veil[3].
+void lwo_game_sky_maps(const lwo_gsky::Quick *lens, const lwo_gsky::Quick *light, float factor,
+                       const float *colour, const uint32_t *atlas, uint32_t atlas_w, float *out, float *info) {
+  static const std::vector<uint32_t> t = lwo_gsky::star_table();
+  lwo_gsky::game_sky_maps(*lens, *light, factor, {colour[0], colour[1], colour[2]}, atlas, atlas_w, t, out, info);
 }
 
 // build_hq's layout (voxel.rs:305-491, emitter_list :542-607) of an interned volume. sizes:

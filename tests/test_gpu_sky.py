@@ -68,3 +68,42 @@ def test_sky_preparation_errors(hip_backend):
     p["misc_f"][2] = -3.0
     with pytest.raises(lw.BackendError):
         hip_backend.sky_cache(p)
+
+
+def moonlit_description():
+    """A night sky: the moon drawn (flipped sprite axes), stars bright, no glow."""
+    moon = frames.celestial_direction(200.0)
+    return frames.sky_description(
+        sky=(0.02, 0.03, 0.08), fog=(0.04, 0.05, 0.09), sky_fog_end=256.0,
+        render_fog=(230.4, 256.0), env_fog=(0.0, 1024.0), sky_disc=True,
+        sunrise=(0.0, 0.0, 0.0, 0.0), sun_angle=200.0, moon=(moon, (2.0, 0.0, 2.0, 2.0)),
+        star_angle=200.0, star_brightness=0.9)
+
+
+@pytest.mark.parametrize("case", ["dusk", "night"])
+def test_the_game_sky_maps_on_the_device_are_the_oracles(hip_backend, oracle, case):
+    from test_game_sky import dusk_camera, dusk_description
+    cam = dusk_camera()
+    # 4 x 3 texels: the sprites' rects in the first two rows, a spare row and column, since a
+    # texel coordinate can round up to a rect's far edge
+    atlas = np.array([0xFFFFFFFF, 0xFF8040C0, 0x80FFFFFF, 0xFF000000,
+                      0xFFFFFFFF, 0xFF8040C0, 0x80FFFFFF, 0xFF000000,
+                      0xFF0000FF, 0xFF00FF00, 0xFFFF0000, 0xFFFFFFFF], "<u4")
+    if case == "dusk":
+        lens_d, light_d, factor = dusk_description(True), dusk_description(False), 0.378182
+    else:
+        lens_d, factor = moonlit_description(), 0.24
+        light_d = moonlit_description()
+        light_d.has_moon, light_d.star_brightness = 0, 0.0
+    lens = frames.quick_sky_new(cam, 16, 9, lens_d)
+    light = frames.quick_sky_new(cam, 16, 9, light_d)
+    colour = np.array([0x92, 0x92, 0xFF], "<f4") / np.float32(255.0)
+    got, info = hip_backend.game_sky_maps(lens, light, factor, colour, atlas, 4, frames.star_table())
+    want, winfo = oracle.game_sky_maps(lens, light, factor, colour, atlas, 4)
+    print(f"{case}: {info['kernel_ms']:.3f} ms on the device (table set-up {info['setup_ms']:.0f} ms), "
+          f"scale {info['scale']}, veil {info['veil']}")
+    for k in ("light", "mean", "veil"):
+        assert np.array_equal(np.array(info[k], "<f4"), np.array(winfo[k], "<f4")), k
+    assert np.float32(info["scale"]) == np.float32(winfo["scale"])
+    bad = np.flatnonzero(got.view("<u4") != want.view("<u4"))
+    assert bad.size == 0, (bad.size, bad[:6] // 3, got[bad[:6]], want[bad[:6]])
