@@ -185,10 +185,45 @@ const char *lw_hip_name(const lw_hip_backend *b);
 int lw_hip_render_hq(lw_hip_backend *b, const lw_scene_view *view, uint32_t spp,
                      lw_progress_fn progress, void *user, float *out_rgb);
 
-/* Backend::render_quick (render/mod.rs:96): quick mode has no HIP path; the reference's
- * CUDA backend falls back to the CPU rasterizer (render/cuda.rs:70-75). Returns
- * LW_EUNSUPPORTED so the caller can take its own rasterizer. */
-int lw_hip_render_quick(lw_hip_backend *b);
+/* Vertex (scene/mesh.rs:14-21): camera-relative position, normalised atlas UV, colour. */
+typedef struct lw_quick_vertex {
+  float pos[3];
+  float uv[2];
+  uint8_t color[4];
+} lw_quick_vertex;
+
+/* QuickFrame (render/mod.rs:53-78): the mesh (vertices and three index lists, three indices
+ * a triangle: opaque and cutout, translucent back to front, see-through), the atlas with its
+ * mip levels above 0 concatenated, the star table (lw_star_table), the output size, the
+ * camera-relative view-projection (column-major, glam's Mat4) and the sky (lw_quick_sky,
+ * include/lw/host.h). */
+typedef struct lw_quick_frame {
+  const lw_quick_vertex *vertices;
+  size_t vertices_len;
+  const uint32_t *opaque;
+  size_t opaque_len;
+  const uint32_t *translucent;
+  size_t translucent_len;
+  const uint32_t *see_through;
+  size_t see_through_len;
+  const uint32_t *atlas_pixels;
+  uint32_t atlas_size[2];
+  const uint32_t *atlas_mips;
+  size_t atlas_mips_len;
+  uint32_t mip_levels; /* at most 16 */
+  uint32_t mip_rows;
+  const uint32_t *stars;
+  size_t stars_len;
+  uint32_t width, height;
+  float view_proj[16];
+  const struct lw_quick_sky *sky;
+} lw_quick_frame;
+
+/* Backend::render_quick (render/mod.rs:96): the frame drawn on the device as the reference's
+ * CPU rasterizer draws it (render/cpu/raster.rs, render/texture.rs; the reference's CUDA
+ * backend falls back to that rasterizer, render/cuda.rs:70-75): width * height packed RGBA8
+ * words (red in the low byte), row major, top row first. */
+int lw_hip_render_quick(lw_hip_backend *b, const lw_quick_frame *frame, uint32_t *out_rgba8);
 
 /* The message of the last failure on this handle ("" if none). */
 const char *lw_hip_last_error(const lw_hip_backend *b);

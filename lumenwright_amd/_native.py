@@ -25,7 +25,7 @@ SIGNATURES = {
     "lw_hip_open": (i32, [i32, ctypes.POINTER(vp)]),
     "lw_hip_name": (ctypes.c_char_p, [vp]),
     "lw_hip_render_hq": (i32, [vp, vp, u32, PROGRESS_FN, vp, vp]),
-    "lw_hip_render_quick": (i32, [vp]),
+    "lw_hip_render_quick": (i32, [vp, vp, vp]),
     "lw_hip_last_error": (ctypes.c_char_p, [vp]),
     "lw_last_error": (ctypes.c_char_p, []),
     "lw_hip_close": (None, [vp]),

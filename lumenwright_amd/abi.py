@@ -298,3 +298,84 @@ class BoundView:
     @property
     def ref(self):
         return ctypes.byref(self.c)
+
+
+# ---- quick mode (lw_quick_vertex, lw_quick_frame) ----
+
+# Vertex (crates/lumenwright/src/scene/mesh.rs:12-21): camera-relative position, atlas UV,
+# sRGB colour.
+QUICK_VERTEX = np.dtype([("pos", "<f4", 3), ("uv", "<f4", 2), ("color", "u1", 4)])
+assert QUICK_VERTEX.itemsize == 24
+
+
+class QuickFrameC(ctypes.Structure):
+    """lw_quick_frame."""
+    _fields_ = [("vertices", ctypes.c_void_p), ("vertices_len", ctypes.c_size_t),
+                ("opaque", ctypes.c_void_p), ("opaque_len", ctypes.c_size_t),
+                ("translucent", ctypes.c_void_p), ("translucent_len", ctypes.c_size_t),
+                ("see_through", ctypes.c_void_p), ("see_through_len", ctypes.c_size_t),
+                ("atlas_pixels", ctypes.c_void_p), ("atlas_size", ctypes.c_uint32 * 2),
+                ("atlas_mips", ctypes.c_void_p), ("atlas_mips_len", ctypes.c_size_t),
+                ("mip_levels", ctypes.c_uint32), ("mip_rows", ctypes.c_uint32),
+                ("stars", ctypes.c_void_p), ("stars_len", ctypes.c_size_t),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("view_proj", ctypes.c_float * 16), ("sky", ctypes.c_void_p)]
+
+
+@dataclass
+class QuickMesh:
+    """Mesh (crates/lumenwright/src/scene/mesh.rs:24-36): the vertices and three index lists,
+    three indices a triangle (opaque and cutout, translucent back to front, see-through)."""
+
+    vertices: np.ndarray
+    opaque: np.ndarray
+    translucent: np.ndarray = field(default_factory=lambda: np.zeros(0, "<u4"))
+    see_through: np.ndarray = field(default_factory=lambda: np.zeros(0, "<u4"))
+
+    def triangle_count(self) -> int:
+        return (self.opaque.size + self.translucent.size + self.see_through.size) // 3
+
+
+@dataclass
+class QuickFrame:
+    """QuickFrame (crates/lumenwright/src/render/mod.rs:53-78). view_proj is glam's Mat4 as 16
+    floats, column-major; sky is a QuickSky (lumenwright_amd.frames.QuickSkyC)."""
+
+    mesh: QuickMesh
+    atlas_pixels: np.ndarray
+    atlas_size: tuple
+    atlas_mips: np.ndarray
+    mip_levels: int
+    mip_rows: int
+    stars: np.ndarray
+    width: int
+    height: int
+    view_proj: np.ndarray
+    sky: object
+
+    def c(self):
+        """The lw_quick_frame and the arrays it points into (keep both alive)."""
+        keep = {
+            "v": np.ascontiguousarray(self.mesh.vertices, dtype=QUICK_VERTEX),
+            "o": np.ascontiguousarray(self.mesh.opaque, dtype="<u4"),
+            "t": np.ascontiguousarray(self.mesh.translucent, dtype="<u4"),
+            "s": np.ascontiguousarray(self.mesh.see_through, dtype="<u4"),
+            "atlas": np.ascontiguousarray(self.atlas_pixels, dtype="<u4"),
+            "mips": np.ascontiguousarray(self.atlas_mips, dtype="<u4"),
+            "stars": np.ascontiguousarray(self.stars, dtype="<u4"),
+            "sky": self.sky,
+        }
+        c = QuickFrameC()
+        c.vertices, c.vertices_len = _ptr(keep["v"]), keep["v"].size
+        c.opaque, c.opaque_len = _ptr(keep["o"]), keep["o"].size
+        c.translucent, c.translucent_len = _ptr(keep["t"]), keep["t"].size
+        c.see_through, c.see_through_len = _ptr(keep["s"]), keep["s"].size
+        c.atlas_pixels = _ptr(keep["atlas"])
+        c.atlas_size[:] = [int(a) for a in self.atlas_size]
+        c.atlas_mips, c.atlas_mips_len = _ptr(keep["mips"]), keep["mips"].size
+        c.mip_levels, c.mip_rows = int(self.mip_levels), int(self.mip_rows)
+        c.stars, c.stars_len = _ptr(keep["stars"]), keep["stars"].size
+        c.width, c.height = int(self.width), int(self.height)
+        c.view_proj[:] = [float(x) for x in np.asarray(self.view_proj, "<f4").reshape(16)]
+        c.sky = ctypes.addressof(self.sky)
+        return c, keep

@@ -16,7 +16,7 @@ import numpy as np
 from . import _native
 from .abi import (EMITTER, FORMS, HIT, MODEL, PARAMS, QUAD, SCENE_EMITTER_GRID, SCENE_EMITTERS,
                   SCENE_GRID, SCENE_QUADS, VIEW, BoundView, BuildStatsC, FitOptionsC, FitStatsC,
-                  GameSkyC, HqFrame, HqScene, TimingC, TuneResultC, VoxelVolume)
+                  GameSkyC, HqFrame, HqScene, QuickFrame, TimingC, TuneResultC, VoxelVolume)
 from .frames import SceneInfoC
 
 LW_OK, LW_EINVAL, LW_ENODEV, LW_EHIP, LW_ENOMEM, LW_EUNSUPPORTED = 0, -1, -2, -3, -4, -5
@@ -87,10 +87,16 @@ class HipBackend:
         """Backend::name (render/mod.rs:94)."""
         return _native.lib().lw_hip_name(self.handle).decode()
 
-    def render_quick(self, frame=None):
-        """Backend::render_quick: there is no HIP rasterizer; the caller takes the CPU
-        rasterizer, as the reference's CUDA backend does (render/cuda.rs:70-75)."""
-        self._check(_native.lib().lw_hip_render_quick(self.handle))
+    def render_quick(self, frame: QuickFrame) -> np.ndarray:
+        """Backend::render_quick (render/mod.rs:96): the frame rasterized on the device as the
+        reference's CPU rasterizer draws it (render/cpu/raster.rs): width * height packed RGBA8
+        words (red in the low byte), row major, top row first. last_timing() then holds the
+        device time (trace_ms, inputs resident) of the call."""
+        c, keep = frame.c()
+        out = np.zeros(int(frame.width) * int(frame.height), dtype="<u4")
+        self._check(_native.lib().lw_hip_render_quick(self.handle, ctypes.byref(c), _addr(out)))
+        del keep
+        return out
 
     def render_hq(self, frame: HqFrame,
                   progress: Optional[Callable[[int, int], None]] = None) -> np.ndarray:

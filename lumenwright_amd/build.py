@@ -19,8 +19,10 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_lib" / "liblumenwright_hip.so"
-SOURCES = [CSRC / "lw_hip.hip", CSRC / "lw_build.hip", CSRC / "lw_host.cpp", CSRC / "lw_quick_sky.cpp"]
+SOURCES = [CSRC / "lw_hip.hip", CSRC / "lw_build.hip", CSRC / "lw_quick.hip", CSRC / "lw_host.cpp",
+           CSRC / "lw_quick_sky.cpp"]
 HEADERS = [CSRC / "lw_math.hpp", CSRC / "lw_device.hpp", CSRC / "lw_host.hpp", CSRC / "lw_build.hpp", CSRC / "lw_sky.hpp", CSRC / "lw_quick_sky.hpp",
+           CSRC / "lw_quick.hpp", CSRC / "lw_libm.hpp",
            ROOT / "include" / "lw" / "abi.h", ROOT / "include" / "lw" / "host.h"]
 
 ARCH = os.environ.get("LW_OFFLOAD_ARCH", "gfx950")

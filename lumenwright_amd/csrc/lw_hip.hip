@@ -31,6 +31,8 @@
 #include "lw_build.hpp"
 #include "lw_sky.hpp"
 #include "lw_quick_sky.hpp"
+#include "lw_libm.hpp"
+#include "lw_quick.hpp"
 
 using namespace lw;
 
@@ -473,67 +475,34 @@ __global__ void __launch_bounds__(kBlock) k_tonemap(const float *__restrict__ hd
   out[i] = px;
 }
 
-// libm's powf(x, y) for x in [0, 1] and the two exponents the post and sky code raise to:
-// 1/2.2f (the display encoding, post.rs:141-144) and 2.2f (the texel decoding, sky.rs:872).
-// powf stays within one ulp but is not always correctly rounded. Here the power is taken in
-// f64 (error near 2^-52) and rounded once to f32. Over every f32 in [0, 1], libm's result
-// differs from the correctly rounded one only where the exact power lies within 2^-33.2
-// (exponent 1/2.2) or 2^-32.9 (exponent 2.2), relative, of an f32 rounding midpoint, so an
-// input whose f64 power lies within kPowBand = 2^-32 of one is "in the band". For those, and only those, the result comes from
-// a table the host makes once with libm itself (ensure_pow_fix): the sorted pairs (input bits,
-// libm result bits) of the band inputs where libm and the rounded f64 power differ.
-constexpr double kPowBand = 0x1p-32;
+// The libm tables (lw_libm.hpp): powf(x, 1/2.2f), the display encoding (post.rs:141-144);
+// powf(x, 2.2f), the texel decoding (sky.rs:872); log2f, the quick sampler's mip level
+// (texture.rs:113).
 constexpr float kPowExponent[2] = {1.0f / 2.2f, 2.2f};
-enum { POW_ENCODE = 0, POW_DECODE = 1 };
-
-// The rounded f64 power of x in [0, 1] (or NaN) and whether x is in the band. A power that
-// rounds to 0 is in the band when it lies near 2^-150, the midpoint below the least
-// subnormal.
-__device__ __forceinline__ float pow_band(float x, float y, bool &in_band) {
-  const double p = pow((double)x, (double)y);
-  const float c = (float)p;
-  in_band = false;
-  if (c > 0.0f && c <= 1.0f) {
-    const uint32_t cb = __float_as_uint(c);
-    const float nb = __uint_as_float((double)c <= p ? cb + 1u : cb - 1u);
-    const double mid = ((double)c + (double)nb) * 0.5;  // exact: adjacent f32 values
-    in_band = fabs(p - mid) < kPowBand * p;
-  } else if (c == 0.0f && p > 0.0) {
-    in_band = fabs(p - 0x1p-150) < kPowBand * p;
-  }
-  return c;
-}
-
-__device__ __forceinline__ float pow_libm(float x, float y, const uint2 *__restrict__ fix, uint32_t nfix) {
-  bool in_band;
-  const float c = pow_band(x, y, in_band);
-  if (!in_band) return c;
-  const uint32_t u = __float_as_uint(x);
-  uint32_t lo = 0, hi = nfix;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (fix[mid].x < u) lo = mid + 1; else hi = mid;
-  }
-  return (lo < nfix && fix[lo].x == u) ? __uint_as_float(fix[lo].y) : c;
-}
+enum { POW_ENCODE = 0, POW_DECODE = 1, LIBM_LOG2 = 2 };
+using lw::pow_band;
+using lw::pow_libm;
 
 __device__ __forceinline__ float encode_f(float v, const uint2 *__restrict__ fix, uint32_t nfix) {
   return pow_libm(clamp_std(v, 0.0f, 1.0f), kPowExponent[POW_ENCODE], fix, nfix);
 }
 
-// The band inputs among the f32 bit patterns [0, n) with their rounded f64 powers, appended
-// to out (one atomic per wave; entries past cap are counted, not written).
-__global__ void __launch_bounds__(kBlock) k_pow_band(uint32_t n, float y, uint2 *__restrict__ out,
-                                                     uint32_t *__restrict__ count, uint32_t cap) {
+// The band inputs among the f32 bit patterns [u0, u0 + n) with their rounded f64 values of
+// powf(x, y) or log2f(x), appended to out (one atomic per wave; entries past cap are counted,
+// not written).
+__global__ void __launch_bounds__(kBlock) k_pow_band(uint32_t u0, uint32_t n, int which, float y,
+                                                     uint2 *__restrict__ out, uint32_t *__restrict__ count,
+                                                     uint32_t cap) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t stride = gridDim.x * blockDim.x;
   // Every lane of a wave runs the same trip count, so the ballot sees the whole wave.
   const uint32_t base0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
   for (uint32_t w0 = base0; w0 < n; w0 += stride) {
-    const uint32_t u = w0 + lane;
+    const uint32_t u = u0 + w0 + lane;
     bool in_band = false;
     float c = 0.0f;
-    if (u < n) c = pow_band(__uint_as_float(u), y, in_band);
+    if (w0 + lane < n)
+      c = which == LIBM_LOG2 ? lw::log2_band(__uint_as_float(u), in_band) : pow_band(__uint_as_float(u), y, in_band);
     const uint64_t m = __ballot(in_band);
     if (m == 0) continue;
     uint32_t base = 0;
@@ -1163,14 +1132,14 @@ struct lw_hip_backend {
   DevBuf<BatchDev> batch_dev;
   DevBuf<float> tone_in;
   DevBuf<uint32_t> tone_out;
-  // libm's corrections to the rounded f64 power per exponent (see pow_libm), made on the
-  // first call that needs them.
+  // libm's corrections to the rounded f64 value per function (POW_ENCODE, POW_DECODE,
+  // LIBM_LOG2; see lw_libm.hpp), made on the first call that needs them.
   struct PowFix {
     DevBuf<uint2> fix;
     uint32_t n = 0;
     bool ready = false;
     double ms = 0.0;
-  } pow_fix[2];
+  } pow_fix[3];
   BatchPlan plan;  // the last batch's layout (its sums stay in acc when it keeps them)
   // Differentiable rendering: the parameters and derivative inputs of the last call.
   DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
@@ -2068,27 +2037,29 @@ ToneDev make_tone(float exposure, uint32_t op_a, uint32_t op_b, float weight) {
   return t;
 }
 
-// The table pow_libm reads for one exponent: every f32 in [0, 1] whose rounded f64 power
-// lies in the band (k_pow_band, one pass over the 2^30 bit patterns), then libm's powf on
-// those inputs here, keeping the ones where it differs from the device's rounding.
+// The table pow_libm or log2_libm reads: every f32 of the function's domain ([0, 1] for the
+// powers, [1, 65536] for log2) whose rounded f64 value lies in the band (k_pow_band, one pass
+// over the domain's bit patterns), then libm on those inputs here, keeping the ones where it
+// differs from the device's rounding.
 const lw_hip_backend::PowFix &ensure_pow_fix(lw_hip_backend *b, int which) {
   lw_hip_backend::PowFix &pf = b->pow_fix[which];
   if (pf.ready) return pf;
   const auto t0 = std::chrono::steady_clock::now();
-  const float y = kPowExponent[which];
-  const uint32_t n = 0x3f800001u;  // the bit patterns of [0, 1]
-  const uint32_t cap = 1u << 24;   // about 6.0M inputs are in the band
+  const float y = which == LIBM_LOG2 ? 0.0f : kPowExponent[which];
+  const uint32_t u0 = which == LIBM_LOG2 ? 0x3f800000u : 0u;     // 1.0f or 0.0f
+  const uint32_t n = which == LIBM_LOG2 ? 0x08000001u : 0x3f800001u;  // through 65536.0f or 1.0f
+  const uint32_t cap = 1u << 24;  // at most about 6.0M inputs are in the band
   DevBuf<uint2> cand;
   DevBuf<uint32_t> count;
   cand.alloc(cap);
   count.alloc(1);
   HIP_TRY(hipMemsetAsync(count.ptr, 0, sizeof(uint32_t), b->stream));
-  k_pow_band<<<dim3(8192), dim3(kBlock), 0, b->stream>>>(n, y, cand.ptr, count.ptr, cap);
+  k_pow_band<<<dim3(8192), dim3(kBlock), 0, b->stream>>>(u0, n, which, y, cand.ptr, count.ptr, cap);
   HIP_TRY(hipGetLastError());
   uint32_t found = 0;
   HIP_TRY(hipMemcpyAsync(&found, count.ptr, sizeof(found), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  if (found > cap) throw InvalidScene{"libm power band holds more inputs than expected"};
+  if (found > cap) throw InvalidScene{"libm band holds more inputs than expected"};
   std::vector<uint2> host(found);
   if (found)
     HIP_TRY(hipMemcpy(host.data(), cand.ptr, (size_t)found * sizeof(uint2), hipMemcpyDeviceToHost));
@@ -2096,7 +2067,7 @@ const lw_hip_backend::PowFix &ensure_pow_fix(lw_hip_backend *b, int which) {
   for (const uint2 &e : host) {
     float x;
     std::memcpy(&x, &e.x, 4);
-    const float g = lw::libm_powf(x, y);
+    const float g = which == LIBM_LOG2 ? lw::libm_log2f(x) : lw::libm_powf(x, y);
     uint32_t gb;
     std::memcpy(&gb, &g, 4);
     if (gb != e.y) fix.push_back(make_uint2(e.x, gb));
@@ -2275,9 +2246,87 @@ const char *lw_hip_name(const lw_hip_backend *b) { return b ? b->name.c_str() : 
 
 const char *lw_hip_last_error(const lw_hip_backend *b) { return b ? b->last_error.c_str() : ""; }
 
-int lw_hip_render_quick(lw_hip_backend *b) {
-  set_err(b, "quick mode has no HIP path; use the CPU rasterizer (render/cuda.rs:70-75)");
-  return LW_EUNSUPPORTED;
+int lw_hip_render_quick(lw_hip_backend *b, const lw_quick_frame *f, uint32_t *out) {
+  if (!b || !f || !out || !f->sky) return LW_EINVAL;
+  return guarded(b, [&] {
+    auto fail = [](const std::string &m) { throw InvalidScene{m}; };
+    if (f->width == 0 || f->height == 0 || (uint64_t)f->width * f->height >= (1ull << 31))
+      fail("frame size out of range");
+    // Whole triangles only: a trailing partial triangle is dropped, as raster.rs:199 drops it.
+    const size_t n_opaque = f->opaque_len / 3 * 3, n_translucent = f->translucent_len / 3 * 3,
+                 n_see_through = f->see_through_len / 3 * 3;
+    const size_t n_idx = n_opaque + n_translucent + n_see_through;
+    if ((n_idx && !f->vertices) || f->vertices_len > 0xFFFFFFFFull || n_idx / 3 > 0x3FFFFFFFull) fail("vertices");
+    std::vector<uint32_t> idx;
+    idx.reserve(n_idx);
+    for (auto list : {std::make_pair(f->opaque, n_opaque), std::make_pair(f->translucent, n_translucent),
+                      std::make_pair(f->see_through, n_see_through)}) {
+      if (list.second && !list.first) fail("an index list is null");
+      for (size_t k = 0; k < list.second; k++) {
+        if (list.first[k] >= f->vertices_len) fail("an index names a vertex past the vertex list");
+        idx.push_back(list.first[k]);
+      }
+    }
+    const uint32_t aw = f->atlas_size[0], ah = f->atlas_size[1];
+    if (!f->atlas_pixels || aw == 0 || ah == 0 || (uint64_t)aw * ah > 0xFFFFFFFFull) fail("atlas");
+    if (f->mip_levels > lwq::kMaxMipLevels || (f->mip_levels && ((aw >> f->mip_levels) == 0 || (ah >> f->mip_levels) == 0)))
+      fail("mip levels out of range");
+    lwq::Input in{};
+    uint64_t mip_words = 0;
+    for (uint32_t l = 1; l <= f->mip_levels; l++) {
+      in.mip_offset[l] = (uint32_t)mip_words;
+      mip_words += (uint64_t)(aw >> l) * (ah >> l);
+    }
+    if (mip_words > f->atlas_mips_len || (mip_words && !f->atlas_mips)) fail("atlas mips shorter than their levels");
+    const size_t nbins = (size_t)LW_STAR_BINS_AZ * LW_STAR_BINS_EL;
+    if (!f->stars || f->stars_len < nbins + 1) fail("star table shorter than its bins");
+    for (size_t k = 0; k < nbins; k++)
+      if (f->stars[k] > f->stars[k + 1]) fail("star table bins out of order");
+    if (nbins + 1 + (size_t)f->stars[nbins] * LW_STAR_WORDS > f->stars_len) fail("star table entries past its end");
+    HIP_TRY(hipSetDevice(b->device));
+    const auto &fix = ensure_pow_fix(b, LIBM_LOG2);
+    DevBuf<lw_quick_vertex> dverts;
+    DevBuf<uint32_t> didx, datlas, dmips, dstars, dout;
+    dverts.upload(f->vertices, f->vertices_len, b->stream);
+    didx.upload(idx.data(), idx.size(), b->stream);
+    datlas.upload(f->atlas_pixels, (size_t)aw * ah, b->stream);
+    dmips.upload(f->atlas_mips, mip_words, b->stream);
+    dstars.upload(f->stars, f->stars_len, b->stream);
+    dout.alloc((size_t)f->width * f->height);
+    in.verts = dverts.ptr;
+    in.n_verts = (uint32_t)f->vertices_len;
+    in.indices = didx.ptr;
+    in.n_opaque = (uint32_t)(n_opaque / 3);
+    in.n_translucent = (uint32_t)(n_translucent / 3);
+    in.n_see_through = (uint32_t)(n_see_through / 3);
+    in.atlas = datlas.ptr;
+    in.atlas_w = aw;
+    in.atlas_h = ah;
+    in.mips = dmips.ptr;
+    in.levels = f->mip_levels;
+    in.mip_rows = f->mip_rows;
+    in.stars = dstars.ptr;
+    in.width = f->width;
+    in.height = f->height;
+    std::memcpy(in.view_proj, f->view_proj, sizeof(in.view_proj));
+    in.sky = *f->sky;
+    lw_disc_sector_trig(in.sectors.s0, in.sectors.c0, in.sectors.s1, in.sectors.c1);
+    in.log2_fix = fix.fix.ptr;
+    in.n_log2_fix = fix.n;
+    lwq::Stats st;
+    try {
+      lwq::render(b->stream, in, dout.ptr, st);
+    } catch (const lwq::Error &e) {
+      if (e.invalid) throw InvalidScene{e.what};
+      throw HipError{e.code, e.what};
+    }
+    HIP_TRY(hipMemcpy(out, dout.ptr, (size_t)f->width * f->height * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    lw_timing tm{};
+    tm.trace_ms = st.kernel_ms;
+    tm.launches = 1;
+    tm.samples = (uint64_t)f->width * f->height;
+    b->timing = tm;
+  });
 }
 
 void lw_hip_close(lw_hip_backend *b) {

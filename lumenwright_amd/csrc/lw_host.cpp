@@ -857,6 +857,7 @@ uint32_t display_byte(float v) {  // post.rs:141-144,165 with Rust's saturating 
 
 float lw::display_pow(float x) { return std::pow(x, 1.0f / 2.2f); }
 float lw::libm_powf(float x, float y) { return std::pow(x, y); }
+float lw::libm_log2f(float x) { return std::log2(x); }
 
 void lw_hable_curve(uint32_t op, float out[8]) {
   Hable h = kFilmicWorlds;

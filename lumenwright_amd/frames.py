@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _native
 from .abi import (EMITTER, EMITTER_SAMPLING_NONE, EMITTER_SAMPLING_ONE, MODEL, PARAMS, QUAD,
-                  HqFrame, HqScene, VoxelVolume)
+                  QUICK_VERTEX, HqFrame, HqScene, VoxelVolume)
 
 f3 = ctypes.c_float * 3
 
@@ -512,3 +512,283 @@ def batch_views(cfg: FrameConfig, scene: HqScene, n_views: int = 64, seed: int =
         p["misc_u"][0] = v
         views.append((p, spp))
     return base, views
+
+
+# ---- quick mode: synthetic frames ----
+
+QUICK_TILE = 16  # texels per block texture side
+
+
+def quick_view_proj(yaw: float, pitch: float, fov_y: float, aspect: float, near: float,
+                    far: float) -> np.ndarray:
+    """Camera::projection(aspect, near, far) * Camera::view(0) (scene/camera.rs:47-70), glam's
+    Mat4 as 16 floats, column-major: the camera-relative view-projection of a quick frame
+    (request.rs:697-698)."""
+    f, r, u = camera_basis(yaw, pitch)
+    view = np.zeros((4, 4), "<f4")  # view[c] = column c
+    view[0] = [r[0], u[0], -f[0], 0.0]
+    view[1] = [r[1], u[1], -f[1], 0.0]
+    view[2] = [r[2], u[2], -f[2], 0.0]
+    view[3] = [0.0, 0.0, 0.0, 1.0]
+    h = np.float32(1.0) / np.float32(np.tan(np.float32(np.radians(np.float32(fov_y))) * np.float32(0.5)))
+    w = h / np.float32(aspect)
+    rr = np.float32(far) / (np.float32(near) - np.float32(far))
+    proj = np.zeros((4, 4), "<f4")
+    proj[0, 0], proj[1, 1], proj[2, 2], proj[2, 3], proj[3, 2] = w, h, rr, -1.0, rr * np.float32(near)
+    out = np.zeros((4, 4), "<f4")
+    for c in range(4):
+        v = view[c]
+        acc = proj[0] * v[0]
+        acc = acc + proj[1] * v[1]
+        acc = acc + proj[2] * v[2]
+        out[c] = acc + proj[3] * v[3]
+    return out.reshape(16)
+
+
+def _box_down(level: np.ndarray) -> np.ndarray:
+    """A 2 x 2 box filter of packed RGBA8 texels (h, w) -> (h / 2, w / 2)."""
+    ch = np.stack([(level >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.float32)
+    m = (ch[0::2, 0::2] + ch[1::2, 0::2] + ch[0::2, 1::2] + ch[1::2, 1::2]) * 0.25
+    m = np.minimum(np.rint(m), 255).astype(np.uint32)
+    return (m[..., 0] | (m[..., 1] << 8) | (m[..., 2] << 16) | (m[..., 3] << 24)).astype("<u4")
+
+
+@dataclass
+class QuickAtlas:
+    """A block atlas with its mip chain (assets/atlas.rs's layout): 16 x 16 block textures in
+    the mipmapped rows at the top (cube, cutout and water tiles), then a region read nearest
+    (text glyphs and the sun sprite)."""
+    pixels: np.ndarray
+    size: tuple
+    mips: np.ndarray
+    levels: int
+    mip_rows: int
+    cube: list
+    cutout: list
+    water: tuple
+    text: tuple   # (x, y, w, h) texels
+    sun: tuple    # (x, y, w, h) texels
+
+
+def quick_atlas(cube_types: int = 56, cutout_types: int = 8, seed: int = 2, levels: int = 4) -> QuickAtlas:
+    rng = np.random.default_rng(seed)
+    T, per_row = QUICK_TILE, 16
+    n_tiles = cube_types + cutout_types + 1
+    block_rows = (n_tiles + per_row - 1) // per_row
+    mip_rows = block_rows * T
+    W, H = per_row * T, mip_rows + 64
+    img = np.zeros((H, W), dtype="<u4")
+
+    def rgba(r, g, b, a):
+        return (r.astype(np.uint32) | (g.astype(np.uint32) << 8) | (b.astype(np.uint32) << 16)
+                | (np.asarray(a, np.uint32) << 24))
+
+    tiles = []
+    for k in range(n_tiles):
+        x, y = (k % per_row) * T, (k // per_row) * T
+        base = rng.integers(40, 216, 3)
+        noise = rng.integers(-30, 31, (T, T, 3))
+        c = np.clip(base[None, None, :] + noise, 0, 255)
+        a = np.full((T, T), 255)
+        if cube_types <= k < cube_types + cutout_types:
+            a = np.where(rng.random((T, T)) < 0.4, 0, 255)  # leaves: holes for the alpha test
+        elif k == n_tiles - 1:
+            c = np.clip(np.array([50, 90, 200])[None, None, :] + noise // 3, 0, 255)
+            a = np.full((T, T), 180)
+        img[y:y + T, x:x + T] = rgba(c[..., 0], c[..., 1], c[..., 2], a)
+        tiles.append((x, y))
+    # text glyphs (white ink, a dark see-through backdrop) and the sun
+    tx, ty, tw, th = 0, mip_rows, 128, 32
+    ink = rng.random((th, tw)) < 0.3
+    img[ty:ty + th, tx:tx + tw] = np.where(ink, 0xFFFFFFFF, 0x40000000).astype("<u4")
+    sx, sy = 128, mip_rows
+    yy, xx = np.mgrid[0:T, 0:T]
+    disc = (xx - 7.5) ** 2 + (yy - 7.5) ** 2 < 36
+    img[sy:sy + T, sx:sx + T] = np.where(disc, 0xFF80F0FF, 0xFF000000).astype("<u4")
+    mips, lv = [], img
+    for _ in range(levels):
+        lv = _box_down(lv)
+        mips.append(lv.reshape(-1))
+    return QuickAtlas(pixels=img.reshape(-1), size=(W, H), mips=np.concatenate(mips) if mips else
+                      np.zeros(0, "<u4"), levels=levels, mip_rows=mip_rows,
+                      cube=tiles[:cube_types], cutout=tiles[cube_types:cube_types + cutout_types],
+                      water=tiles[-1], text=(tx, ty, tw, th), sun=(sx, sy, T, T))
+
+
+# Per face: outward normal, the corner offset of its base and its two edges (t1 x t2 = normal,
+# so the corners base, +t1, +t1+t2, +t2 wind counter-clockwise seen from outside).
+_FACES = {
+    "+y": ((0, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0), 1.0),
+    "-y": ((0, -1, 0), (0, 0, 0), (1, 0, 0), (0, 0, 1), 0.5),
+    "+x": ((1, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), 0.6),
+    "-x": ((-1, 0, 0), (0, 0, 0), (0, 0, 1), (0, 1, 0), 0.6),
+    "+z": ((0, 0, 1), (0, 0, 1), (1, 0, 0), (0, 1, 0), 0.8),
+    "-z": ((0, 0, -1), (0, 0, 0), (0, 1, 0), (1, 0, 0), 0.8),
+}
+
+
+def _hash3(x, y, z):
+    h = (np.asarray(x, np.int64) * 73856093) ^ (np.asarray(y, np.int64) * 19349663) ^ (np.asarray(z, np.int64) * 83492791)
+    return (h & 0x7FFFFFFF).astype(np.int64)
+
+
+class _QuadSink:
+    def __init__(self, atlas: QuickAtlas, eye):
+        self.atlas, self.eye = atlas, np.asarray(eye, np.float64)
+        self.parts = []
+
+    def faces(self, face: str, blocks: np.ndarray, tiles: np.ndarray, alpha=255, y_off=0.0, rect=None):
+        """One quad per block (n, 3 integer corners) on `face`, textured with atlas tiles (n, 2)
+        (or one rect (x, y, w, h) for all)."""
+        if len(blocks) == 0:
+            return
+        _, base, t1, t2, shade = _FACES[face]
+        p0 = blocks.astype(np.float64) + np.array(base, np.float64) + np.array([0.0, y_off, 0.0])
+        t1, t2 = np.array(t1, np.float64), np.array(t2, np.float64)
+        corners = np.stack([p0, p0 + t1, p0 + t1 + t2, p0 + t2], 1) - self.eye  # (n, 4, 3)
+        W, H = self.atlas.size
+        if rect is None:
+            u0, v0 = tiles[:, 0] / W, tiles[:, 1] / H
+            u1, v1 = (tiles[:, 0] + QUICK_TILE) / W, (tiles[:, 1] + QUICK_TILE) / H
+        else:
+            u0 = np.full(len(blocks), rect[0] / W)
+            v0 = np.full(len(blocks), rect[1] / H)
+            u1 = np.full(len(blocks), (rect[0] + rect[2]) / W)
+            v1 = np.full(len(blocks), (rect[1] + rect[3]) / H)
+        uv = np.stack([np.stack([u0, v1], -1), np.stack([u1, v1], -1), np.stack([u1, v0], -1),
+                       np.stack([u0, v0], -1)], 1)
+        ao = np.array([1.0, 0.8, 0.9, 0.7])[(_hash3(blocks[:, 0], blocks[:, 1], blocks[:, 2])[:, None]
+                                             + np.arange(4)[None, :]) % 4]
+        light = np.clip(np.rint(255.0 * shade * 0.92 * ao), 0, 255)
+        v = np.zeros((len(blocks), 4), dtype=QUICK_VERTEX)
+        v["pos"] = corners.astype("<f4")
+        v["uv"] = uv.astype("<f4")
+        v["color"][..., 0] = light
+        v["color"][..., 1] = light
+        v["color"][..., 2] = light
+        v["color"][..., 3] = alpha
+        self.parts.append(v)
+
+    def take(self):
+        """The quads so far as (vertices (4n), centres (n, 3)); clears the sink."""
+        if not self.parts:
+            return np.zeros(0, QUICK_VERTEX), np.zeros((0, 3))
+        v = np.concatenate(self.parts)
+        self.parts = []
+        return v.reshape(-1), v["pos"].astype(np.float64).mean(axis=1)
+
+
+def _quad_indices(n: int, first: int) -> np.ndarray:
+    q = first + 4 * np.arange(n, dtype=np.int64)[:, None]
+    return (q + np.array([0, 1, 2, 0, 2, 3])[None, :]).astype("<u4").reshape(-1)
+
+
+def quick_frame(cfg: "FrameConfig", draw_distance: float = 128.0, width: Optional[int] = None,
+                height: Optional[int] = None, displays: int = 3):
+    """A quick-mode frame of the configuration's world and camera (request.rs:691-747 with the
+    QuickOptions defaults: draw distance 128, 4 mip levels, no supersampling), the mesh made
+    from the synthetic world's height field: every exposed face of the terrain and of leaf
+    blocks (opaque and cutout), water surfaces below the world's water level (translucent, back
+    to front) and text displays before the camera (see-through, far to near). Chunks outside
+    the view frustum or the draw distance are left out, as the reference's mesh builder leaves
+    them out (scene/mesh.rs:329). Returns (QuickFrame, atlas)."""
+    from .abi import QuickFrame, QuickMesh
+    spec = cfg.world
+    W, H = width or cfg.width, height or cfg.height
+    gx, gz = spec.chunks_x * 16, spec.chunks_z * 16
+    cx, cz = gx // 2, gz // 2
+    eye = np.array([cx + 0.5, world_height(spec, cx, cz) + cfg.camera_height, cz + 0.5])
+    near, far = 0.05, draw_distance * 2.0 + 512.0
+    vp = quick_view_proj(cfg.yaw, cfg.pitch, cfg.fov_y, W / H, near, far)
+    atlas = quick_atlas(max(spec.cube_types, 1), max(spec.cutout_types, 1), spec.texture_seed)
+    R = int(draw_distance)
+    x0, x1 = max(cx - R, 0), min(cx + R, gx - 1)
+    z0, z1 = max(cz - R, 0), min(cz + R, gz - 1)
+    xs, zs = np.arange(x0, x1 + 1), np.arange(z0, z1 + 1)
+    L = _native.lib()
+    sc = spec.c()
+    hm = np.array([[L.lw_world_height(ctypes.byref(sc), int(x), int(z)) for z in zs] for x in xs], np.int64)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    # chunk culling: the draw distance, then the frustum (all 8 corners outside one plane)
+    m = vp.reshape(4, 4).astype(np.float64)  # rows are columns
+    keep = np.zeros_like(hm, dtype=bool)
+    for bx in range(x0 // 16, x1 // 16 + 1):
+        for bz in range(z0 // 16, z1 // 16 + 1):
+            sel = (X // 16 == bx) & (Z // 16 == bz)
+            if not sel.any():
+                continue
+            dx = max(bx * 16 - eye[0], 0, eye[0] - (bx * 16 + 16))
+            dz = max(bz * 16 - eye[2], 0, eye[2] - (bz * 16 + 16))
+            if np.hypot(dx, dz) > draw_distance:
+                continue
+            y_lo, y_hi = hm[sel].min() - 1, hm[sel].max() + 3
+            c = np.array([[bx * 16 + i * 16, y, bz * 16 + k * 16] for i in (0, 1) for k in (0, 1)
+                          for y in (y_lo, y_hi)], np.float64) - eye
+            clip = c @ m[:3] + m[3]
+            if (np.all(clip[:, 0] > clip[:, 3]) or np.all(clip[:, 0] < -clip[:, 3]) or
+                    np.all(clip[:, 1] > clip[:, 3]) or np.all(clip[:, 1] < -clip[:, 3]) or
+                    np.all(clip[:, 2] < 0)):
+                continue
+            keep |= sel
+    n_cube = len(atlas.cube)
+    cube = np.array(atlas.cube, np.int64)
+    sink = _QuadSink(atlas, eye)
+    kx, kz, kh = X[keep], Z[keep], hm[keep]
+    top = np.stack([kx, kh, kz], -1)
+    sink.faces("+y", top, cube[_hash3(kx, kh, kz) % n_cube])
+    pad = np.pad(hm, 1, mode="edge")
+    for face, (ox, oz) in (("+x", (1, 0)), ("-x", (-1, 0)), ("+z", (0, 1)), ("-z", (0, -1))):
+        hn = pad[1 + ox:1 + ox + hm.shape[0], 1 + oz:1 + oz + hm.shape[1]][keep]
+        depth = np.maximum(kh - hn, 0)
+        rep = np.repeat(np.arange(kh.size), depth)
+        y = kh[rep] - (np.arange(rep.size) - np.repeat(np.cumsum(depth) - depth, depth))
+        blk = np.stack([kx[rep], y, kz[rep]], -1)
+        sink.faces(face, blk, cube[_hash3(blk[:, 0], blk[:, 1] // 4, blk[:, 2]) % n_cube])
+    # leaves: a two-block cutout column on a few dry columns
+    wl = spec.water_level
+    leafy = (_hash3(kx, 7, kz) % 23 == 0) & (kh >= wl)
+    cut = np.array(atlas.cutout, np.int64)
+    for dy in (1, 2):
+        blk = np.stack([kx[leafy], kh[leafy] + dy, kz[leafy]], -1)
+        t = cut[_hash3(blk[:, 0], blk[:, 1], blk[:, 2]) % len(cut)]
+        for face in _FACES:
+            sink.faces(face, blk, t)
+    opaque_v, _ = sink.take()
+    # water surfaces, back to front
+    wet = kh < wl
+    blk = np.stack([kx[wet], np.full(wet.sum(), wl), kz[wet]], -1)
+    sink.faces("+y", blk, np.tile(np.array(atlas.water, np.int64), (len(blk), 1)), y_off=-0.125)
+    water_v, centres = sink.take()
+    order = np.argsort(-np.einsum("ij,ij->i", centres, centres), kind="stable")
+    water_v = water_v.reshape(-1, 4)[order].reshape(-1)
+    # text displays facing the camera, far to near
+    f, r, u = (np.asarray(a, np.float64) for a in camera_basis(cfg.yaw, cfg.pitch))
+    disp = []
+    for k in range(displays):
+        centre = f * (10.0 - 2.5 * k) + r * (1.5 * (k - 1)) + u * 0.6 * k
+        p0 = centre - r * 2.0 - u * 0.5
+        corners = np.stack([p0, p0 + 4.0 * r, p0 + 4.0 * r + u, p0 + u])
+        tx, ty, tw, th = atlas.text
+        Wa, Ha = atlas.size
+        v = np.zeros(4, QUICK_VERTEX)
+        v["pos"] = corners.astype("<f4")
+        v["uv"] = np.array([[tx, ty + th], [tx + tw, ty + th], [tx + tw, ty], [tx, ty]], np.float64) / [Wa, Ha]
+        v["color"] = [255, 255, 255, 255]
+        disp.append(v)
+    see_v = np.concatenate(disp) if disp else np.zeros(0, QUICK_VERTEX)
+    verts = np.concatenate([opaque_v, water_v, see_v])
+    n_o, n_w, n_s = opaque_v.size // 4, water_v.size // 4, see_v.size // 4
+    mesh = QuickMesh(vertices=verts, opaque=_quad_indices(n_o, 0),
+                     translucent=_quad_indices(n_w, 4 * n_o),
+                     see_through=_quad_indices(n_s, 4 * (n_o + n_w)))
+    sun = celestial_direction(90.29)
+    desc = sky_description(
+        sky=(0.18, 0.24, 0.37), fog=(0.62, 0.40, 0.33), sky_fog_end=draw_distance,
+        render_fog=(draw_distance - min(max(draw_distance / 10.0, 4.0), 64.0), draw_distance),
+        env_fog=(0.0, 1024.0), sky_disc=True, sunrise=(0.98, 0.57, 0.25, 0.62), sun_angle=90.29,
+        sun=(sun, atlas.sun), star_angle=90.29, star_brightness=0.38)
+    sky = quick_sky_new(camera_rays(cfg.yaw, cfg.pitch, cfg.fov_y, W, H), W, H, desc)
+    frame = QuickFrame(mesh=mesh, atlas_pixels=atlas.pixels, atlas_size=atlas.size,
+                       atlas_mips=atlas.mips, mip_levels=atlas.levels, mip_rows=atlas.mip_rows,
+                       stars=star_table(), width=W, height=H, view_proj=vp, sky=sky)
+    return frame, atlas

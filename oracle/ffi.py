@@ -113,6 +113,12 @@ def lib():
         L.lwo_game_sky_maps.restype = None
         L.lwo_blend_chunky_sky.argtypes = [vp, ctypes.c_float, vp, vp]
         L.lwo_blend_chunky_sky.restype = i32
+        L.lwo_log2f_band.argtypes = [ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
+        L.lwo_log2f_band.restype = None
+        L.lwo_rasterize.argtypes = [vp, sz, vp, vp, vp, vp, vp, u32, u32, u32, u32, vp, vp, vp]
+        L.lwo_rasterize.restype = i32
+        L.lwo_atlas_sample.argtypes = [vp, vp, vp, u32, u32, i32, vp, sz, vp]
+        L.lwo_atlas_sample.restype = None
         L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
         L.lwo_build_layout.restype = i32
         L.lwo_build_fetch.argtypes = [vp, vp, vp]
@@ -557,3 +563,45 @@ def game_sky_maps(lens, light, factor: float, colour, atlas, atlas_w: int):
                             _addr(at), atlas_w, _addr(out), _addr(info))
     return out, {"light": list(info[0:3]), "mean": list(info[3:6]), "scale": float(info[6]),
                  "veil": list(info[7:10])}
+
+
+def log2f_band(threads: int = 0):
+    """libm's log2f against the rounded f64 log2 over every f32 in [1, 65536], as powf_band."""
+    st = np.zeros(4, dtype=np.uint64)
+    mr = ctypes.c_double()
+    lib().lwo_log2f_band(threads or os.cpu_count() or 1, _addr(st), ctypes.byref(mr), None, 0)
+    return (int(st[0]), int(st[1]), int(st[2]), int(st[3]), mr.value)
+
+
+def rasterize(frame):
+    """rasterize (raster.rs:168-372) of a lumenwright_amd.abi.QuickFrame with the client's
+    stars: width * height packed RGBA8 words."""
+    from lumenwright_amd.abi import QUICK_VERTEX
+    m = frame.mesh
+    v = np.ascontiguousarray(m.vertices, dtype=QUICK_VERTEX)
+    lists = [np.ascontiguousarray(a, dtype="<u4") for a in (m.opaque, m.translucent, m.see_through)]
+    ptrs = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in lists])
+    lens = (ctypes.c_size_t * 3)(*[a.size for a in lists])
+    atlas = np.ascontiguousarray(frame.atlas_pixels, dtype="<u4")
+    mips = np.ascontiguousarray(frame.atlas_mips, dtype="<u4")
+    size = np.array(frame.atlas_size, dtype="<u4")
+    vp = _f32(np.asarray(frame.view_proj).reshape(16))
+    out = np.zeros(int(frame.width) * int(frame.height), dtype="<u4")
+    _check(lib().lwo_rasterize(_addr(v), v.size, ptrs, lens, _addr(atlas), _addr(size),
+                               _addr(mips) if mips.size else None, frame.mip_levels, frame.mip_rows,
+                               frame.width, frame.height, _addr(vp), ctypes.byref(frame.sky),
+                               _addr(out)))
+    return out
+
+
+def atlas_sample(level0, mips, size, levels: int, mip_rows: int, q, mode: int = 0):
+    """AtlasView (texture.rs:46-145) at points q (n, 6) = (u, v, du.x, du.y, dv.x, dv.y):
+    mode 0 sample, 1 sample_block, 2 sample_grad. Returns (n, 4) RGBA."""
+    q = _f32(q).reshape(-1, 6)
+    l0 = np.ascontiguousarray(level0, dtype="<u4")
+    mp = np.ascontiguousarray(mips, dtype="<u4")
+    sz = np.array(size, dtype="<u4")
+    out = np.zeros((q.shape[0], 4), dtype="<f4")
+    lib().lwo_atlas_sample(_addr(l0), _addr(mp) if mp.size else None, _addr(sz), levels, mip_rows,
+                           mode, _addr(q), q.shape[0], _addr(out))
+    return out

@@ -9,6 +9,7 @@
 #include "lw_oracle_post.hpp"
 #include "lw_oracle_build.hpp"
 #include "lw_oracle_sky.hpp"
+#include "lw_oracle_quick.hpp"
 
 #include <algorithm>
 #include <array>
@@ -379,53 +380,58 @@ uint64_t lwo_check_encode_thresholds(const float *thr, unsigned threads, uint32_
   return total;
 }
 
-// libm's powf(x, y) over every f32 x in [0, 1] (y = 1/2.2f, the display encoding, or 2.2f,
-// the texel decoding) against the f64 power p = pow(x, y) rounded once to f32. A deviation is
-// an x where the two differ; x is in the band when p lies within 2^-32 * p of the f32 rounding
-// midpoint on p's side of round(p) (2^-150 when p rounds to 0). stats: [0] deviations, [1] deviations outside the band, [2] inputs in the band,
-// [3] deviations that are not a neighbour of round(p); *max_rel: the largest |p - mid| / p
-// among the deviations. With `band` non-null (room for `cap`), the band inputs are written
-// there in ascending order.
-static bool band_of(float x, float y, double &rel) {
-  const double p = std::pow((double)x, (double)y);
-  const float c = (float)p;
-  rel = 1.0;
-  if (c == 0.0f && p > 0.0) {
-    rel = std::fabs(p - 0x1p-150) / p;
-    return std::fabs(p - 0x1p-150) < 0x1p-32 * p;
+// libm against its function in f64 rounded once to f32, over every f32 input in a range: powf(x,
+// y) over [0, 1] (y = 1/2.2f, the display encoding, or 2.2f, the texel decoding) and log2f over
+// [1, 65536] (the quick mode's mip level). A deviation is an input where the two differ. The
+// distance of the f64 value p from the f32 rounding midpoint on p's side of round(p) (2^-150
+// when p rounds to 0) is taken relative to |p| for powf and absolute for log2f; the input is in
+// the band when that distance is below `band` (2^-32 relative, 2^-28 absolute). stats: [0]
+// deviations, [1] deviations outside the band, [2] inputs in the band, [3] deviations that are
+// not a neighbour of round(p); *max_dist: the largest distance among the deviations. With
+// `band` non-null (room for `cap`), the band inputs are written there in ascending order.
+extern "C++" {
+static bool band_of(double p, double band, bool relative, double &dist) {
+  const double ap = std::fabs(p);
+  const float c = (float)ap;
+  dist = relative ? 1.0 : 1e300;
+  if (c == 0.0f && ap > 0.0) {
+    dist = std::fabs(ap - 0x1p-150) / (relative ? ap : 1.0);
+    return dist < band;
   }
-  if (!(c > 0.0f && c <= 1.0f)) return false;
+  if (!(c > 0.0f && c <= 3.4028234e38f)) return false;
   uint32_t cb;
   std::memcpy(&cb, &c, 4);
-  const uint32_t nbb = (double)c <= p ? cb + 1u : cb - 1u;
+  const uint32_t nbb = (double)c <= ap ? cb + 1u : cb - 1u;
   float nb;
   std::memcpy(&nb, &nbb, 4);
   const double mid = ((double)c + (double)nb) * 0.5;
-  rel = std::fabs(p - mid) / p;
-  return std::fabs(p - mid) < 0x1p-32 * p;
+  dist = std::fabs(ap - mid) / (relative ? ap : 1.0);
+  return dist < band;
 }
 
-void lwo_powf_band(float y, unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band,
-                   uint64_t cap) {
-  const uint32_t one = 0x3f800000u;
+template <typename Libm, typename Exact>
+static void scan_band(uint32_t u0, uint32_t u1, Libm libm, Exact exact, double width, bool relative,
+                      unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band, uint64_t cap) {
   if (threads == 0) threads = 1;
   std::vector<std::array<uint64_t, 4>> st(threads, std::array<uint64_t, 4>{0, 0, 0, 0});
   std::vector<double> mr(threads, 0.0);
   std::vector<std::vector<uint32_t>> found(threads);
+  const uint64_t n = (uint64_t)u1 - u0 + 1;
   auto work = [&](unsigned t) {
-    const uint32_t lo = (uint32_t)((uint64_t)(one + 1) * t / threads);
-    const uint32_t hi = (uint32_t)((uint64_t)(one + 1) * (t + 1) / threads);
-    for (uint32_t u = lo; u < hi; u++) {
+    const uint64_t lo = u0 + n * t / threads, hi = u0 + n * (t + 1) / threads;
+    for (uint64_t w = lo; w < hi; w++) {
+      const uint32_t u = (uint32_t)w;
       float x;
       std::memcpy(&x, &u, 4);
+      const double p = exact(x);
       double rel;
-      const bool in = band_of(x, y, rel);
+      const bool in = band_of(p, width, relative, rel);
       if (in) {
         if (band) found[t].push_back(u);
         st[t][2]++;
       }
-      const float g = std::pow(x, y);
-      const float c = (float)std::pow((double)x, (double)y);
+      const float g = libm(x);
+      const float c = (float)p;
       uint32_t gb, cb;
       std::memcpy(&gb, &g, 4);
       std::memcpy(&cb, &c, 4);
@@ -450,6 +456,19 @@ void lwo_powf_band(float y, unsigned threads, uint64_t stats[4], double *max_rel
     for (uint32_t u : found[t])
       if (at < cap) band[at++] = u;
   }
+}
+}  // extern "C++"
+
+void lwo_powf_band(float y, unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band,
+                   uint64_t cap) {
+  scan_band(0u, 0x3f800000u, [y](float x) { return std::pow(x, y); },
+            [y](float x) { return std::pow((double)x, (double)y); }, 0x1p-32, true, threads, stats, max_rel, band,
+            cap);
+}
+
+void lwo_log2f_band(unsigned threads, uint64_t stats[4], double *max_rel, uint32_t *band, uint64_t cap) {
+  scan_band(0x3f800000u, 0x47800000u, [](float x) { return std::log2(x); },
+            [](float x) { return std::log2((double)x); }, 0x1p-28, false, threads, stats, max_rel, band, cap);
 }
 
 // map_direction (sky.rs:856-860) and blend_chunky_sky (sky.rs:836-851) with the kernel's
@@ -546,6 +565,58 @@ void lwo_game_sky_maps(const lwo_gsky::Quick *lens, const lwo_gsky::Quick *light
                        const float *colour, const uint32_t *atlas, uint32_t atlas_w, float *out, float *info) {
   static const std::vector<uint32_t> t = lwo_gsky::star_table();
   lwo_gsky::game_sky_maps(*lens, *light, factor, {colour[0], colour[1], colour[2]}, atlas, atlas_w, t, out, info);
+}
+
+// rasterize (raster.rs:168-372): a quick frame's packed RGBA8 pixels, row major, with the
+// client's stars. lists: the opaque, translucent and see-through index lists, list_len their
+// lengths in indices.
+int lwo_rasterize(const lwo_quick::Vertex *verts, size_t n_verts, const uint32_t *const *lists,
+                  const size_t *list_len, const uint32_t *atlas, const uint32_t *atlas_size, const uint32_t *mips,
+                  uint32_t levels, uint32_t mip_rows, uint32_t width, uint32_t height, const float *view_proj,
+                  const lwo_gsky::Quick *sky, uint32_t *out) {
+  return guarded([&] {
+    static const std::vector<uint32_t> table = lwo_gsky::star_table();
+    for (int k = 0; k < 3; k++)
+      for (size_t i = 0; i < list_len[k] / 3 * 3; i++)
+        if (lists[k][i] >= n_verts) throw std::runtime_error("vertex index out of range");
+    lwo_quick::Frame f;
+    f.verts = verts;
+    f.n_verts = n_verts;
+    for (int k = 0; k < 3; k++) {
+      f.lists[k] = lists[k];
+      f.list_len[k] = list_len[k];
+    }
+    f.atlas = atlas;
+    f.aw = atlas_size[0];
+    f.ah = atlas_size[1];
+    f.mips = mips;
+    f.levels = levels;
+    f.mip_rows = mip_rows;
+    f.width = width;
+    f.height = height;
+    std::memcpy(f.view_proj, view_proj, sizeof(f.view_proj));
+    f.sky = sky;
+    const std::vector<uint32_t> px = lwo_quick::rasterize(f, table);
+    std::copy(px.begin(), px.end(), out);
+  });
+}
+
+// AtlasView (texture.rs:46-145) at n points q = (u, v, du.x, du.y, dv.x, dv.y): mode 0 sample,
+// 1 sample_block, 2 sample_grad. out: n RGBA in [0, 1].
+void lwo_atlas_sample(const uint32_t *level0, const uint32_t *mips, const uint32_t *size, uint32_t levels,
+                      uint32_t mip_rows, int mode, const float *q, size_t n, float *out) {
+  const lwo_quick::AtlasView v{level0, mips, size[0], size[1], levels, mip_rows};
+  for (size_t i = 0; i < n; i++) {
+    const float *a = q + 6 * i;
+    lwo_quick::V4 c;
+    if (mode == 2) c = v.sample_grad(a[0], a[1], a[2], a[3], a[4], a[5]);
+    else if (mode == 1 || v.is_mipmapped(a[1])) c = v.sample_block(a[0], a[1], a[2], a[3], a[4], a[5]);
+    else c = v.nearest(a[0], a[1]);
+    out[4 * i] = c.x;
+    out[4 * i + 1] = c.y;
+    out[4 * i + 2] = c.z;
+    out[4 * i + 3] = c.w;
+  }
 }
 
 // build_hq's layout (voxel.rs:305-491, emitter_list :542-607) of an interned volume. sizes:
