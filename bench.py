@@ -219,6 +219,30 @@ def sky_timing(b, frames, cpu: bool) -> dict:
     return out
 
 
+def quick_timing(b, frames, cfg, cpu: bool, reps: int = 5) -> dict:
+    """Quick mode on the device (lw_hip_render_quick) at the configuration's size: the synthetic
+    world's mesh at draw distance 128; the best of a few calls' device time (inputs resident)
+    and whole-call time (uploads and download included). With `cpu`, the oracle's
+    single-threaded rasterizer on the same frame, timed once."""
+    frame, _ = frames.quick_frame(cfg)
+    b.render_quick(frame)  # makes the log2 table
+    dev, call = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        b.render_quick(frame)
+        call.append((time.perf_counter() - t) * 1e3)
+        dev.append(b.last_timing()["trace_ms"])
+    out = {"width": frame.width, "height": frame.height, "triangles": frame.mesh.triangle_count(),
+           "device_ms": min(dev), "call_ms": min(call)}
+    if cpu:
+        from oracle import ffi
+        ffi.build()
+        t = time.perf_counter()
+        ffi.rasterize(frame)
+        out["cpu_port_ms"] = (time.perf_counter() - t) * 1e3
+    return out
+
+
 def bench_plain(args, lw, frames):
     cfg = frames.CONFIGS[args.config]
     if args.spp:
@@ -256,10 +280,12 @@ def bench_plain(args, lw, frames):
     assert np.isfinite(img).all()
     extra = {"post": post_timing(b, frames, img),
              "scene_build": build_timing(b, world, frame.sky, not args.no_cpu_baseline),
-             "sky_prep": sky_timing(b, frames, not args.no_cpu_baseline)}
+             "sky_prep": sky_timing(b, frames, not args.no_cpu_baseline),
+             "quick": quick_timing(b, frames, cfg, not args.no_cpu_baseline)}
     log(f"post-processing: {extra['post']}")
     log(f"sky preparation: {extra['sky_prep']}")
     log(f"scene build: {extra['scene_build']}")
+    log(f"quick mode: {extra['quick']}")
     if args.copy_probe:
         ms = b.copy_probe(args.copy_probe)
         extra["copy_probe"] = {"bytes": args.copy_probe, "ms": ms,

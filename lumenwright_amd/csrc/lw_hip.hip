@@ -1145,6 +1145,10 @@ struct lw_hip_backend {
   DevBuf<float> diff_pal, diff_inten, diff_dsky, diff_dldi;
   DevBuf<uint32_t> diff_qg, diff_qig;
   DevBuf<double> diff_grads;
+  // Quick mode: the frame's inputs and output, and the rasterizer's own buffers (grow-only).
+  DevBuf<lw_quick_vertex> quick_verts;
+  DevBuf<uint32_t> quick_idx, quick_atlas, quick_mips, quick_stars, quick_out;
+  lwq::Workspace *quick_ws = nullptr;
 };
 
 namespace {
@@ -2285,14 +2289,21 @@ int lw_hip_render_quick(lw_hip_backend *b, const lw_quick_frame *f, uint32_t *ou
     if (nbins + 1 + (size_t)f->stars[nbins] * LW_STAR_WORDS > f->stars_len) fail("star table entries past its end");
     HIP_TRY(hipSetDevice(b->device));
     const auto &fix = ensure_pow_fix(b, LIBM_LOG2);
-    DevBuf<lw_quick_vertex> dverts;
-    DevBuf<uint32_t> didx, datlas, dmips, dstars, dout;
-    dverts.upload(f->vertices, f->vertices_len, b->stream);
-    didx.upload(idx.data(), idx.size(), b->stream);
-    datlas.upload(f->atlas_pixels, (size_t)aw * ah, b->stream);
-    dmips.upload(f->atlas_mips, mip_words, b->stream);
-    dstars.upload(f->stars, f->stars_len, b->stream);
-    dout.alloc((size_t)f->width * f->height);
+    if (!b->quick_ws) b->quick_ws = lwq::workspace_new();
+    auto put = [&](auto &buf, const auto *src, size_t count) {
+      buf.ensure(std::max<size_t>(count, 1));
+      if (count) HIP_TRY(hipMemcpyAsync(buf.ptr, src, count * sizeof(*src), hipMemcpyHostToDevice, b->stream));
+      return buf.ptr;
+    };
+    auto &dverts = b->quick_verts;
+    auto &didx = b->quick_idx, &datlas = b->quick_atlas, &dmips = b->quick_mips, &dstars = b->quick_stars;
+    auto &dout = b->quick_out;
+    put(dverts, f->vertices, f->vertices_len);
+    put(didx, idx.data(), idx.size());
+    put(datlas, f->atlas_pixels, (size_t)aw * ah);
+    put(dmips, f->atlas_mips, (size_t)mip_words);
+    put(dstars, f->stars, f->stars_len);
+    dout.ensure((size_t)f->width * f->height);
     in.verts = dverts.ptr;
     in.n_verts = (uint32_t)f->vertices_len;
     in.indices = didx.ptr;
@@ -2315,7 +2326,7 @@ int lw_hip_render_quick(lw_hip_backend *b, const lw_quick_frame *f, uint32_t *ou
     in.n_log2_fix = fix.n;
     lwq::Stats st;
     try {
-      lwq::render(b->stream, in, dout.ptr, st);
+      lwq::render(b->stream, in, *b->quick_ws, dout.ptr, st);
     } catch (const lwq::Error &e) {
       if (e.invalid) throw InvalidScene{e.what};
       throw HipError{e.code, e.what};
@@ -2342,6 +2353,7 @@ void lw_hip_close(lw_hip_backend *b) {
   b->wf_queue.free();
   b->wf_ctr.free();
   b->batch_dev.free();
+  if (b->quick_ws) lwq::workspace_free(b->quick_ws);
   if (b->pinned) (void)hipHostFree(b->pinned);
   for (auto &ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);

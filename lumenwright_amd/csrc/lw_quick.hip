@@ -3,16 +3,19 @@
 // (crates/lumenwright/src/render/texture.rs), in the same f32 operations and order, so a frame
 // is the CPU rasterizer's pixel for pixel.
 //
-//   k_verts     one lane per vertex: the view-projection and the attributes (raster.rs:176-193)
-//   k_tris      one lane per triangle of the three passes: trivial reject, near-plane
-//               clipping into up to two triangles, screen set-up (raster.rs:51-135,195-227)
-//   scan        the surviving triangles keep their order (opaque, translucent back to front,
-//               see-through)
-//   k_bin_keys  one lane per triangle: a (tile, triangle) key per 32 x 32 tile its box
-//               touches, radix-sorted, so a tile's list is in triangle order (raster.rs:234-252)
-//   k_tiles     one workgroup per tile, four pixels a lane: the sky, then every triangle of
-//               the tile's list in order, with the depth test, the atlas sampling, the fog
-//               and the blending of raster.rs:266-357
+//   k_verts        one lane per vertex: the view-projection and the attributes (raster.rs:176-193)
+//   k_tris         one lane per triangle of the three passes: trivial reject, near-plane
+//                  clipping into up to two triangles, screen set-up (raster.rs:51-135,195-227),
+//                  into two slots per triangle, so slot order is the reference's triangle order
+//   k_bin_counts,  one lane per slot: a (tile, slot) key per 8 x 8 tile its box touches, radix-
+//   k_bin_keys     sorted, so a tile's list is in triangle order (raster.rs:234-252 bins by
+//                  32 x 32 tiles; a pixel sees the same triangles in the same order either way)
+//   k_tile_bounds  each tile's run of the sorted keys (a binary search a tile)
+//   k_tiles        one wave per 8 x 8 tile, one pixel a lane: the sky, then the tile's list in
+//                  order. Each lane queues the triangles that cover its pixel (edge and depth-
+//                  range tests) in LDS, and the wave shades the queues in lockstep: the depth
+//                  test, the atlas sampling, the fog and the blending of raster.rs:266-357, in
+//                  each pixel's triangle order.
 #include "lw_quick.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -28,7 +31,9 @@ namespace lwq {
 namespace {
 
 constexpr uint32_t kB = 256;
-constexpr uint32_t kTile = 32;  // raster.rs:12
+constexpr uint32_t kTile = 8;    // device bins: one wave64 a tile, one pixel a lane
+constexpr uint32_t kWaves = kB / 64;
+constexpr uint32_t kQueue = 8;   // covering triangles a lane holds before its wave shades them
 constexpr float kMaxAnisotropy = 4.0f;  // assets/atlas.rs:25-27
 
 #define LWQ_TRY(expr)                                                                         \
@@ -212,38 +217,49 @@ __global__ void __launch_bounds__(kB) k_tris(const uint32_t *__restrict__ idx, u
   }
 }
 
-__global__ void __launch_bounds__(kB) k_compact(const Tri *__restrict__ slots, const uint32_t *__restrict__ valid,
-                                                const uint32_t *__restrict__ pos, size_t n_slots,
-                                                Tri *__restrict__ tris) {
-  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n_slots; s += (size_t)gridDim.x * blockDim.x)
-    if (valid[s]) tris[pos[s]] = slots[s];
-}
-
-// The tiles a triangle's box touches (raster.rs:238-251).
-__global__ void __launch_bounds__(kB) k_bin_counts(const Tri *__restrict__ tris, uint32_t n,
-                                                   unsigned long long *__restrict__ counts) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int32_t *b = tris[i].bbox;
-    const uint64_t nx = (uint64_t)((uint32_t)b[2] / kTile - (uint32_t)b[0] / kTile + 1);
-    const uint64_t ny = (uint64_t)((uint32_t)b[3] / kTile - (uint32_t)b[1] / kTile + 1);
-    counts[i] = nx * ny;
+// The tiles a slot's box touches (0 for an empty slot).
+__global__ void __launch_bounds__(kB) k_bin_counts(const Tri *__restrict__ slots, const uint32_t *__restrict__ valid,
+                                                   size_t n, unsigned long long *__restrict__ counts) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint64_t c = 0;
+    if (valid[i]) {
+      const int32_t *b = slots[i].bbox;
+      const uint64_t nx = (uint64_t)((uint32_t)b[2] / kTile - (uint32_t)b[0] / kTile + 1);
+      const uint64_t ny = (uint64_t)((uint32_t)b[3] / kTile - (uint32_t)b[1] / kTile + 1);
+      c = nx * ny;
+    }
+    counts[i] = c;
   }
 }
 
-__global__ void __launch_bounds__(kB) k_bin_keys(const Tri *__restrict__ tris, uint32_t n,
-                                                 const unsigned long long *__restrict__ off, uint32_t tiles_x,
-                                                 unsigned long long *__restrict__ keys,
-                                                 uint32_t *__restrict__ tile_count) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int32_t *b = tris[i].bbox;
+__global__ void __launch_bounds__(kB) k_bin_keys(const Tri *__restrict__ slots, const uint32_t *__restrict__ valid,
+                                                 size_t n, const unsigned long long *__restrict__ off,
+                                                 uint32_t tiles_x, unsigned long long *__restrict__ keys) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (!valid[i]) continue;
+    const int32_t *b = slots[i].bbox;
     unsigned long long j = off[i];
     for (uint32_t ty = (uint32_t)b[1] / kTile; ty <= (uint32_t)b[3] / kTile; ty++)
-      for (uint32_t tx = (uint32_t)b[0] / kTile; tx <= (uint32_t)b[2] / kTile; tx++) {
-        const uint32_t tile = ty * tiles_x + tx;
-        keys[j++] = ((unsigned long long)tile << 32) | i;
-        atomicAdd(&tile_count[tile], 1u);
-      }
+      for (uint32_t tx = (uint32_t)b[0] / kTile; tx <= (uint32_t)b[2] / kTile; tx++)
+        keys[j++] = ((unsigned long long)(ty * tiles_x + tx) << 32) | (unsigned long long)i;
   }
+}
+
+// start[t] = the first sorted key of tile t (n_keys for t = n_tiles): a binary search a tile.
+__global__ void __launch_bounds__(kB) k_tile_bounds(const unsigned long long *__restrict__ sorted, uint32_t n_keys,
+                                                    uint32_t n_tiles, uint32_t *__restrict__ start) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tiles) return;
+  uint32_t lo = 0, hi = n_keys;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if ((uint32_t)(sorted[mid] >> 32) < t) lo = mid + 1; else hi = mid;
+  }
+  start[t] = lo;
+}
+
+__global__ void __launch_bounds__(kB) k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // ---- the atlas (texture.rs) ----
@@ -368,95 +384,113 @@ __device__ __forceinline__ float fog_factor(const lw_quick_sky &q, float x, floa
   return rmax(lw::linear_fog(sph, q.fog_dist[1], q.fog_dist[2]), lw::linear_fog(cyl, q.fog_color[3], q.fog_dist[0]));
 }
 
-// raster.rs:266-357: one 32 x 32 tile, four pixels a lane.
-__global__ void __launch_bounds__(kB) k_tiles(const Tri *__restrict__ tris, const unsigned long long *__restrict__ keys,
+// Whether triangle t covers pixel (px, py) by the tests that do not depend on earlier
+// fragments: the box, the three edges and the depth range (raster.rs:285-306; a NaN depth
+// passes, as there).
+__device__ __forceinline__ bool covers(const Tri &t, uint32_t px, uint32_t py) {
+  if ((int32_t)px < t.bbox[0] || (int32_t)px > t.bbox[2] || (int32_t)py < t.bbox[1] || (int32_t)py > t.bbox[3])
+    return false;
+  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const float w0 = edge(t.v[1], t.v[2], fx, fy);
+  const float w1 = edge(t.v[2], t.v[0], fx, fy);
+  const float w2 = edge(t.v[0], t.v[1], fx, fy);
+  if (w0 < 0.0f || w1 < 0.0f || w2 < 0.0f) return false;
+  const float z = w0 * t.area_inv * t.v[0].z + w1 * t.area_inv * t.v[1].z + w2 * t.area_inv * t.v[2].z;
+  return !(z < 0.0f || z > 1.0f);
+}
+
+// One covering fragment of triangle t at pixel (px, py) (raster.rs:293-354): the depth test,
+// the perspective-correct attributes, the atlas sample, the fog and the blend or the write.
+__device__ void shade(const Tri &t, uint32_t px, uint32_t py, const Atlas &A, const lw_quick_sky &sky,
+                      const V3 fog, float &depth, uint32_t &color) {
+  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const float w0 = edge(t.v[1], t.v[2], fx, fy);
+  const float w1 = edge(t.v[2], t.v[0], fx, fy);
+  const float w2 = edge(t.v[0], t.v[1], fx, fy);
+  const float b0 = w0 * t.area_inv, b1 = w1 * t.area_inv, b2 = w2 * t.area_inv;
+  const float z = b0 * t.v[0].z + b1 * t.v[1].z + b2 * t.v[2].z;
+  const bool see_through = (t.flags & TRI_SEE_THROUGH) != 0u, translucent = (t.flags & TRI_TRANSLUCENT) != 0u;
+  if (z >= depth && !see_through) return;
+  const float iw = b0 * t.v[0].w + b1 * t.v[1].w + b2 * t.v[2].w;
+  const float pw = 1.0f / iw;
+  float at[9];
+  for (int a = 0; a < 9; a++) at[a] = (b0 * t.a[0][a] + b1 * t.a[1][a] + b2 * t.a[2][a]) * pw;
+  // The plane of the UVs at another pixel centre (a GPU helper lane).
+  auto uv_at = [&](float x, float y, float &u, float &v) {
+    const float c0 = edge(t.v[1], t.v[2], x, y) * t.area_inv;
+    const float c1 = edge(t.v[2], t.v[0], x, y) * t.area_inv;
+    const float c2 = edge(t.v[0], t.v[1], x, y) * t.area_inv;
+    const float w = 1.0f / (c0 * t.v[0].w + c1 * t.v[1].w + c2 * t.v[2].w);
+    u = (c0 * t.a[0][0] + c1 * t.a[1][0] + c2 * t.a[2][0]) * w;
+    v = (c0 * t.a[0][1] + c1 * t.a[1][1] + c2 * t.a[2][1]) * w;
+  };
+  float4 tex;
+  if (is_mipmapped(A, at[1])) {
+    // Fine derivatives: the difference across this pixel's 2x2 quad.
+    const float qx = (float)(px & ~1u) + 0.5f, qy = (float)(py & ~1u) + 0.5f;
+    float u1, v1, u0, v0, u2, v2, u3, v3;
+    uv_at(qx + 1.0f, fy, u1, v1);
+    uv_at(qx, fy, u0, v0);
+    uv_at(fx, qy + 1.0f, u2, v2);
+    uv_at(fx, qy, u3, v3);
+    tex = sample_block(A, at[0], at[1], u1 - u0, v1 - v0, u2 - u3, v2 - v3);
+  } else {
+    tex = nearest(A, at[0], at[1]);
+  }
+  V3 rgb = v3(tex.x * at[2], tex.y * at[3], tex.z * at[4]);
+  rgb = lw::lerp3(rgb, fog, fog_factor(sky, at[6], at[7], at[8]));
+  if (translucent) {
+    const float a = tex.w * at[5];
+    if (a <= 0.0f) return;
+    const float4 d = unpack(color);
+    const V3 mix = lw::lerp3(v3(d.x, d.y, d.z), rgb, a);
+    color = pack(mix.x, mix.y, mix.z);
+  } else {
+    if (tex.w < 0.5f) return;
+    depth = z;
+    color = pack(rgb.x, rgb.y, rgb.z);
+  }
+}
+
+// raster.rs:266-372 for one 8 x 8 tile per wave: the sky, then the tile's triangles in order.
+// A lane queues the triangles that cover its pixel; when a queue is full, and at the end, the
+// wave shades every lane's queue in order, so each lane shades its own fragment in lockstep
+// with the others rather than waiting on the few lanes a small triangle covers.
+__global__ void __launch_bounds__(kB) k_tiles(const Tri *__restrict__ slots, const unsigned long long *__restrict__ keys,
                                               const uint32_t *__restrict__ tile_start, const Input q,
-                                              uint32_t tiles_x, uint32_t *__restrict__ out) {
-  const uint32_t ti = blockIdx.x, tx = ti % tiles_x, ty = ti / tiles_x;
-  const uint32_t x0 = tx * kTile, y0 = ty * kTile;
-  const uint32_t tw = min(kTile, q.width - x0), th = min(kTile, q.height - y0);
+                                              uint32_t tiles_x, uint32_t n_tiles, uint32_t *__restrict__ out) {
+  __shared__ uint32_t queue[kWaves][kQueue][64];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t tile = blockIdx.x * kWaves + wave;
+  if (tile >= n_tiles) return;  // the whole wave
+  const uint32_t px = (tile % tiles_x) * kTile + lane % kTile, py = (tile / tiles_x) * kTile + lane / kTile;
+  const bool live = px < q.width && py < q.height;
   const Atlas A{q.atlas, q.mips, q.atlas_w, q.atlas_h, q.levels, q.mip_rows, q.mip_offset, q.log2_fix,
                 q.n_log2_fix};
   const V3 fog = lw::xyz(q.sky.fog_color);
-  uint32_t color[4];
-  float depth[4];
-  uint32_t pxs[4], pys[4];
-  bool live[4];
-  for (int k = 0; k < 4; k++) {
-    const uint32_t p = threadIdx.x + kB * k;
-    pxs[k] = x0 + p % kTile;
-    pys[k] = y0 + p / kTile;
-    live[k] = p % kTile < tw && p / kTile < th;
-    depth[k] = INFINITY;
-    color[k] = 0u;
-    if (live[k]) {
-      const V3 c = lw::qs_background(q.sky, q.sectors, pixel_dir(q.sky, (float)pxs[k], (float)pys[k]), q.atlas,
-                                     q.atlas_w, q.atlas_w * q.atlas_h, q.stars);
-      color[k] = pack(c.x, c.y, c.z);
-    }
+  uint32_t color = 0u;
+  float depth = INFINITY;
+  if (live) {
+    const V3 c = lw::qs_background(q.sky, q.sectors, pixel_dir(q.sky, (float)px, (float)py), q.atlas, q.atlas_w,
+                                   q.atlas_w * q.atlas_h, q.stars);
+    color = pack(c.x, c.y, c.z);
   }
-  for (uint32_t e = tile_start[ti]; e < tile_start[ti + 1]; e++) {
-    const Tri &t = tris[(uint32_t)keys[e]];
-    const uint32_t bx0 = max((uint32_t)t.bbox[0], x0), by0 = max((uint32_t)t.bbox[1], y0);
-    const uint32_t bx1 = min((uint32_t)t.bbox[2], x0 + tw - 1), by1 = min((uint32_t)t.bbox[3], y0 + th - 1);
-    if (bx0 > bx1 || by0 > by1) continue;
-    const bool see_through = (t.flags & TRI_SEE_THROUGH) != 0u, translucent = (t.flags & TRI_TRANSLUCENT) != 0u;
-    for (int k = 0; k < 4; k++) {
-      const uint32_t px = pxs[k], py = pys[k];
-      if (!live[k] || px < bx0 || px > bx1 || py < by0 || py > by1) continue;
-      const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-      const float w0 = edge(t.v[1], t.v[2], fx, fy);
-      const float w1 = edge(t.v[2], t.v[0], fx, fy);
-      const float w2 = edge(t.v[0], t.v[1], fx, fy);
-      if (w0 < 0.0f || w1 < 0.0f || w2 < 0.0f) continue;
-      const float b0 = w0 * t.area_inv, b1 = w1 * t.area_inv, b2 = w2 * t.area_inv;
-      const float z = b0 * t.v[0].z + b1 * t.v[1].z + b2 * t.v[2].z;
-      // The open depth comparisons reject a NaN.
-      if ((z >= depth[k] && !see_through) || z < 0.0f || z > 1.0f) continue;
-      const float iw = b0 * t.v[0].w + b1 * t.v[1].w + b2 * t.v[2].w;
-      const float pw = 1.0f / iw;
-      float at[9];
-      for (int a = 0; a < 9; a++) at[a] = (b0 * t.a[0][a] + b1 * t.a[1][a] + b2 * t.a[2][a]) * pw;
-      // The plane of the UVs at another pixel centre (a GPU helper lane).
-      auto uv_at = [&](float x, float y, float &u, float &v) {
-        const float c0 = edge(t.v[1], t.v[2], x, y) * t.area_inv;
-        const float c1 = edge(t.v[2], t.v[0], x, y) * t.area_inv;
-        const float c2 = edge(t.v[0], t.v[1], x, y) * t.area_inv;
-        const float w = 1.0f / (c0 * t.v[0].w + c1 * t.v[1].w + c2 * t.v[2].w);
-        u = (c0 * t.a[0][0] + c1 * t.a[1][0] + c2 * t.a[2][0]) * w;
-        v = (c0 * t.a[0][1] + c1 * t.a[1][1] + c2 * t.a[2][1]) * w;
-      };
-      float4 tex;
-      if (is_mipmapped(A, at[1])) {
-        // Fine derivatives: the difference across this pixel's 2x2 quad.
-        const float qx = (float)(px & ~1u) + 0.5f, qy = (float)(py & ~1u) + 0.5f;
-        float u1, v1, u0, v0, u2, v2, u3, v3;
-        uv_at(qx + 1.0f, fy, u1, v1);
-        uv_at(qx, fy, u0, v0);
-        uv_at(fx, qy + 1.0f, u2, v2);
-        uv_at(fx, qy, u3, v3);
-        tex = sample_block(A, at[0], at[1], u1 - u0, v1 - v0, u2 - u3, v2 - v3);
-      } else {
-        tex = nearest(A, at[0], at[1]);
-      }
-      V3 rgb = v3(tex.x * at[2], tex.y * at[3], tex.z * at[4]);
-      const float ff = fog_factor(q.sky, at[6], at[7], at[8]);
-      rgb = lw::lerp3(rgb, fog, ff);
-      if (translucent) {
-        const float a = tex.w * at[5];
-        if (a <= 0.0f) continue;
-        const float4 d = unpack(color[k]);
-        const V3 mix = lw::lerp3(v3(d.x, d.y, d.z), rgb, a);
-        color[k] = pack(mix.x, mix.y, mix.z);
-      } else {
-        if (tex.w < 0.5f) continue;
-        depth[k] = z;
-        color[k] = pack(rgb.x, rgb.y, rgb.z);
-      }
+  uint32_t e = tile_start[tile];
+  const uint32_t end = tile_start[tile + 1];
+  uint32_t qlen = 0;
+  for (;;) {
+    while (e < end) {
+      const uint32_t s = __builtin_amdgcn_readfirstlane((uint32_t)keys[e]);
+      e++;
+      if (live && covers(slots[s], px, py)) queue[wave][qlen++][lane] = s;
+      if (__ballot(qlen == kQueue) != 0ull) break;
     }
+    for (uint32_t j = 0; __ballot(j < qlen) != 0ull; j++)
+      if (j < qlen) shade(slots[queue[wave][j][lane]], px, py, A, q.sky, fog, depth, color);
+    qlen = 0;
+    if (e >= end) break;
   }
-  for (int k = 0; k < 4; k++)
-    if (live[k]) out[(size_t)pys[k] * q.width + pxs[k]] = color[k];
+  if (live) out[(size_t)py * q.width + px] = color;
 }
 
 unsigned grid_for(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + kB - 1) / kB, 1u << 20)); }
@@ -470,34 +504,36 @@ uint32_t bit_width(uint64_t v) {
   return b;
 }
 
-template <class In, class Out, class Acc>
-void exclusive_scan(const In *in, Out *out, size_t n, Acc init, hipStream_t s) {
-  size_t bytes = 0;
-  LWQ_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, init, n, rocprim::plus<Acc>(), s));
-  Buf<char> tmp;
-  tmp.alloc(std::max<size_t>(bytes, 1));
-  LWQ_TRY(rocprim::exclusive_scan(tmp.p, bytes, in, out, init, n, rocprim::plus<Acc>(), s));
-  LWQ_TRY(hipStreamSynchronize(s));
-}
-
-template <class T>
-T read_one(const T *p, hipStream_t s) {
-  T v;
-  LWQ_TRY(hipMemcpyAsync(&v, p, sizeof(T), hipMemcpyDeviceToHost, s));
-  LWQ_TRY(hipStreamSynchronize(s));
-  return v;
-}
-
 }  // namespace
 
-void render(hipStream_t s, const Input &in, uint32_t *out, Stats &st) {
+// Grow-only device buffers kept between calls.
+struct Workspace {
+  Buf<ClipVert> verts;
+  Buf<Tri> slots;
+  Buf<uint32_t> valid, tile_start;
+  Buf<unsigned long long> counts, off, keys, sorted;
+  Buf<char> temp;
+};
+
+Workspace *workspace_new() { return new Workspace(); }
+void workspace_free(Workspace *ws) { delete ws; }
+
+namespace {
+template <class T>
+T *grow(Buf<T> &b, size_t n) {
+  if (n > b.n) b.alloc(n);
+  return b.p;
+}
+}  // namespace
+
+void render(hipStream_t s, const Input &in, Workspace &ws, uint32_t *out, Stats &st) {
   st = Stats{};
   const uint64_t n_tri = (uint64_t)in.n_opaque + in.n_translucent + in.n_see_through;
   if (n_tri >= (1ull << 30)) throw Error{true, hipSuccess, "too many triangles"};
   const uint32_t tiles_x = (in.width + kTile - 1) / kTile, tiles_y = (in.height + kTile - 1) / kTile;
   const uint64_t n_tiles = (uint64_t)tiles_x * tiles_y;
+  const size_t n_slots = 2 * (size_t)n_tri;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  LWQ_TRY(hipEventCreate(&ev0));
   struct Events {
     hipEvent_t *a, *b;
     ~Events() {
@@ -505,65 +541,63 @@ void render(hipStream_t s, const Input &in, uint32_t *out, Stats &st) {
       if (*b) (void)hipEventDestroy(*b);
     }
   } events{&ev0, &ev1};
+  LWQ_TRY(hipEventCreate(&ev0));
   LWQ_TRY(hipEventCreate(&ev1));
+  // Every buffer up to the sorted keys' size is known before the first launch.
+  ClipVert *verts = grow(ws.verts, std::max<size_t>(in.n_verts, 1));
+  Tri *slots = grow(ws.slots, std::max<size_t>(n_slots, 1));
+  uint32_t *valid = grow(ws.valid, std::max<size_t>(n_slots, 1));
+  unsigned long long *counts = grow(ws.counts, std::max<size_t>(n_slots, 1));
+  unsigned long long *off = grow(ws.off, std::max<size_t>(n_slots, 1));
+  uint32_t *tile_start = grow(ws.tile_start, n_tiles + 1);
+  size_t scan_bytes = 0;
+  LWQ_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, counts, off, 0ull, std::max<size_t>(n_slots, 1),
+                                  rocprim::plus<unsigned long long>(), s));
+  grow(ws.temp, std::max<size_t>(scan_bytes, 1));
   LWQ_TRY(hipEventRecord(ev0, s));
-
-  Buf<ClipVert> verts;
-  verts.alloc(std::max<uint32_t>(in.n_verts, 1));
   if (in.n_verts) {
-    k_verts<<<dim3((in.n_verts + kB - 1) / kB), dim3(kB), 0, s>>>(in.verts, in.n_verts, in, verts.p);
+    k_verts<<<dim3((in.n_verts + kB - 1) / kB), dim3(kB), 0, s>>>(in.verts, in.n_verts, in, verts);
     LWQ_TRY(hipGetLastError());
   }
-  // Triangles, two slots each, then compacted in order.
-  Buf<Tri> slots, tris;
-  Buf<uint32_t> valid, pos;
-  uint64_t n_live = 0;
-  if (n_tri) {
-    slots.alloc(2 * n_tri);
-    valid.alloc(2 * n_tri);
-    pos.alloc(2 * n_tri);
-    k_tris<<<dim3((unsigned)((n_tri + kB - 1) / kB)), dim3(kB), 0, s>>>(
-        in.indices, (uint32_t)n_tri, in.n_opaque, in.n_translucent, verts.p, (float)in.width,
-        (float)in.height, slots.p, valid.p);
-    LWQ_TRY(hipGetLastError());
-    exclusive_scan(valid.p, pos.p, 2 * n_tri, 0u, s);
-    n_live = (uint64_t)read_one(pos.p + 2 * n_tri - 1, s) + read_one(valid.p + 2 * n_tri - 1, s);
-    tris.alloc(std::max<uint64_t>(n_live, 1));
-    k_compact<<<dim3(grid_for(2 * n_tri)), dim3(kB), 0, s>>>(slots.p, valid.p, pos.p, 2 * n_tri, tris.p);
-    LWQ_TRY(hipGetLastError());
-  }
-  st.triangles = n_live;
-  // Bins: (tile, triangle) keys sorted; a tile's list is its run, triangles in order.
-  Buf<unsigned long long> counts, off, keys, sorted;
-  Buf<uint32_t> tile_count, tile_start;
-  tile_count.alloc(n_tiles + 1);
-  tile_start.alloc(n_tiles + 1);
-  LWQ_TRY(hipMemsetAsync(tile_count.p, 0, (n_tiles + 1) * sizeof(uint32_t), s));
   uint64_t n_keys = 0;
-  if (n_live) {
-    counts.alloc(n_live);
-    off.alloc(n_live);
-    k_bin_counts<<<dim3(grid_for(n_live)), dim3(kB), 0, s>>>(tris.p, (uint32_t)n_live, counts.p);
+  if (n_tri) {
+    // Triangles into two slots each (slot order is triangle order), their tile counts and
+    // the keys' offsets; the key count is the one value read back before the last launches.
+    k_tris<<<dim3((unsigned)((n_tri + kB - 1) / kB)), dim3(kB), 0, s>>>(
+        in.indices, (uint32_t)n_tri, in.n_opaque, in.n_translucent, verts, (float)in.width, (float)in.height,
+        slots, valid);
     LWQ_TRY(hipGetLastError());
-    exclusive_scan(counts.p, off.p, n_live, 0ull, s);
-    n_keys = read_one(off.p + n_live - 1, s) + read_one(counts.p + n_live - 1, s);
-    if (n_keys >= (1ull << 31)) throw Error{true, hipSuccess, "triangles cover too many tiles"};
-    keys.alloc(n_keys);
-    sorted.alloc(n_keys);
-    k_bin_keys<<<dim3(grid_for(n_live)), dim3(kB), 0, s>>>(tris.p, (uint32_t)n_live, off.p, tiles_x, keys.p,
-                                                          tile_count.p);
+    k_bin_counts<<<dim3(grid_for(n_slots)), dim3(kB), 0, s>>>(slots, valid, n_slots, counts);
     LWQ_TRY(hipGetLastError());
+    LWQ_TRY(rocprim::exclusive_scan(ws.temp.p, scan_bytes, counts, off, 0ull, n_slots,
+                                    rocprim::plus<unsigned long long>(), s));
+    unsigned long long tail[2];
+    LWQ_TRY(hipMemcpyAsync(&tail[0], off + n_slots - 1, sizeof(tail[0]), hipMemcpyDeviceToHost, s));
+    LWQ_TRY(hipMemcpyAsync(&tail[1], counts + n_slots - 1, sizeof(tail[1]), hipMemcpyDeviceToHost, s));
+    LWQ_TRY(hipStreamSynchronize(s));
+    n_keys = tail[0] + tail[1];
+  }
+  if (n_keys >= (1ull << 32)) throw Error{true, hipSuccess, "triangles cover too many tiles"};
+  st.bin_entries = n_keys;
+  if (n_keys) {
+    unsigned long long *keys = grow(ws.keys, n_keys), *sorted = grow(ws.sorted, n_keys);
+    k_bin_keys<<<dim3(grid_for(n_slots)), dim3(kB), 0, s>>>(slots, valid, n_slots, off, tiles_x, keys);
+    LWQ_TRY(hipGetLastError());
+    // The keys are written in slot order, and the radix sort is stable: sorting the tile bits
+    // alone keeps each tile's slots in order.
     size_t bytes = 0;
     const uint32_t end_bit = 32 + std::max(1u, bit_width(n_tiles - 1));
-    LWQ_TRY(rocprim::radix_sort_keys(nullptr, bytes, keys.p, sorted.p, n_keys, 0, end_bit, s));
-    Buf<char> tmp;
-    tmp.alloc(std::max<size_t>(bytes, 1));
-    LWQ_TRY(rocprim::radix_sort_keys(tmp.p, bytes, keys.p, sorted.p, n_keys, 0, end_bit, s));
-    LWQ_TRY(hipStreamSynchronize(s));
+    LWQ_TRY(rocprim::radix_sort_keys(nullptr, bytes, keys, sorted, n_keys, 32, end_bit, s));
+    LWQ_TRY(rocprim::radix_sort_keys(grow(ws.temp, std::max<size_t>(bytes, 1)), bytes, keys, sorted, n_keys, 32,
+                                     end_bit, s));
+    k_tile_bounds<<<dim3((unsigned)((n_tiles + kB) / kB)), dim3(kB), 0, s>>>(sorted, (uint32_t)n_keys,
+                                                                             (uint32_t)n_tiles, tile_start);
+  } else {
+    k_fill_u32<<<dim3(grid_for(n_tiles + 1)), dim3(kB), 0, s>>>(tile_start, n_tiles + 1, 0u);
   }
-  st.bin_entries = n_keys;
-  exclusive_scan(tile_count.p, tile_start.p, n_tiles + 1, 0u, s);
-  k_tiles<<<dim3((unsigned)n_tiles), dim3(kB), 0, s>>>(tris.p, sorted.p, tile_start.p, in, tiles_x, out);
+  LWQ_TRY(hipGetLastError());
+  k_tiles<<<dim3((unsigned)((n_tiles + kWaves - 1) / kWaves)), dim3(kB), 0, s>>>(
+      slots, ws.sorted.p, tile_start, in, tiles_x, (uint32_t)n_tiles, out);
   LWQ_TRY(hipGetLastError());
   LWQ_TRY(hipEventRecord(ev1, s));
   LWQ_TRY(hipStreamSynchronize(s));

@@ -43,11 +43,16 @@ struct Input {
 };
 
 struct Stats {
-  uint64_t triangles = 0, opaque = 0, bin_entries = 0;
-  double kernel_ms = 0.0;
+  uint64_t bin_entries = 0;  // (tile, triangle) pairs
+  double kernel_ms = 0.0;    // the device time of the call's launches, inputs resident
 };
 
+// Device buffers a backend keeps between calls (grow-only); workspace_free releases them.
+struct Workspace;
+Workspace *workspace_new();
+void workspace_free(Workspace *ws);
+
 // Renders into out (width * height packed RGBA8 words on the device) and waits for it.
-void render(hipStream_t stream, const Input &in, uint32_t *out, Stats &stats);
+void render(hipStream_t stream, const Input &in, Workspace &ws, uint32_t *out, Stats &stats);
 
 }  // namespace lwq
