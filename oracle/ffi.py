@@ -119,6 +119,10 @@ def lib():
         L.lwo_rasterize.restype = i32
         L.lwo_atlas_sample.argtypes = [vp, vp, vp, u32, u32, i32, vp, sz, vp]
         L.lwo_atlas_sample.restype = None
+        L.lwo_shadow.argtypes = [vp, sz, vp, vp, vp, vp]
+        L.lwo_shadow.restype = i32
+        L.lwo_sun_texture.argtypes = [vp, sz, vp, vp]
+        L.lwo_sun_texture.restype = i32
         L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
         L.lwo_build_layout.restype = i32
         L.lwo_build_fetch.argtypes = [vp, vp, vp]
@@ -234,6 +238,24 @@ def trace(view, o, d, tmin=None, tmax=None):
     tmax = _f32(np.full(n, np.inf) if tmax is None else np.broadcast_to(tmax, (n,)))
     out = np.zeros(n, dtype=HIT)
     _check(lib().lwo_trace(view.ref, n, _addr(o), _addr(d), _addr(tmin), _addr(tmax), _addr(out)))
+    return out
+
+
+def shadow(view, o, d, tmax):
+    """shadow (integrator.rs:316-346): the transmittance along rays o, d (n, 3) up to tmax."""
+    o = _f32(o).reshape(-1, 3)
+    d = _f32(d).reshape(-1, 3)
+    tm = _f32(np.broadcast_to(tmax, (o.shape[0],)))
+    out = np.zeros_like(o)
+    _check(lib().lwo_shadow(view.ref, o.shape[0], _addr(o), _addr(d), _addr(tm), _addr(out)))
+    return out
+
+
+def sun_texture(view, d):
+    """sun_texture (integrator.rs:252-277) along directions d (n, 3)."""
+    d = _f32(d).reshape(-1, 3)
+    out = np.zeros_like(d)
+    _check(lib().lwo_sun_texture(view.ref, d.shape[0], _addr(d), _addr(out)))
     return out
 
 

@@ -258,6 +258,31 @@ uint32_t lwo_water_interface(const Params *p, const float *pos, const float *d, 
 }
 
 
+// shadow (integrator.rs:316-346): the transmittance from o along d up to tmax, for n rays.
+int lwo_shadow(const View *view, size_t n, const float *o, const float *d, const float *tmax, float *out) {
+  return guarded([&] {
+    for (size_t i = 0; i < n; i++) {
+      const V3 t = shadow(*view, v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
+                          tmax[i]);
+      out[3 * i] = t.x;
+      out[3 * i + 1] = t.y;
+      out[3 * i + 2] = t.z;
+    }
+  });
+}
+
+// sun_texture (integrator.rs:252-277) along n directions.
+int lwo_sun_texture(const View *view, size_t n, const float *d, float *out) {
+  return guarded([&] {
+    for (size_t i = 0; i < n; i++) {
+      const V3 c = sun_texture(*view, v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+      out[3 * i] = c.x;
+      out[3 * i + 1] = c.y;
+      out[3 * i + 2] = c.z;
+    }
+  });
+}
+
 // Differentiable rendering's CPU reference (lw_oracle_diff.hpp): the image under the
 // parameters of a lw_diff_desc, and the gradient of sum(dldi * image).
 int lwo_render_diff(const View *view, uint32_t spp, const DiffDesc *diff, unsigned threads,
