@@ -57,8 +57,9 @@ extern "C" {
 #define LW_ENODEV (-2)  /* no HIP device / runtime (cuda.rs:29-41 maps this to an error) */
 #define LW_EHIP (-3)    /* a HIP runtime call failed */
 #define LW_ENOMEM (-4)  /* device allocation failed */
-#define LW_EUNSUPPORTED (-5) /* a valid request this backend does not serve (the differentiable
-                                 calls with LW_EMITTER_SAMPLING_ALL) */
+#define LW_EUNSUPPORTED (-5) /* a valid request this backend does not serve: differentiable
+                                 rendering with emitter sampling ONE_BLOCK or ALL, sky parameters
+                                 with sky maps or the sun sprite, a fit over the sky parameters */
 
 /* GpuQuad, 80 bytes (scene.rs:44-57): p0.w = flags, eu.w = tint rgba8, ev.w = emittance. */
 typedef struct lw_quad {
