@@ -1108,7 +1108,7 @@ struct lw_hip_backend {
   int mode = 1;
   // LW_WAVES, LW_TRACE_WAVES: register budgets (see dispatch). The direct form measured
   // fastest at 6 waves per SIMD (80 VGPRs): 564 Msamples/s at the 720p 256-spp frame
-  // against 528 at 5 and 412 at 8. The persistent form, which has no 6-wave variant, then
+  // against 528 at 5, 516 at 7 and 412 at 8. The persistent form, which has no 6-wave variant, then
   // runs at its own best, 4.
   int waves = 6, trace_waves = 1;
   // LW_XCD_REMAP=1: the direct kernel gives each XCD a contiguous run of tiles.
