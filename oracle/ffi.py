@@ -123,6 +123,10 @@ def lib():
         L.lwo_shadow.restype = i32
         L.lwo_sun_texture.argtypes = [vp, sz, vp, vp]
         L.lwo_sun_texture.restype = i32
+        L.lwo64_render_hq.argtypes = [vp, u32, ctypes.c_uint, vp]
+        L.lwo64_render_hq.restype = i32
+        L.lwo64_render_samples.argtypes = [vp, sz, vp, vp, vp, vp]
+        L.lwo64_render_samples.restype = i32
         L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
         L.lwo_build_layout.restype = i32
         L.lwo_build_fetch.argtypes = [vp, vp, vp]
@@ -329,6 +333,26 @@ def render_hq(view, spp: int, threads: int = 0):
     out = np.zeros((h, w, 3), dtype="<f4")
     threads = threads or os.cpu_count() or 1
     _check(lib().lwo_render_hq(view.ref, spp, threads, _addr(out)))
+    return out
+
+
+def render_hq_f64(view, spp: int, threads: int = 0):
+    """CpuBackend::render_hq with every operation in f64 and libm's double functions
+    (lw_oracle_f64.cpp): (h, w, 3) float64, as render_hq's shape."""
+    w, h = int(view.params["frame"][0]), int(view.params["frame"][1])
+    out = np.zeros((h, w, 3), dtype=np.float64)
+    rc = lib().lwo64_render_hq(view.ref, spp, threads or os.cpu_count() or 1, _addr(out))
+    if rc != 0:
+        raise RuntimeError("oracle: f64 render failed")
+    return out
+
+
+def render_samples_f64(view, px, py, sample):
+    """render_sample in f64 for (px, py, sample) triples: (n, 3) float64."""
+    px, py, sm = (np.ascontiguousarray(a, dtype="<u4") for a in (px, py, sample))
+    out = np.zeros((px.size, 3), dtype=np.float64)
+    if lib().lwo64_render_samples(view.ref, px.size, _addr(px), _addr(py), _addr(sm), _addr(out)) != 0:
+        raise RuntimeError("oracle: f64 samples failed")
     return out
 
 

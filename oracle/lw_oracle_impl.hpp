@@ -11,9 +11,11 @@
 //   * none in the arithmetic: exp is glibc's expf, which Rust std's f32::exp calls;
 //   * an out-of-bounds slice index, a panic in Rust, throws OracleError here.
 //
-// This file is included twice: by lw_oracle.cpp (LWO_COUNTING 0) and by
-// lw_oracle_counted.cpp (LWO_COUNTING 1), which adds the algorithmic-byte counters
-// of the measurement rules (8 B per DDA step, 80 B per quad tested, ...).
+// This file is included three times: by lw_oracle.cpp (LWO_COUNTING 0), by
+// lw_oracle_counted.cpp (LWO_COUNTING 1), which adds the algorithmic-byte counters of the
+// measurement rules (8 B per DDA step, 80 B per quad tested, ...), and by lw_oracle_f64.cpp
+// (LWO_F64 1), where `real` is double and the transcendental functions are libm's double
+// ones: the f64 render against which the f32 path's rounding error is stated.
 //
 // Parity pins (tests/test_oracle_*.py): the reference's own known-answer tests in
 // crates/lumenwright-kernel/src/tests.rs:128-831 (exact hits, emission 6.5, sky maps,
@@ -122,25 +124,36 @@ static inline uint32_t pcg_hash(uint32_t v) {
   uint32_t word = ((state >> ((state >> 28) + 4u)) ^ state) * 277803737u;
   return (word >> 22) ^ word;
 }
+// The arithmetic type: float as the reference computes, or double for the f64 render that
+// states the f32 path's own rounding error (LWO_F64).
+#ifndef LWO_F64
+#define LWO_F64 0
+#endif
+#if LWO_F64
+typedef double real;
+#else
+typedef float real;
+#endif
+
 struct Rng {
   uint32_t state;
   static Rng make(uint32_t pixel, uint32_t sample, uint32_t seed) {
     return Rng{pcg_hash(pixel ^ pcg_hash(sample + pcg_hash(seed)))};
   }
-  float next() {
+  real next() {
     state = pcg_hash(state);
-    return (float)(state >> 8) * (1.0f / 16777216.0f);
+    return (real)(state >> 8) * (real)(1.0f / 16777216.0f);
   }
 };
 
 // ---- Rust `as` casts: truncate toward zero, saturate, NaN -> 0 ----
-static inline int32_t as_i32(float x) {
+static inline int32_t as_i32(real x) {
   if (x != x) return 0;
   if (x >= 2147483648.0f) return INT32_MAX;
   if (x < -2147483648.0f) return INT32_MIN;
   return (int32_t)x;
 }
-static inline uint32_t as_u32(float x) {
+static inline uint32_t as_u32(real x) {
   if (x != x) return 0;
   if (x >= 4294967296.0f) return UINT32_MAX;
   if (x <= 0.0f) return 0;
@@ -158,6 +171,33 @@ static inline uint32_t f_bits(float f) {
 }
 
 // ---- math.rs ----
+#if LWO_F64
+// The f64 render: every operation in double, the transcendental functions libm's double
+// ones in place of the reference's f32 approximations; the constants are the reference's.
+static const real PI_F = 3.14159265358979323846;
+static const real FRAC_PI_2_F = 1.57079632679489661923;
+static const real SQRT_2_F = 1.41421356237309504880;
+static const real LN_2_F = 0.693147180559945309417;
+static const real GAMMA = 2.2f;
+
+static inline real f_sqrt(real x) { return std::sqrt(x); }
+static inline real f_floor(real x) { return std::floor(x); }
+static inline real f_exp(real x) { return std::exp(x); }
+static inline real f_abs(real x) { return x < 0.0 ? -x : x; }
+static inline real f_min(real a, real b) { return a < b ? a : b; }
+static inline real f_max(real a, real b) { return a > b ? a : b; }
+static inline real f_clamp(real x, real lo, real hi) { return f_min(f_max(x, lo), hi); }
+static inline real f_rem(real a, real b) { return std::fmod(a, b); }
+static inline void sin_cos_turns(real u, real &s_out, real &c_out) {
+  const real a = u * (2.0 * 3.14159265358979323846);
+  s_out = std::sin(a);
+  c_out = std::cos(a);
+}
+static inline real f_acos(real x) { return std::acos(f_clamp(x, -1.0, 1.0)); }
+static inline real f_atan2(real y, real x) { return (x == 0.0 && y == 0.0) ? 0.0 : std::atan2(y, x); }
+static inline real f_ln(real x) { return x <= 0.0 ? (x == 0.0 ? -INFINITY : NAN) : std::log(x); }
+static inline real gamma_to_linear(real c) { return c <= 0.0 ? 0.0 : f_exp(GAMMA * f_ln(c)); }
+#else
 static const float PI_F = 3.14159265358979323846f;
 static const float FRAC_PI_2_F = 1.57079632679489661923f;
 static const float SQRT_2_F = 1.41421356237309504880f;
@@ -256,41 +296,42 @@ static inline float f_ln(float x) {
 // math.rs:213-216
 static inline float gamma_to_linear(float c) { return c <= 0.0f ? 0.0f : f_exp(GAMMA * f_ln(c)); }
 
+#endif  // LWO_F64
 // math.rs:218-344
 struct V3 {
-  float x, y, z;
+  real x, y, z;
 };
-static inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+static inline V3 v3(real x, real y, real z) { return V3{x, y, z}; }
 static inline V3 from4(const float a[4]) { return V3{a[0], a[1], a[2]}; }
 static inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 static inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-static inline V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
-static inline V3 operator/(V3 a, float s) { return a * (1.0f / s); }
+static inline V3 operator*(V3 a, real s) { return V3{a.x * s, a.y * s, a.z * s}; }
+static inline V3 operator/(V3 a, real s) { return a * (1.0f / s); }
 static inline V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
-static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline real dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline V3 cross(V3 a, V3 b) {
   return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-static inline float length(V3 a) { return f_sqrt(dot(a, a)); }
+static inline real length(V3 a) { return f_sqrt(dot(a, a)); }
 static inline V3 normalize(V3 a) {
-  float l = length(a);
+  real l = length(a);
   return l > 0.0f ? a * (1.0f / l) : a;
 }
 static inline V3 mul(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
-static inline float max_elem(V3 a) { return f_max(a.x, f_max(a.y, a.z)); }
-static inline float get(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-static inline V3 transmittance(V3 a, float t) {
+static inline real max_elem(V3 a) { return f_max(a.x, f_max(a.y, a.z)); }
+static inline real get(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static inline V3 transmittance(V3 a, real t) {
   return V3{f_exp(-a.x * t), f_exp(-a.y * t), f_exp(-a.z * t)};
 }
-static inline V3 lerp(V3 a, V3 b, float t) { return a + (b - a) * t; }
+static inline V3 lerp(V3 a, V3 b, real t) { return a + (b - a) * t; }
 static const V3 ZERO3 = {0.0f, 0.0f, 0.0f};
 static const V3 ONE3 = {1.0f, 1.0f, 1.0f};
 
 // math.rs:347-356 (Duff et al. 2017)
 static inline void onb(V3 n, V3 &t_out, V3 &b_out) {
-  float sign = n.z >= 0.0f ? 1.0f : -1.0f;
-  float a = -1.0f / (sign + n.z);
-  float b = n.x * n.y * a;
+  real sign = n.z >= 0.0f ? 1.0f : -1.0f;
+  real a = -1.0f / (sign + n.z);
+  real b = n.x * n.y * a;
   t_out = V3{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
   b_out = V3{b, sign + n.y * n.y * a, -n.y};
 }
@@ -308,7 +349,7 @@ static const uint32_t EMITTER_GRID_HEADER = 8u;
 
 static inline uint32_t q_flags(const Quad &q) { return f_bits(q.p0[3]); }
 static inline uint32_t q_tint(const Quad &q) { return f_bits(q.eu[3]); }
-static inline float q_emittance(const Quad &q) { return q.ev[3]; }
+static inline real q_emittance(const Quad &q) { return q.ev[3]; }
 static inline V3 q_normal(const Quad &q) { return cross(from4(q.ev), from4(q.eu)); }
 
 static inline uint32_t grid_at(const View &s, size_t i) {
@@ -321,11 +362,11 @@ static inline const Quad &quad_at(const View &s, size_t i) {
 // scene.rs:264-267
 static inline V3 grid_size(const View &s) {
   const uint32_t *d = s.params->grid_dims;
-  return V3{(float)(d[0] * 16u), (float)(d[1] * 16u), (float)(d[2] * 16u)};
+  return V3{(real)(d[0] * 16u), (real)(d[1] * 16u), (real)(d[2] * 16u)};
 }
 
 // scene.rs:288-294
-static inline uint32_t texel(const View &s, float u, float v) {
+static inline uint32_t texel(const View &s, real u, real v) {
   int32_t w = (int32_t)s.params->off2[1];
   int32_t h = (int32_t)s.params->off2[2];
   int32_t x = as_i32(f_floor(u));
@@ -338,37 +379,37 @@ static inline uint32_t texel(const View &s, float u, float v) {
 }
 
 // scene.rs:299-306
-static inline void unpack_rgba(uint32_t c, float out[4]) {
-  out[0] = (float)(c & 0xFFu) / 255.0f;
-  out[1] = (float)((c >> 8) & 0xFFu) / 255.0f;
-  out[2] = (float)((c >> 16) & 0xFFu) / 255.0f;
-  out[3] = (float)((c >> 24) & 0xFFu) / 255.0f;
+static inline void unpack_rgba(uint32_t c, real out[4]) {
+  out[0] = (real)(c & 0xFFu) / 255.0f;
+  out[1] = (real)((c >> 8) & 0xFFu) / 255.0f;
+  out[2] = (real)((c >> 16) & 0xFFu) / 255.0f;
+  out[3] = (real)((c >> 24) & 0xFFu) / 255.0f;
 }
 
 // ---- trace.rs ----
 struct Hit {
-  float t;
+  real t;
   uint32_t quad, texel;
-  float s, tt;
+  real s, tt;
 };
 static const Hit NO_HIT = {INFINITY, UINT32_MAX, 0u, 0.0f, 0.0f};
 static const uint32_t MAX_STEPS = 65536u;
-static const float COPLANAR = 1e-5f;
+static const real COPLANAR = 1e-5f;
 
 // trace.rs:37-58
-static inline bool intersect_quad(const Quad &q, V3 o, V3 d, float tmin, float tmax, float &t_out,
-                                  float &s_out, float &tt_out) {
+static inline bool intersect_quad(const Quad &q, V3 o, V3 d, real tmin, real tmax, real &t_out,
+                                  real &s_out, real &tt_out) {
   V3 eu = from4(q.eu), ev = from4(q.ev);
   V3 n = cross(eu, ev);
-  float denom = dot(n, d);
+  real denom = dot(n, d);
   if (f_abs(denom) < 1e-12f) return false;
   V3 p0 = from4(q.p0);
-  float t = dot(n, p0 - o) / denom;
+  real t = dot(n, p0 - o) / denom;
   if (!(t > tmin && t < tmax)) return false;
   V3 p = o + d * t - p0;
-  float nn = dot(n, n);
-  float s = dot(cross(p, ev), n) / nn;
-  float tt = dot(cross(eu, p), n) / nn;
+  real nn = dot(n, n);
+  real s = dot(cross(p, ev), n) / nn;
+  real tt = dot(cross(eu, p), n) / nn;
   if (s < 0.0f || s > 1.0f || tt < 0.0f || tt > 1.0f) return false;
   t_out = t;
   s_out = s;
@@ -377,21 +418,21 @@ static inline bool intersect_quad(const Quad &q, V3 o, V3 d, float tmin, float t
 }
 
 // trace.rs:62-69
-static inline uint32_t quad_texel(const View &s, const Quad &q, float ss, float t) {
+static inline uint32_t quad_texel(const View &s, const Quad &q, real ss, real t) {
   ss = f_min(f_max(ss, 0.0005f), 0.9995f);
   t = f_min(f_max(t, 0.0005f), 0.9995f);
-  float u = q.uv[0] + ss * q.uv[2] + t * q.uvt[0];
-  float v = q.uv[1] + ss * q.uv[3] + t * q.uvt[1];
+  real u = q.uv[0] + ss * q.uv[2] + t * q.uvt[0];
+  real v = q.uv[1] + ss * q.uv[3] + t * q.uvt[1];
   return texel(s, u, v);
 }
 
 // trace.rs:81-109
-static inline void test_quad(const View &s, uint32_t qi, V3 o, V3 d, float tmin, Hit &best) {
+static inline void test_quad(const View &s, uint32_t qi, V3 o, V3 d, real tmin, Hit &best) {
   const Quad &q = quad_at(s, qi);
   LWO_COUNT(bytes, 80);
   LWO_COUNT(quads_tested, 1);
-  float reach = (best.quad != UINT32_MAX && qi > best.quad) ? best.t + COPLANAR : best.t;
-  float t, ss, tt;
+  real reach = (best.quad != UINT32_MAX && qi > best.quad) ? best.t + COPLANAR : best.t;
+  real t, ss, tt;
   if (intersect_quad(q, o, d, tmin, reach, t, ss, tt)) {
     if ((q_flags(q) & FLAG_FRONT_ONLY) != 0 && dot(q_normal(q), d) >= 0.0f) return;
     uint32_t tx = quad_texel(s, q, ss, tt);
@@ -409,7 +450,7 @@ static inline bool is_see_through(const View &s, uint32_t qi) {
 }
 
 // trace.rs:120-131
-static inline bool see_through_hit(const View &s, V3 o, V3 d, float tmin, Hit &out) {
+static inline bool see_through_hit(const View &s, V3 o, V3 d, real tmin, Hit &out) {
   const uint32_t *st = s.params->see_through;
   Hit best = NO_HIT;
   for (uint32_t i = 0; i < st[1]; i++) test_quad(s, st[0] + i, o, d, tmin, best);
@@ -421,14 +462,14 @@ static inline bool see_through_hit(const View &s, V3 o, V3 d, float tmin, Hit &o
 }
 
 // trace.rs:293-319
-static inline float safe_inv(float x) {
+static inline real safe_inv(real x) {
   if (f_abs(x) < 1e-12f) return x < 0.0f ? -1e12f : 1e12f;
   return 1.0f / x;
 }
-static inline int32_t sign_i(float x) { return x > 0.0f ? 1 : (x < 0.0f ? -1 : 0); }
-static inline float next_boundary(float o, float inv, int32_t cell, int32_t step) {
+static inline int32_t sign_i(real x) { return x > 0.0f ? 1 : (x < 0.0f ? -1 : 0); }
+static inline real next_boundary(real o, real inv, int32_t cell, int32_t step) {
   if (step == 0) return INFINITY;
-  float bound = (float)(step > 0 ? cell + 1 : cell);
+  real bound = (real)(step > 0 ? cell + 1 : cell);
   return (bound - o) * inv;
 }
 
@@ -438,7 +479,7 @@ static inline int32_t clamp_i(int32_t v, int32_t lo, int32_t hi) {
 }
 
 // trace.rs:137-290; cell_out receives the cell the traversal stopped in.
-static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
+static bool trace(const View &s, V3 o, V3 d, real tmin, real tmax, Hit &out,
                   int32_t cell_out[3]) {
   const Params &p = *s.params;
   const uint32_t *dims = p.grid_dims;
@@ -448,10 +489,10 @@ static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
   V3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
   V3 ta = mul(ZERO3 - o, inv);
   V3 tb = mul(size - o, inv);
-  float t_enter = f_max(f_max(f_min(ta.x, tb.x), f_min(ta.y, tb.y)), f_min(ta.z, tb.z));
-  float t_exit = f_min(f_min(f_max(ta.x, tb.x), f_max(ta.y, tb.y)), f_max(ta.z, tb.z));
-  float t = f_max(t_enter, 0.0f);
-  float t_end = f_min(t_exit, tmax);
+  real t_enter = f_max(f_max(f_min(ta.x, tb.x), f_min(ta.y, tb.y)), f_min(ta.z, tb.z));
+  real t_exit = f_min(f_min(f_max(ta.x, tb.x), f_max(ta.y, tb.y)), f_max(ta.z, tb.z));
+  real t = f_max(t_enter, 0.0f);
+  real t_end = f_min(t_exit, tmax);
   int32_t cell[3] = {0, 0, 0};
   cell_out[0] = cell_out[1] = cell_out[2] = -1;
   if (t > t_end) return false;
@@ -462,12 +503,12 @@ static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
   cell[1] = clamp_i(as_i32(f_floor(start.y)), 0, gs[1] - 1);
   cell[2] = clamp_i(as_i32(f_floor(start.z)), 0, gs[2] - 1);
   int32_t step[3] = {sign_i(d.x), sign_i(d.y), sign_i(d.z)};
-  float next[3];
+  real next[3];
   for (int a = 0; a < 3; a++) next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
 
   uint32_t top_off = p.off[0];
   uint32_t ov_top_off = p.off[2];
-  float best_t = t_end + 1e-3f;
+  real best_t = t_end + 1e-3f;
 
   for (uint32_t it = 0; it < MAX_STEPS; it++) {
     uint32_t bx = (uint32_t)(cell[0] >> 4);
@@ -485,12 +526,12 @@ static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
     if (entry == EMPTY && ov == EMPTY) {
       // Skip the whole brick: jump to its exit face.
       int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
-      float t_out = INFINITY;
+      real t_out = INFINITY;
       int axis = 0;
       for (int a = 0; a < 3; a++) {
         if (step[a] != 0) {
           int32_t bound = step[a] > 0 ? (bc[a] + 1) * 16 : bc[a] * 16;
-          float tt = ((float)bound - get(o, a)) * get(inv, a);
+          real tt = ((real)bound - get(o, a)) * get(inv, a);
           if (tt < t_out) {
             t_out = tt;
             axis = a;
@@ -512,9 +553,9 @@ static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
       continue;
     }
 
-    float t_cell_exit = f_min(next[0], f_min(next[1], next[2]));
-    float t_lo = f_max(t - 1e-4f, tmin);
-    float t_hi = t_cell_exit + 1e-4f;
+    real t_cell_exit = f_min(next[0], f_min(next[1], next[2]));
+    real t_lo = f_max(t - 1e-4f, tmin);
+    real t_hi = t_cell_exit + 1e-4f;
 
     uint32_t model;
     if (entry == EMPTY) {
@@ -533,7 +574,7 @@ static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
     if (model != 0) {
       const Model &m = at(s.models, s.models_len, model, "models");
       LWO_COUNT(bytes, 16);
-      V3 lo = o - v3((float)cell[0], (float)cell[1], (float)cell[2]);
+      V3 lo = o - v3((real)cell[0], (real)cell[1], (real)cell[2]);
       for (uint32_t i = 0; i < m.quad_count; i++) test_quad(s, m.quad_offset + i, lo, d, t_lo, cand);
     }
     if (ov != EMPTY) {
@@ -571,7 +612,7 @@ static bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out,
   return false;
 }
 
-static inline bool trace(const View &s, V3 o, V3 d, float tmin, float tmax, Hit &out) {
+static inline bool trace(const View &s, V3 o, V3 d, real tmin, real tmax, Hit &out) {
   int32_t cell[3];
   return trace(s, o, d, tmin, tmax, out, cell);
 }
@@ -581,7 +622,7 @@ static const uint32_t MAX_EVENTS = 48u;
 
 // integrator.rs:26-34
 static inline V3 linear_rgb(uint32_t c) {
-  float u[4];
+  real u[4];
   unpack_rgba(c, u);
   return V3{gamma_to_linear(u[0]), gamma_to_linear(u[1]), gamma_to_linear(u[2])};
 }
@@ -590,21 +631,21 @@ static inline V3 linear_rgb(uint32_t c) {
 static V3 preetham_radiance(const View &s, V3 d) {
   const Params &p = *s.params;
   V3 sun = from4(p.sky_sun);
-  float cos_theta = f_max(d.y + p.sky_zenith[3], 0.0f);
-  float cos_gamma = f_clamp(dot(d, sun), -1.0f, 1.0f);
-  float gamma = f_acos(cos_gamma);
-  float cos2_gamma = cos_gamma * cos_gamma;
-  auto perez = [&](int k) -> float {
-    float a = cos_theta > 0.0f ? p.perez_a[k] * f_exp(p.perez_b[k] / cos_theta) : 0.0f;
+  real cos_theta = f_max(d.y + p.sky_zenith[3], 0.0f);
+  real cos_gamma = f_clamp(dot(d, sun), -1.0f, 1.0f);
+  real gamma = f_acos(cos_gamma);
+  real cos2_gamma = cos_gamma * cos_gamma;
+  auto perez = [&](int k) -> real {
+    real a = cos_theta > 0.0f ? p.perez_a[k] * f_exp(p.perez_b[k] / cos_theta) : 0.0f;
     return (1.0f + a) *
            (1.0f + p.perez_c[k] * f_exp(p.perez_d[k] * gamma) + p.perez_e[k] * cos2_gamma);
   };
-  float x = p.sky_zenith[0] * perez(0);
-  float y = p.sky_zenith[1] * perez(1);
-  float lum = p.sky_zenith[2] * perez(2);
+  real x = p.sky_zenith[0] * perez(0);
+  real y = p.sky_zenith[1] * perez(1);
+  real lum = p.sky_zenith[2] * perez(2);
   if (y <= 5e-8f) return ZERO3;
-  float f = lum / y;
-  float cx = x * f, cy = lum, cz = (1.0f - x - y) * f;
+  real f = lum / y;
+  real cx = x * f, cy = lum, cz = (1.0f - x - y) * f;
   return V3{2.3706743f * cx - 0.9000405f * cy - 0.4706338f * cz,
             -0.513885f * cx + 1.4253036f * cy + 0.0885814f * cz,
             0.0052982f * cx - 0.0146949f * cy + 1.0093968f * cz} *
@@ -612,13 +653,13 @@ static V3 preetham_radiance(const View &s, V3 d) {
 }
 
 // integrator.rs:79-104
-static inline void rgb_to_hsl(V3 c, float out[3]) {
-  float r = f_clamp(c.x, 0.0f, 1.0f), g = f_clamp(c.y, 0.0f, 1.0f), b = f_clamp(c.z, 0.0f, 1.0f);
-  float cmax = f_max(f_max(r, g), b);
-  float cmin = f_min(f_min(r, g), b);
-  float delta = cmax - cmin;
-  float lightness = (cmax + cmin) / 2.0f;
-  float hue;
+static inline void rgb_to_hsl(V3 c, real out[3]) {
+  real r = f_clamp(c.x, 0.0f, 1.0f), g = f_clamp(c.y, 0.0f, 1.0f), b = f_clamp(c.z, 0.0f, 1.0f);
+  real cmax = f_max(f_max(r, g), b);
+  real cmin = f_min(f_min(r, g), b);
+  real delta = cmax - cmin;
+  real lightness = (cmax + cmin) / 2.0f;
+  real hue;
   if (delta == 0.0f) {
     hue = 0.0f;
   } else if (cmax == r) {
@@ -628,17 +669,17 @@ static inline void rgb_to_hsl(V3 c, float out[3]) {
   } else {
     hue = (((r - g) / delta) + 4.0f) / 6.0f;
   }
-  float saturation = delta < 5e-8f ? 0.0f : delta / (1.0f - f_abs(2.0f * lightness - 1.0f));
+  real saturation = delta < 5e-8f ? 0.0f : delta / (1.0f - f_abs(2.0f * lightness - 1.0f));
   out[0] = hue;
   out[1] = saturation;
   out[2] = lightness;
 }
 
 // integrator.rs:107-127
-static inline V3 rgb_from_hsl(float hue, float saturation, float lightness) {
-  float c = f_min(1.0f, (1.0f - f_abs(2.0f * lightness - 1.0f)) * saturation);
-  float h = hue * 6.0f;
-  float x = c * (1.0f - f_abs(f_rem(h, 2.0f) - 1.0f));
+static inline V3 rgb_from_hsl(real hue, real saturation, real lightness) {
+  real c = f_min(1.0f, (1.0f - f_abs(2.0f * lightness - 1.0f)) * saturation);
+  real h = hue * 6.0f;
+  real x = c * (1.0f - f_abs(f_rem(h, 2.0f) - 1.0f));
   V3 rgb;
   if (h < 1.0f) {
     rgb = v3(c, x, 0.0f);
@@ -653,7 +694,7 @@ static inline V3 rgb_from_hsl(float hue, float saturation, float lightness) {
   } else {
     rgb = v3(c, 0.0f, x);
   }
-  float m = f_max(0.0f, lightness - 0.5f * c);
+  real m = f_max(0.0f, lightness - 0.5f * c);
   return rgb + v3(m, m, m);
 }
 
@@ -661,13 +702,13 @@ static inline V3 rgb_from_hsl(float hue, float saturation, float lightness) {
 static void build_sky_cache(const View &s, uint32_t n, float *out, size_t out_len) {
   uint32_t m = n + 1;
   for (uint32_t i = 0; i < m; i++) {
-    float sin_t, cos_t;
-    sin_cos_turns((float)i / (float)n, sin_t, cos_t);
+    real sin_t, cos_t;
+    sin_cos_turns((real)i / (real)n, sin_t, cos_t);
     for (uint32_t j = 0; j < m; j++) {
-      float sin_p, cos_p;
-      sin_cos_turns((float)j / (float)(2u * n) - 0.25f, sin_p, cos_p);
-      float r = f_abs(cos_p);
-      float hsl[3];
+      real sin_p, cos_p;
+      sin_cos_turns((real)j / (real)(2u * n) - 0.25f, sin_p, cos_p);
+      real r = f_abs(cos_p);
+      real hsl[3];
       rgb_to_hsl(preetham_radiance(s, v3(cos_t * r, sin_p, sin_t * r)), hsl);
       size_t a = (size_t)((i * m + j) * 3u);
       if (a + 2 >= out_len) throw OracleError("sky cache output too small");
@@ -678,50 +719,50 @@ static void build_sky_cache(const View &s, uint32_t n, float *out, size_t out_le
   }
 }
 
-static inline float sky_at(const View &s, size_t i) { return at(s.sky, s.sky_len, i, "sky"); }
+static inline real sky_at(const View &s, size_t i) { return at(s.sky, s.sky_len, i, "sky"); }
 
 // integrator.rs:158-188
 static V3 sky_radiance(const View &s, V3 d) {
   const Params &p = *s.params;
-  float n = p.misc_f[2];
+  real n = p.misc_f[2];
   if (n < 1.0f) return ZERO3;
   LWO_COUNT(bytes, 48);
   LWO_COUNT(sky_lookups, 1);
-  float theta = f_atan2(d.z, d.x) / (2.0f * PI_F);
+  real theta = f_atan2(d.z, d.x) / (2.0f * PI_F);
   theta = f_rem(f_rem(theta, 1.0f) + 1.0f, 1.0f);
-  float phi = 1.0f - f_acos(f_clamp(d.y, -1.0f, 1.0f)) / PI_F;
-  float x = theta * n, y = phi * n;
-  float fx = f_floor(f_clamp(x, 0.0f, n - 1.0f));
-  float fy = f_floor(f_clamp(y, 0.0f, n - 1.0f));
+  real phi = 1.0f - f_acos(f_clamp(d.y, -1.0f, 1.0f)) / PI_F;
+  real x = theta * n, y = phi * n;
+  real fx = f_floor(f_clamp(x, 0.0f, n - 1.0f));
+  real fy = f_floor(f_clamp(y, 0.0f, n - 1.0f));
   uint32_t m = as_u32(n) + 1u;
-  auto idx = [&](float i, float j) -> size_t { return (size_t)((as_u32(i) * m + as_u32(j)) * 3u); };
+  auto idx = [&](real i, real j) -> size_t { return (size_t)((as_u32(i) * m + as_u32(j)) * 3u); };
   size_t a00 = idx(fx, fy), a10 = idx(fx + 1.0f, fy), a01 = idx(fx, fy + 1.0f),
          a11 = idx(fx + 1.0f, fy + 1.0f);
-  float hsl[3];
+  real hsl[3];
   for (int k = 0; k < 3; k++) {
-    float y0 = sky_at(s, a00 + k) + (x - fx) * (sky_at(s, a10 + k) - sky_at(s, a00 + k));
-    float y1 = sky_at(s, a01 + k) + (x - fx) * (sky_at(s, a11 + k) - sky_at(s, a01 + k));
+    real y0 = sky_at(s, a00 + k) + (x - fx) * (sky_at(s, a10 + k) - sky_at(s, a00 + k));
+    real y1 = sky_at(s, a01 + k) + (x - fx) * (sky_at(s, a11 + k) - sky_at(s, a01 + k));
     hsl[k] = y0 + (y - fy) * (y1 - y0);
   }
   return rgb_from_hsl(hsl[0], hsl[1], hsl[2]) * p.sky_sun[3];
 }
 
 // integrator.rs:196-217
-static V3 map_lookup(const View &s, uint32_t base, float w, float h, V3 d) {
+static V3 map_lookup(const View &s, uint32_t base, real w, real h, V3 d) {
   LWO_COUNT(bytes, 48);
   LWO_COUNT(sky_lookups, 1);
-  float u = f_atan2(d.z, d.x) / (2.0f * PI_F);
+  real u = f_atan2(d.z, d.x) / (2.0f * PI_F);
   u = f_rem(f_rem(u, 1.0f) + 1.0f, 1.0f);
-  float v = 1.0f - f_acos(f_clamp(d.y, -1.0f, 1.0f)) / PI_F;
-  float x = u * w - 0.5f;
-  float y = f_clamp(v * h - 0.5f, 0.0f, h - 1.0f);
-  float fx = f_floor(x);
-  float fy = f_floor(y);
-  float tx = x - fx, ty = y - fy;
-  float i0 = fx < 0.0f ? w - 1.0f : fx;
-  float i1 = fx + 1.0f >= w ? 0.0f : fx + 1.0f;
-  float j1 = f_min(fy + 1.0f, h - 1.0f);
-  auto texel_at = [&](float i, float j) -> V3 {
+  real v = 1.0f - f_acos(f_clamp(d.y, -1.0f, 1.0f)) / PI_F;
+  real x = u * w - 0.5f;
+  real y = f_clamp(v * h - 0.5f, 0.0f, h - 1.0f);
+  real fx = f_floor(x);
+  real fy = f_floor(y);
+  real tx = x - fx, ty = y - fy;
+  real i0 = fx < 0.0f ? w - 1.0f : fx;
+  real i1 = fx + 1.0f >= w ? 0.0f : fx + 1.0f;
+  real j1 = f_min(fy + 1.0f, h - 1.0f);
+  auto texel_at = [&](real i, real j) -> V3 {
     size_t k = (size_t)(base + (as_u32(j) * as_u32(w) + as_u32(i)) * 3u);
     return v3(sky_at(s, k), sky_at(s, k + 1), sky_at(s, k + 2));
   };
@@ -749,22 +790,22 @@ static V3 sun_texture(const View &s, V3 d) {
   const Params &p = *s.params;
   const float *sprite = p.sun_sprite;
   if (sprite[2] <= 0.0f || dot(d, from4(p.sun_dir)) < 0.5f) return ZERO3;
-  float width = p.sun_sv[3];
-  float width2 = width * 2.0f;
-  float a = 0.5f * PI_F - f_acos(dot(d, from4(p.sun_su))) + width;
-  float b = 0.5f * PI_F - f_acos(dot(d, from4(p.sun_sv))) + width;
+  real width = p.sun_sv[3];
+  real width2 = width * 2.0f;
+  real a = 0.5f * PI_F - f_acos(dot(d, from4(p.sun_su))) + width;
+  real b = 0.5f * PI_F - f_acos(dot(d, from4(p.sun_sv))) + width;
   if (a < 0.0f || a >= width2 || b < 0.0f || b >= width2) return ZERO3;
-  float x = f_clamp(f_floor(a / width2 * sprite[2] - 5e-8f), 0.0f, sprite[2] - 1.0f);
-  float y = f_clamp(f_floor((1.0f - b / width2) * sprite[3] - 5e-8f), 0.0f, sprite[3] - 1.0f);
+  real x = f_clamp(f_floor(a / width2 * sprite[2] - 5e-8f), 0.0f, sprite[2] - 1.0f);
+  real y = f_clamp(f_floor((1.0f - b / width2) * sprite[3] - 5e-8f), 0.0f, sprite[3] - 1.0f);
   return linear_rgb(texel(s, sprite[0] + x + 0.5f, sprite[1] + y + 0.5f)) * p.sun_su[3];
 }
 
 // integrator.rs:279-287
 static inline V3 cosine_hemisphere(V3 n, Rng &rng) {
-  float r1 = rng.next();
-  float r2 = rng.next();
-  float r = f_sqrt(r1);
-  float sin_phi, cos_phi;
+  real r1 = rng.next();
+  real r2 = rng.next();
+  real r = f_sqrt(r1);
+  real sin_phi, cos_phi;
   sin_cos_turns(r2, sin_phi, cos_phi);
   V3 t, b;
   onb(n, t, b);
@@ -775,12 +816,12 @@ static inline V3 cosine_hemisphere(V3 n, Rng &rng) {
 static inline V3 sample_sun(const View &s, Rng &rng) {
   const Params &p = *s.params;
   V3 sun = from4(p.sun_dir);
-  float cos_max = p.sun_dir[3];
-  float r1 = rng.next();
-  float r2 = rng.next();
-  float cos_t = 1.0f - r1 * (1.0f - cos_max);
-  float sin_t = f_sqrt(f_max(1.0f - cos_t * cos_t, 0.0f));
-  float sin_phi, cos_phi;
+  real cos_max = p.sun_dir[3];
+  real r1 = rng.next();
+  real r2 = rng.next();
+  real cos_t = 1.0f - r1 * (1.0f - cos_max);
+  real sin_t = f_sqrt(f_max(1.0f - cos_t * cos_t, 0.0f));
+  real sin_phi, cos_phi;
   sin_cos_turns(r2, sin_phi, cos_phi);
   V3 t, b;
   onb(sun, t, b);
@@ -799,8 +840,8 @@ static inline V3 shading_normal(const Quad &q, V3 d, bool &front) {
 }
 
 // integrator.rs:318-346
-static V3 shadow(const View &s, V3 o, V3 d, float tmax) {
-  float eps = s.params->misc_f[1];
+static V3 shadow(const View &s, V3 o, V3 d, real tmax) {
+  real eps = s.params->misc_f[1];
   V3 tr = ONE3;
   for (int i = 0; i < 8; i++) {
     Hit h;
@@ -811,7 +852,7 @@ static V3 shadow(const View &s, V3 o, V3 d, float tmax) {
     if ((flags & FLAG_WATER) != 0) {
       if (dot(q_normal(q), d) > 0.0f) tr = mul(tr, transmittance(from4(s.params->water_color), h.t));
     } else if ((flags & FLAG_ALPHA_MASK) == ALPHA_TRANSLUCENT) {
-      float a = (float)(h.texel >> 24) / 255.0f;
+      real a = (real)(h.texel >> 24) / 255.0f;
       V3 col = mul(linear_rgb(h.texel), linear_rgb(q_tint(q)));
       tr = mul(tr, lerp(ONE3, col, a));
     } else {
@@ -824,8 +865,8 @@ static V3 shadow(const View &s, V3 o, V3 d, float tmax) {
 }
 
 // integrator.rs:349-353
-static inline uint32_t pick(uint32_t n, float u) {
-  uint32_t i = as_u32(u * (float)n);
+static inline uint32_t pick(uint32_t n, real u) {
+  uint32_t i = as_u32(u * (real)n);
   return i < n ? i : n - 1;
 }
 
@@ -835,7 +876,7 @@ static inline uint32_t eg_at(const View &s, size_t i) {
 
 // integrator.rs:357-373
 static inline void emitter_cell(const View &s, V3 p, uint32_t &start, uint32_t &count) {
-  float cell = (float)eg_at(s, 0);
+  real cell = (real)eg_at(s, 0);
   int32_t size[3] = {(int32_t)eg_at(s, 1), (int32_t)eg_at(s, 2), (int32_t)eg_at(s, 3)};
   int32_t c[3] = {
       (int32_t)((uint32_t)as_i32(f_floor(p.x / cell)) - eg_at(s, 4)),
@@ -858,24 +899,24 @@ static inline void emitter_cell(const View &s, V3 p, uint32_t &start, uint32_t &
 
 // integrator.rs:380-414
 static V3 sample_emitter_face(const View &s, V3 origin, V3 n, const Emitter &e, uint32_t face,
-                              float scaler, Rng &rng) {
+                              real scaler, Rng &rng) {
   const Params &p = *s.params;
   const Quad &q = quad_at(s, (size_t)(f_bits(e.pos[3]) + face));
   LWO_COUNT(bytes, 80);
-  float ss = rng.next();
-  float t = rng.next();
+  real ss = rng.next();
+  real t = rng.next();
   V3 at_p = from4(e.pos) + from4(q.p0) + from4(q.eu) * ss + from4(q.ev) * t;
   V3 to = at_p - origin;
   if (dot(to, n) <= 0.0f) return ZERO3;
-  float dist = length(to);
+  real dist = length(to);
   V3 l = to / dist;
-  float eps = p.misc_f[1];
+  real eps = p.misc_f[1];
   Hit h;
   if (!trace(s, origin, l, eps, dist + 2.0f * eps, h)) return ZERO3;
   if (f_abs(h.t - dist) >= 2.0f * eps) return ZERO3;
   V3 nq = q_normal(q);
-  float area = length(nq);
-  float cos_l = f_abs(dot(nq, l)) / area;
+  real area = length(nq);
+  real cos_l = f_abs(dot(nq, l)) / area;
   const Quad &hq = quad_at(s, h.quad);
   V3 c = mul(linear_rgb(h.texel), linear_rgb(q_tint(hq)));
   return c * (cos_l / f_max(dist * dist, 1.0f) * area * q_emittance(q) * p.misc_f[0] * scaler);
@@ -891,7 +932,7 @@ static V3 sample_emitter(const View &s, V3 origin, V3 n, const Emitter &e, Rng &
   }
   V3 sum = ZERO3;
   for (uint32_t f = 0; f < faces; f++)
-    sum = sum + sample_emitter_face(s, origin, n, e, f, 1.0f / (float)faces, rng);
+    sum = sum + sample_emitter_face(s, origin, n, e, f, 1.0f / (real)faces, rng);
   return sum;
 }
 
@@ -917,28 +958,28 @@ static V3 sample_emitters(const View &s, V3 pos, V3 origin, V3 n, Rng &rng) {
       Emitter e = at(s.emitters, s.emitters_len, i, "emitters");
       sum = sum + sample_emitter(s, origin, n, e, rng);
     }
-    return sum * (PI_F / (float)count);
+    return sum * (PI_F / (real)count);
   }
   return ZERO3;
 }
 
 // integrator.rs:477-505
 static inline void water_interface(const Params &p, V3 pos, V3 d, V3 n, bool front, bool in_water,
-                                   float eps, Rng &rng, V3 &d_out, V3 &o_out, bool &w_out) {
-  float ior = p.water_color[3];
-  float eta = front ? 1.0f / ior : ior;
-  float cos_i = -dot(n, d);
-  float sin2_t = eta * eta * f_max(1.0f - cos_i * cos_i, 0.0f);
-  float r0 = (1.0f - ior) / (1.0f + ior);
+                                   real eps, Rng &rng, V3 &d_out, V3 &o_out, bool &w_out) {
+  real ior = p.water_color[3];
+  real eta = front ? 1.0f / ior : ior;
+  real cos_i = -dot(n, d);
+  real sin2_t = eta * eta * f_max(1.0f - cos_i * cos_i, 0.0f);
+  real r0 = (1.0f - ior) / (1.0f + ior);
   r0 = r0 * r0;
-  float m = 1.0f - cos_i;
-  float fresnel = r0 + (1.0f - r0) * m * m * m * m * m;
+  real m = 1.0f - cos_i;
+  real fresnel = r0 + (1.0f - r0) * m * m * m * m * m;
   if (sin2_t >= 1.0f || rng.next() < fresnel) {
     d_out = normalize(d + n * (2.0f * cos_i));
     o_out = pos + n * eps;
     w_out = in_water;
   } else {
-    float cos_t = f_sqrt(1.0f - sin2_t);
+    real cos_t = f_sqrt(1.0f - sin2_t);
     d_out = normalize(d * eta + n * (eta * cos_i - cos_t));
     o_out = pos - n * eps;
     w_out = front;
@@ -946,10 +987,10 @@ static inline void water_interface(const Params &p, V3 pos, V3 d, V3 n, bool fro
 }
 
 // integrator.rs:510-526
-static inline bool water_plane(const View &s, V3 o, V3 d, float &t_out) {
+static inline bool water_plane(const View &s, V3 o, V3 d, real &t_out) {
   const Params &p = *s.params;
   if (p.water_plane[1] == 0.0f || d.y == 0.0f) return false;
-  float t = (p.water_plane[0] - o.y) / d.y;
+  real t = (p.water_plane[0] - o.y) / d.y;
   if (!(t > 0.0f && t < INFINITY)) return false;
   V3 a = o + d * t;
   V3 size = grid_size(s);
@@ -970,16 +1011,16 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
   uint32_t width = p.frame[0];
   uint32_t height = p.frame[1];
   uint32_t max_bounces = p.frame[3];
-  float eps = p.misc_f[1];
+  real eps = p.misc_f[1];
   Rng rng = Rng::make(py * width + px, sample, p.misc_u[0]);
   LWO_COUNT(samples, 1);
 
-  float tan_half = p.cam_pos[3];
-  float aspect = p.cam_fwd[3];
-  float jx = rng.next();
-  float jy = rng.next();
-  float sx = (2.0f * ((float)px + jx) / (float)width - 1.0f) * aspect * tan_half;
-  float sy = (1.0f - 2.0f * ((float)py + jy) / (float)height) * tan_half;
+  real tan_half = p.cam_pos[3];
+  real aspect = p.cam_fwd[3];
+  real jx = rng.next();
+  real jy = rng.next();
+  real sx = (2.0f * ((real)px + jx) / (real)width - 1.0f) * aspect * tan_half;
+  real sy = (1.0f - 2.0f * ((real)py + jy) / (real)height) * tan_half;
   V3 d = normalize(from4(p.cam_fwd) + from4(p.cam_right) * sx + from4(p.cam_up) * sy);
   V3 o = from4(p.cam_pos);
 
@@ -989,7 +1030,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
   bool specular = true;
   bool in_water = false;
   uint32_t strategy = p.misc_u[3] > 0 ? p.emitter[0] : EMITTER_SAMPLING_NONE;
-  float tmin = 0.0f;
+  real tmin = 0.0f;
   V3 sigma = from4(p.water_color);
 
   for (uint32_t ev = 0; ev < MAX_EVENTS; ev++) {
@@ -1001,7 +1042,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
       if (see_through_hit(s, o, d, hit.t, st)) {
         const Quad &sq = quad_at(s, st.quad);
         if ((q_flags(sq) & FLAG_ALPHA_MASK) == ALPHA_TRANSLUCENT) {
-          if (rng.next() < (float)(st.texel >> 24) / 255.0f) {
+          if (rng.next() < (real)(st.texel >> 24) / 255.0f) {
             hit = st;
             forced = true;
           }
@@ -1010,7 +1051,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
         }
       }
     }
-    float tp;
+    real tp;
     if (water_plane(s, o, d, tp) && (!has_hit || tp < hit.t)) {
       if (in_water) beta = mul(beta, transmittance(sigma, tp));
       bool front = d.y < 0.0f;
@@ -1037,7 +1078,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
     LWO_COUNT(bytes, 80);
     uint32_t flags = q_flags(q);
     V3 pos = o + d * h.t;
-    float alpha = (float)(h.texel >> 24) / 255.0f;
+    real alpha = (real)(h.texel >> 24) / 255.0f;
     V3 albedo = mul(linear_rgb(h.texel), linear_rgb(q_tint(q)));
     bool front;
     V3 n = shading_normal(q, d, front);
@@ -1057,7 +1098,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
     }
 
     if (bounces >= max_bounces) break;
-    float emittance = q_emittance(q) * p.misc_f[0];
+    real emittance = q_emittance(q) * p.misc_f[0];
     bool emits = emittance > 0.0f &&
                  (p.emitter[1] == 0 || strategy == EMITTER_SAMPLING_NONE || bounces == 0);
     if (emits) radiance = radiance + mul(beta, mul(albedo, albedo)) * emittance;
@@ -1065,11 +1106,11 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
 
     V3 origin = pos + n * eps;
     V3 sun = from4(p.sun_dir);
-    float sin_r = f_sqrt(f_max(1.0f - p.sun_dir[3] * p.sun_dir[3], 0.0f));
+    real sin_r = f_sqrt(f_max(1.0f - p.sun_dir[3] * p.sun_dir[3], 0.0f));
     bool lit = p.sun_color[0] > 0.0f || p.sun_color[1] > 0.0f || p.sun_color[2] > 0.0f;
     if (lit && dot(n, sun) > -sin_r && p.sun_dir[1] > -sin_r) {
       V3 l = sample_sun(s, rng);
-      float cos_l = dot(n, l);
+      real cos_l = dot(n, l);
       if (cos_l > 0.0f && l.y >= 0.0f) {
         V3 tr = shadow(s, origin, l, INFINITY);
         radiance = radiance + mul(mul(mul(beta, albedo), from4(p.sun_color)), tr) * cos_l;
@@ -1081,7 +1122,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
 
     beta = mul(beta, albedo);
     if (bounces >= 3) {
-      float q_rr = f_clamp(max_elem(beta), 0.05f, 0.95f);
+      real q_rr = f_clamp(max_elem(beta), 0.05f, 0.95f);
       if (rng.next() > q_rr) break;
       beta = beta / q_rr;
     }
@@ -1100,7 +1141,7 @@ static inline bool finite3(V3 c) {
 // CpuBackend::render_hq (crates/lumenwright/src/render/cpu/mod.rs:39-87): rows in
 // parallel (one row per task, taken dynamically as rayon's work stealing does), f64
 // per-pixel accumulation of the finite samples, divided by spp.
-static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, float *out,
+static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, real *out,
                       Counters *counters) {
   Params params = *view_in.params;
   params.frame[2] = 0;
@@ -1130,10 +1171,10 @@ static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, fl
               acc[2] += (double)c.z;
             }
           }
-          float *px = out + ((size_t)y * w + x) * 3;
-          px[0] = (float)(acc[0] / (double)spp);
-          px[1] = (float)(acc[1] / (double)spp);
-          px[2] = (float)(acc[2] / (double)spp);
+          real *px = out + ((size_t)y * w + x) * 3;
+          px[0] = (real)(acc[0] / (double)spp);
+          px[1] = (real)(acc[1] / (double)spp);
+          px[2] = (real)(acc[2] / (double)spp);
         }
       }
     } catch (const std::exception &e) {

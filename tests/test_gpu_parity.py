@@ -173,6 +173,15 @@ def test_render_hq_matches_the_cpu_backend(hip_backend, oracle, frame0, frame1):
             assert same.all()
 
 
+def test_device_image_error_against_the_f64_render(hip_backend, oracle, frame0):
+    # The image's error against the f64 render (oracle/lw_oracle_f64.cpp) at config 0 in full.
+    want = oracle.render_hq_f64(frame0.view(), frame0.spp).reshape(-1, 3)
+    got = hip_backend.render_hq(frame0).astype(np.float64)
+    d = np.abs(got - want)
+    print(f"cfg0 device image against f64: MAE {d.mean():.3e}, relative {d.mean() / want.mean():.2e}")
+    assert d.mean() < 1e-3
+
+
 def test_render_hq_kat_scenes_on_the_device(hip_backend):
     # tests.rs:475-492: the lamp seen from the camera is 0.5 * 13 in every sample.
     s = scenes.lamp_scene(EMITTER_SAMPLING_ONE, False)
