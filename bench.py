@@ -29,7 +29,10 @@ Besides the throughput the config-1 line carries:
   cpu_baseline  the oracle (the C++ restatement of the reference's CPU backend) on the
                 host's cores over a bounded slice of the same frame (the first S samples of
                 every pixel), timed in this run;
-  mrays_per_s   Msamples/s times the trace calls per sample the oracle counts;
+  mrays_per_s   the trace calls of a step (camera, extension, shadow and emitter rays),
+                counted on the device by an untimed replay of the step's paths
+                (lw_hip_count_traces), over the step's time; mrays_per_s_oracle_rate is
+                Msamples/s times the oracle's trace calls per sample, as a cross-check;
   post          the device tonemap and tone mix of the last frame (kernel time, GB/s);
   scene_build   the world's layout made on the device from its voxel volume
                 (lw_hip_scene_build), and the oracle's CPU restatement of it;
@@ -278,6 +281,7 @@ def bench_plain(args, lw, frames):
     samples = w * h * spp * args.steps
     value = samples / elapsed / 1e6
     assert np.isfinite(img).all()
+    traces = b.count_traces(scene, frame.params, spp)  # untimed replay of a step's paths
     extra = {"post": post_timing(b, frames, img),
              "scene_build": build_timing(b, world, frame.sky, not args.no_cpu_baseline),
              "sky_prep": sky_timing(b, frames, not args.no_cpu_baseline),
@@ -310,7 +314,9 @@ def bench_plain(args, lw, frames):
         "config": {"workload": cfg.name, "width": w, "height": h, "spp": spp,
                    "max_bounces": cfg.max_bounces, "kernel": form,
                    "parallelism": "1 GPU"},
-        "mrays_per_s": mrays(args.config, value),
+        "mrays_per_s": traces * args.steps / elapsed / 1e6,
+        "traces_per_sample": traces / (w * h * spp),
+        "mrays_per_s_oracle_rate": mrays(args.config, value),
         "autotune": tuned,
         "kernel_ms_per_step": trace_ms / args.steps,
         "accumulate_ms_per_step": acc_ms / args.steps,

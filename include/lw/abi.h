@@ -227,6 +227,13 @@ typedef struct lw_quick_frame {
  * words (red in the low byte), row major, top row first. */
 int lw_hip_render_quick(lw_hip_backend *b, const lw_quick_frame *frame, uint32_t *out_rgba8);
 
+/* The number of trace calls (trace.rs:137-290; camera and extension rays, each sun shadow
+ * segment, each emitter ray) a render of the scene with these params and spp makes: the
+ * paths of lw_hip_render_hq_scene, replayed by a counting kernel that is not timed. No
+ * counterpart in the reference, which counts no rays; the measurement's Mrays/s. */
+int lw_hip_count_traces(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params, uint32_t spp,
+                        uint64_t *traces);
+
 /* The message of the last failure on this handle ("" if none). */
 const char *lw_hip_last_error(const lw_hip_backend *b);
 

@@ -159,6 +159,15 @@ class HipBackend:
                                                          cb, None, _addr(out)))
         return out
 
+    def count_traces(self, scene: HipScene, params: np.ndarray, spp: int) -> int:
+        """lw_hip_count_traces: the trace calls a render_hq_scene of (params, spp) makes,
+        counted on the device by an untimed replay of its paths."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        n = ctypes.c_uint64()
+        self._check(_native.lib().lw_hip_count_traces(self.handle, scene.handle, _addr(p), spp,
+                                                      ctypes.byref(n)))
+        return int(n.value)
+
     def render_hq_batch(self, scene: HipScene, views,
                         progress: Optional[Callable[[int, int], None]] = None) -> list:
         """Renders [(params, spp), ...] of one resident scene as one batch

@@ -538,12 +538,14 @@ __device__ __forceinline__ V3 sample_sun(const lw_params &p, Rng &rng) {
 __device__ __forceinline__ V3 quad_normal(const float4 *q) { return cross(xyz(q[2]), xyz(q[1])); }
 
 // integrator.rs:318-346
+// n_traces, when given, counts the trace calls (the device's ray count).
 __device__ __forceinline__ V3 shadow(const DevScene &s, const lw_params &p, const float *lut, V3 o,
-                                  V3 d, float tmax) {
+                                  V3 d, float tmax, uint32_t *n_traces = nullptr) {
   const float eps = p.misc_f[1];
   V3 tr{1.0f, 1.0f, 1.0f};
   for (int i = 0; i < 8; i++) {
     Hit h;
+    if (n_traces) ++*n_traces;
     if (!trace(s, p, o, d, eps, tmax, h, nullptr)) return tr;
     const float4 *q = s.quads + (size_t)h.quad * 5;
     const uint32_t flags = f_bits(q[0].w);
@@ -592,7 +594,7 @@ __device__ __forceinline__ void emitter_cell(const DevScene &s, V3 p, uint32_t &
 __device__ __forceinline__ V3 sample_emitter_face(const DevScene &s, const lw_params &p,
                                                   const float *lut, V3 origin, V3 n,
                                                   const lw_emitter &e, uint32_t face, float scaler,
-                                                  Rng &rng) {
+                                                  Rng &rng, uint32_t *n_traces = nullptr) {
   const float4 *q = s.quads + (size_t)(f_bits(e.pos[3]) + face) * 5;
   const float ss = rng.next();
   const float t = rng.next();
@@ -604,6 +606,7 @@ __device__ __forceinline__ V3 sample_emitter_face(const DevScene &s, const lw_pa
   const V3 l = to / dist;
   const float eps = p.misc_f[1];
   Hit h;
+  if (n_traces) ++*n_traces;
   if (!trace(s, p, origin, l, eps, dist + 2.0f * eps, h, nullptr)) return V3{0.0f, 0.0f, 0.0f};
   if (f_abs(h.t - dist) >= 2.0f * eps) return V3{0.0f, 0.0f, 0.0f};
   const V3 nq = cross(xyz(q2), xyz(q1));
@@ -616,22 +619,24 @@ __device__ __forceinline__ V3 sample_emitter_face(const DevScene &s, const lw_pa
 
 // integrator.rs:418-433
 __device__ __forceinline__ V3 sample_emitter(const DevScene &s, const lw_params &p, const float *lut,
-                                          V3 origin, V3 n, const lw_emitter &e, Rng &rng) {
+                                          V3 origin, V3 n, const lw_emitter &e, Rng &rng,
+                                          uint32_t *n_traces = nullptr) {
   const uint32_t faces = e.faces[0];
   if (faces == 0) return V3{0.0f, 0.0f, 0.0f};
   if (p.emitter[0] == LW_EMITTER_SAMPLING_ONE) {
     const uint32_t f = pick(faces, rng.next());
-    return sample_emitter_face(s, p, lut, origin, n, e, f, 1.0f, rng);
+    return sample_emitter_face(s, p, lut, origin, n, e, f, 1.0f, rng, n_traces);
   }
   V3 sum{0.0f, 0.0f, 0.0f};
   for (uint32_t f = 0; f < faces; f++)
-    sum = sum + sample_emitter_face(s, p, lut, origin, n, e, f, 1.0f / (float)faces, rng);
+    sum = sum + sample_emitter_face(s, p, lut, origin, n, e, f, 1.0f / (float)faces, rng, n_traces);
   return sum;
 }
 
 // integrator.rs:441-469
 __device__ __forceinline__ V3 sample_emitters(const DevScene &s, const lw_params &p,
-                                              const float *lut, V3 pos, V3 origin, V3 n, Rng &rng) {
+                                              const float *lut, V3 pos, V3 origin, V3 n, Rng &rng,
+                                              uint32_t *n_traces = nullptr) {
   uint32_t start, count;
   emitter_cell(s, pos, start, count);
   if (count == 0) return V3{0.0f, 0.0f, 0.0f};
@@ -642,13 +647,13 @@ __device__ __forceinline__ V3 sample_emitters(const DevScene &s, const lw_params
   if (strategy == LW_EMITTER_SAMPLING_ONE || strategy == LW_EMITTER_SAMPLING_ONE_BLOCK) {
     const uint32_t i = g[list + pick(count, rng.next())];
     const lw_emitter e = s.emitters[i];
-    return sample_emitter(s, p, lut, origin, n, e, rng) * PI_F;
+    return sample_emitter(s, p, lut, origin, n, e, rng, n_traces) * PI_F;
   }
   if (strategy == LW_EMITTER_SAMPLING_ALL) {
     V3 sum{0.0f, 0.0f, 0.0f};
     for (uint32_t k = 0; k < count; k++) {
       const lw_emitter e = s.emitters[g[list + k]];
-      sum = sum + sample_emitter(s, p, lut, origin, n, e, rng);
+      sum = sum + sample_emitter(s, p, lut, origin, n, e, rng, n_traces);
     }
     return sum * (PI_F / (float)count);
   }
@@ -749,7 +754,7 @@ __device__ __forceinline__ void sample_begin(SampleState &S, const Cam &c, uint3
 // One iteration of the event loop; false once the path has ended (a `break` of the loop or
 // MAX_EVENTS iterations), true where the loop would go on (`continue` or the loop's end).
 __device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s, const lw_params &p,
-                                             const float *lut) {
+                                             const float *lut, uint32_t *n_traces = nullptr) {
   if (S.ev >= MAX_EVENTS) return false;
   S.ev++;
   const uint32_t max_bounces = p.frame[3];
@@ -762,6 +767,7 @@ __device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s, 
   const bool lit = p.sun_color[0] > 0.0f || p.sun_color[1] > 0.0f || p.sun_color[2] > 0.0f;
 
   Hit hit;
+  if (n_traces) ++*n_traces;
   bool has_hit = trace(s, p, S.o, S.d, S.tmin, INFINITY, hit, nullptr);
   bool forced = false;
   if (S.bounces == 0 && p.see_through[1] > 0 && has_hit && !is_see_through(p, hit.quad)) {
@@ -837,12 +843,12 @@ __device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s, 
     const V3 l = sample_sun(p, S.rng);
     const float cos_l = dot(n, l);
     if (cos_l > 0.0f && l.y >= 0.0f) {
-      const V3 tr = shadow(s, p, lut, origin, l, INFINITY);
+      const V3 tr = shadow(s, p, lut, origin, l, INFINITY, n_traces);
       S.radiance = S.radiance + mul(mul(mul(S.beta, albedo), sun_color), tr) * cos_l;
     }
   }
   if (!emits && strategy != LW_EMITTER_SAMPLING_NONE) {
-    S.radiance = S.radiance + mul(mul(S.beta, albedo), sample_emitters(s, p, lut, pos, origin, n, S.rng));
+    S.radiance = S.radiance + mul(mul(S.beta, albedo), sample_emitters(s, p, lut, pos, origin, n, S.rng, n_traces));
   }
 
   S.beta = mul(S.beta, albedo);

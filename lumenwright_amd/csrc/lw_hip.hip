@@ -77,6 +77,38 @@ __global__ void __launch_bounds__(kBlock, W) k_render(const lw_params p, const D
   o[2] = c.z;
 }
 
+// The trace calls of samples [sample_start, sample_start + gridDim.y) of every pixel, as
+// k_render renders them (the same lanes, the same paths): summed per workgroup in LDS, one
+// 64-bit atomic a workgroup.
+__global__ void __launch_bounds__(kBlock) k_count_traces(const lw_params p, const DevScene s,
+                                                         uint32_t sample_start,
+                                                         unsigned long long *__restrict__ total) {
+  __shared__ float lut_s[256];
+  __shared__ uint32_t sums[kBlock / 64];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint32_t tiles_x = (w + 15u) / 16u;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
+  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  uint32_t n = 0;
+  if (px < w && py < h) {
+    SampleState S;
+    sample_begin(S, p, px, py, sample_start + blockIdx.y);
+    while (sample_event(S, s, p, lut_s, &n)) {
+    }
+  }
+  for (uint32_t off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+  if (lane == 0) sums[wave] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t k = 0; k < kBlock / 64; k++) t += sums[k];
+    atomicAdd(total, t);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_accumulate(uint32_t npix, uint32_t samples,
                                                        const float *__restrict__ rad,
                                                        double *__restrict__ acc) {
@@ -2418,6 +2450,34 @@ int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene, const l
                            uint32_t spp, lw_progress_fn progress, void *user, float *out_rgb) {
   if (!b || !scene || !params || !out_rgb) return LW_EINVAL;
   return guarded(b, [&] { render_frame(b, *scene, *params, spp, progress, user, out_rgb); });
+}
+
+int lw_hip_count_traces(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params, uint32_t spp,
+                        uint64_t *traces) {
+  if (!b || !scene || !params || !traces) return LW_EINVAL;
+  return guarded(b, [&] {
+    check_layout(*scene, *params);
+    lw_params p = *params;
+    p.frame[2] = 0;
+    const uint32_t w = p.frame[0], h = p.frame[1];
+    if (w == 0 || h == 0) throw InvalidScene{"empty frame"};
+    const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
+    HIP_TRY(hipSetDevice(b->device));
+    const DevScene ds = scene->dev();
+    DevBuf<unsigned long long> total;
+    total.alloc(1);
+    HIP_TRY(hipMemsetAsync(total.ptr, 0, sizeof(unsigned long long), b->stream));
+    const uint32_t n = std::max(spp, 1u);
+    for (uint32_t done = 0; done < n; done += 4096u) {
+      const uint32_t nb = std::min(4096u, n - done);
+      k_count_traces<<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(p, ds, done, total.ptr);
+      HIP_TRY(hipGetLastError());
+    }
+    unsigned long long t = 0;
+    HIP_TRY(hipMemcpyAsync(&t, total.ptr, sizeof(t), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    *traces = t;
+  });
 }
 
 int lw_hip_render_hq(lw_hip_backend *b, const lw_scene_view *view, uint32_t spp,

@@ -182,6 +182,20 @@ def test_device_image_error_against_the_f64_render(hip_backend, oracle, frame0):
     assert d.mean() < 1e-3
 
 
+def test_device_trace_count_is_the_oracles(hip_backend, oracle, frame0, frame1):
+    # lw_hip_count_traces against the instrumented oracle's trace calls for the same render.
+    for fr, spp in [(frame0, 4), (frame1, 2)]:
+        sc = hip_backend.upload(fr)
+        try:
+            got = hip_backend.count_traces(sc, fr.params, spp)
+        finally:
+            sc.free()
+        _, c = oracle.render_hq_counted(fr.view(), spp)
+        w, h = int(fr.params["frame"][0]), int(fr.params["frame"][1])
+        print(f"{w}x{h}x{spp}: {got} traces, {got / (w * h * spp):.3f} per sample")
+        assert got == c.traces
+
+
 def test_render_hq_kat_scenes_on_the_device(hip_backend):
     # tests.rs:475-492: the lamp seen from the camera is 0.5 * 13 in every sample.
     s = scenes.lamp_scene(EMITTER_SAMPLING_ONE, False)
