@@ -225,8 +225,9 @@ def sky_timing(b, frames, cpu: bool) -> dict:
 def quick_timing(b, frames, cfg, cpu: bool, reps: int = 5) -> dict:
     """Quick mode on the device (lw_hip_render_quick) at the configuration's size: the synthetic
     world's mesh at draw distance 128; the best of a few calls' device time (inputs resident)
-    and whole-call time (uploads and download included). With `cpu`, the oracle's
-    single-threaded rasterizer on the same frame, timed once."""
+    and whole-call time (uploads and download included). With `cpu`, the oracle's rasterizer
+    on the same frame with its tiles on the host's cores (as raster.rs runs them on rayon's
+    pool), timed once, and its pixels against the device's."""
     frame, _ = frames.quick_frame(cfg)
     b.render_quick(frame)  # makes the log2 table
     dev, call = [], []
@@ -240,9 +241,13 @@ def quick_timing(b, frames, cfg, cpu: bool, reps: int = 5) -> dict:
     if cpu:
         from oracle import ffi
         ffi.build()
+        cores = host_cores()
+        got = b.render_quick(frame)
         t = time.perf_counter()
-        ffi.rasterize(frame)
+        want = ffi.rasterize(frame, cores)
         out["cpu_port_ms"] = (time.perf_counter() - t) * 1e3
+        out["cpu_port_cores"] = cores
+        out["pixels_identical"] = bool(np.array_equal(got, want))
     return out
 
 

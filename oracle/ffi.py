@@ -115,7 +115,7 @@ def lib():
         L.lwo_blend_chunky_sky.restype = i32
         L.lwo_log2f_band.argtypes = [ctypes.c_uint, vp, vp, vp, ctypes.c_uint64]
         L.lwo_log2f_band.restype = None
-        L.lwo_rasterize.argtypes = [vp, sz, vp, vp, vp, vp, vp, u32, u32, u32, u32, vp, vp, vp]
+        L.lwo_rasterize.argtypes = [vp, sz, vp, vp, vp, vp, vp, u32, u32, u32, u32, vp, vp, ctypes.c_uint, vp]
         L.lwo_rasterize.restype = i32
         L.lwo_atlas_sample.argtypes = [vp, vp, vp, u32, u32, i32, vp, sz, vp]
         L.lwo_atlas_sample.restype = None
@@ -619,9 +619,9 @@ def log2f_band(threads: int = 0):
     return (int(st[0]), int(st[1]), int(st[2]), int(st[3]), mr.value)
 
 
-def rasterize(frame):
+def rasterize(frame, threads: int = 1):
     """rasterize (raster.rs:168-372) of a lumenwright_amd.abi.QuickFrame with the client's
-    stars: width * height packed RGBA8 words."""
+    stars, its tiles on `threads` threads: width * height packed RGBA8 words."""
     from lumenwright_amd.abi import QUICK_VERTEX
     m = frame.mesh
     v = np.ascontiguousarray(m.vertices, dtype=QUICK_VERTEX)
@@ -636,7 +636,7 @@ def rasterize(frame):
     _check(lib().lwo_rasterize(_addr(v), v.size, ptrs, lens, _addr(atlas), _addr(size),
                                _addr(mips) if mips.size else None, frame.mip_levels, frame.mip_rows,
                                frame.width, frame.height, _addr(vp), ctypes.byref(frame.sky),
-                               _addr(out)))
+                               threads, _addr(out)))
     return out
 
 

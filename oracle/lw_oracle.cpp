@@ -593,12 +593,12 @@ void lwo_game_sky_maps(const lwo_gsky::Quick *lens, const lwo_gsky::Quick *light
 }
 
 // rasterize (raster.rs:168-372): a quick frame's packed RGBA8 pixels, row major, with the
-// client's stars. lists: the opaque, translucent and see-through index lists, list_len their
-// lengths in indices.
+// client's stars, the tiles on `threads` threads. lists: the opaque, translucent and
+// see-through index lists, list_len their lengths in indices.
 int lwo_rasterize(const lwo_quick::Vertex *verts, size_t n_verts, const uint32_t *const *lists,
                   const size_t *list_len, const uint32_t *atlas, const uint32_t *atlas_size, const uint32_t *mips,
                   uint32_t levels, uint32_t mip_rows, uint32_t width, uint32_t height, const float *view_proj,
-                  const lwo_gsky::Quick *sky, uint32_t *out) {
+                  const lwo_gsky::Quick *sky, unsigned threads, uint32_t *out) {
   return guarded([&] {
     static const std::vector<uint32_t> table = lwo_gsky::star_table();
     for (int k = 0; k < 3; k++)
@@ -621,7 +621,7 @@ int lwo_rasterize(const lwo_quick::Vertex *verts, size_t n_verts, const uint32_t
     f.height = height;
     std::memcpy(f.view_proj, view_proj, sizeof(f.view_proj));
     f.sky = sky;
-    const std::vector<uint32_t> px = lwo_quick::rasterize(f, table);
+    const std::vector<uint32_t> px = lwo_quick::rasterize(f, table, threads ? threads : 1);
     std::copy(px.begin(), px.end(), out);
   });
 }

@@ -10,9 +10,11 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "lw_oracle_sky.hpp"
@@ -242,7 +244,7 @@ static float fog_factor(const Quick &q, float x, float y, float z) {
 }
 
 // raster.rs:168-372
-static std::vector<uint32_t> rasterize(const Frame &f, const std::vector<uint32_t> &stars) {
+static std::vector<uint32_t> rasterize(const Frame &f, const std::vector<uint32_t> &stars, unsigned threads = 1) {
   const float w = (float)f.width, h = (float)f.height;
   const float *m = f.view_proj;
   std::vector<ClipVert> verts(f.n_verts);
@@ -289,7 +291,11 @@ static std::vector<uint32_t> rasterize(const Frame &f, const std::vector<uint32_
   const Quick &sky = *f.sky;
   const lwo_gsky::V fog = lwo_gsky::v4(sky.fog_color);
   std::vector<uint32_t> out((size_t)f.width * f.height);
-  for (uint32_t ti = 0; ti < tiles_x * tiles_y; ti++) {
+  // The tiles in parallel, taken dynamically (raster.rs:263 maps them with rayon); each
+  // writes only its own pixels.
+  std::atomic<uint32_t> next_tile{0};
+  auto worker = [&]() {
+  for (uint32_t ti = next_tile.fetch_add(1); ti < tiles_x * tiles_y; ti = next_tile.fetch_add(1)) {
     const uint32_t x0 = (ti % tiles_x) * TILE, y0 = (ti / tiles_x) * TILE;
     const uint32_t tw = std::min(TILE, f.width - x0), th = std::min(TILE, f.height - y0);
     std::vector<uint32_t> color((size_t)tw * th);
@@ -354,6 +360,11 @@ static std::vector<uint32_t> rasterize(const Frame &f, const std::vector<uint32_
     }
     for (uint32_t k = 0; k < tw * th; k++) out[(size_t)(y0 + k / tw) * f.width + x0 + k % tw] = color[k];
   }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; t++) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
   return out;
 }
 
