@@ -157,7 +157,8 @@ def build_timing(b, scene, sky, cpu: bool, reps: int = 3) -> dict:
     """lw_hip_scene_build of the frame's world from its interned voxel volume: the best of a
     few builds -- the device span (HIP events from the first layout kernel to the last) and the
     whole call with the volume's upload -- and the volume's bytes over the device span. With
-    `cpu`, the oracle's single-threaded restatement of the same layout, timed once."""
+    `cpu`, the oracle's restatement of the same layout, timed once: single-threaded, as the
+    reference runs this part (voxel.rs:305-491 and emitter_list have no rayon)."""
     vol = scene.volume
     best = None
     for _ in range(reps):
@@ -175,6 +176,7 @@ def build_timing(b, scene, sky, cpu: bool, reps: int = 3) -> dict:
         t = time.perf_counter()
         ffi.build_layout(vol)
         out["cpu_port_ms"] = (time.perf_counter() - t) * 1e3
+        out["cpu_port_cores"] = 1
     return out
 
 
@@ -182,7 +184,8 @@ def sky_timing(b, frames, cpu: bool) -> dict:
     """Sky preparation on the device: the game's two maps (lw_hip_game_sky_maps, the dusk sky
     of the reference's own test, sky.rs:1462-1491), Chunky's sky cache and its blend into
     the maps; the best device time of a few calls each. With `cpu`, the oracle's
-    single-threaded game_sky_maps timed once."""
+    game_sky_maps with its texels on the host's cores (sky.rs:869-872 maps them with rayon),
+    timed once."""
     cam = frames.CameraRaysC()
     cam.forward[:] = [-0.83153975, -0.3506493, 0.43079773]
     cam.right[:] = [-0.45915133, 0.0, -0.888358]
@@ -216,9 +219,11 @@ def sky_timing(b, frames, cpu: bool) -> dict:
     if cpu:
         from oracle import ffi
         ffi.build()
+        cores = host_cores()
         t = time.perf_counter()
-        ffi.game_sky_maps(lens, light, 0.378182, colour, atlas, 2)
+        ffi.game_sky_maps(lens, light, 0.378182, colour, atlas, 2, threads=cores)
         out["cpu_port_game_sky_maps_ms"] = (time.perf_counter() - t) * 1e3
+        out["cpu_port_cores"] = cores
     return out
 
 

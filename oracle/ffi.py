@@ -109,7 +109,7 @@ def lib():
         L.lwo_lightmap_sky.restype = None
         L.lwo_quick_background.argtypes = [vp, vp, sz, vp, u32, vp]
         L.lwo_quick_background.restype = None
-        L.lwo_game_sky_maps.argtypes = [vp, vp, ctypes.c_float, vp, vp, u32, vp, vp]
+        L.lwo_game_sky_maps.argtypes = [vp, vp, ctypes.c_float, vp, vp, u32, ctypes.c_uint, vp, vp]
         L.lwo_game_sky_maps.restype = None
         L.lwo_blend_chunky_sky.argtypes = [vp, ctypes.c_float, vp, vp]
         L.lwo_blend_chunky_sky.restype = i32
@@ -600,13 +600,14 @@ def quick_background(q, d, atlas, atlas_w: int):
     return out
 
 
-def game_sky_maps(lens, light, factor: float, colour, atlas, atlas_w: int):
-    """game_sky_maps: (the two maps, dict of light, mean, scale, veil)."""
+def game_sky_maps(lens, light, factor: float, colour, atlas, atlas_w: int, threads: int = 0):
+    """game_sky_maps, the texels on `threads` threads (all cores by default): (the two maps,
+    dict of light, mean, scale, veil)."""
     out = np.zeros((2048 * 1024 + 360 * 180) * 3, dtype="<f4")
     info = np.zeros(10, dtype="<f4")
     at = np.ascontiguousarray(atlas, dtype="<u4")
     lib().lwo_game_sky_maps(ctypes.byref(lens), ctypes.byref(light), factor, _addr(_f32(colour)),
-                            _addr(at), atlas_w, _addr(out), _addr(info))
+                            _addr(at), atlas_w, threads or os.cpu_count() or 1, _addr(out), _addr(info))
     return out, {"light": list(info[0:3]), "mean": list(info[3:6]), "scale": float(info[6]),
                  "veil": list(info[7:10])}
 

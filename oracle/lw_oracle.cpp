@@ -587,9 +587,11 @@ void lwo_quick_background(const lwo_gsky::Quick *q, const float *d, size_t n, co
 
 // game_sky_maps (sky.rs:906-953): out gets both maps; info light[3], mean[3], scale, veil[3].
 void lwo_game_sky_maps(const lwo_gsky::Quick *lens, const lwo_gsky::Quick *light, float factor,
-                       const float *colour, const uint32_t *atlas, uint32_t atlas_w, float *out, float *info) {
+                       const float *colour, const uint32_t *atlas, uint32_t atlas_w, unsigned threads, float *out,
+                       float *info) {
   static const std::vector<uint32_t> t = lwo_gsky::star_table();
-  lwo_gsky::game_sky_maps(*lens, *light, factor, {colour[0], colour[1], colour[2]}, atlas, atlas_w, t, out, info);
+  lwo_gsky::game_sky_maps(*lens, *light, factor, {colour[0], colour[1], colour[2]}, atlas, atlas_w, t, out, info,
+                          threads ? threads : 1);
 }
 
 // rasterize (raster.rs:168-372): a quick frame's packed RGBA8 pixels, row major, with the

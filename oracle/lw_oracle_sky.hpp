@@ -10,7 +10,9 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -352,15 +354,24 @@ static V map_direction(uint32_t i, uint32_t j, uint32_t w, uint32_t h) {
   return {std::cos(el) * std::cos(az), std::sin(el), std::cos(el) * std::sin(az)};
 }
 
-// sky.rs:864-881: c^2.2 per channel (math.rs GAMMA)
+// sky.rs:864-881: c^2.2 per channel (math.rs GAMMA); the texels on `threads` threads, rows
+// taken dynamically, as the reference maps them with rayon (:869-872).
 static void draw_map(const Quick &q, uint32_t w, uint32_t h, const uint32_t *atlas, uint32_t atlas_w,
-                     const std::vector<uint32_t> &table, float *out) {
-  for (uint32_t k = 0; k < w * h; k++) {
-    V c = background(q, map_direction(k % w, k / w, w, h), atlas, atlas_w, table);
-    out[3 * k] = std::pow(c.x, 2.2f);
-    out[3 * k + 1] = std::pow(c.y, 2.2f);
-    out[3 * k + 2] = std::pow(c.z, 2.2f);
-  }
+                     const std::vector<uint32_t> &table, float *out, unsigned threads = 1) {
+  std::atomic<uint32_t> next_row{0};
+  auto worker = [&]() {
+    for (uint32_t j = next_row.fetch_add(1); j < h; j = next_row.fetch_add(1))
+      for (uint32_t k = j * w; k < (j + 1) * w; k++) {
+        V c = background(q, map_direction(k % w, k / w, w, h), atlas, atlas_w, table);
+        out[3 * k] = std::pow(c.x, 2.2f);
+        out[3 * k + 1] = std::pow(c.y, 2.2f);
+        out[3 * k + 2] = std::pow(c.z, 2.2f);
+      }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < threads; t++) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
 }
 
 // sky.rs:798-815
@@ -380,10 +391,11 @@ static V lightmap_sky(float factor, V colour, float brightness) {
 
 // sky.rs:906-953. out: the lens's map then the light's; info: light, mean, scale, veil.
 static void game_sky_maps(const Quick &lens, const Quick &light, float factor, V colour, const uint32_t *atlas,
-                          uint32_t atlas_w, const std::vector<uint32_t> &table, float *out, float info[10]) {
+                          uint32_t atlas_w, const std::vector<uint32_t> &table, float *out, float info[10],
+                          unsigned threads = 1) {
   float *seen = out, *game = out + (size_t)LENS_W * LENS_H * 3;
-  draw_map(lens, LENS_W, LENS_H, atlas, atlas_w, table, seen);
-  draw_map(light, LIGHT_W, LIGHT_H, atlas, atlas_w, table, game);
+  draw_map(lens, LENS_W, LENS_H, atlas, atlas_w, table, seen, threads);
+  draw_map(light, LIGHT_W, LIGHT_H, atlas, atlas_w, table, game, threads);
   double acc[3] = {0, 0, 0}, weight = 0.0;
   for (uint32_t k = 0; k < LIGHT_W * LIGHT_H; k++) {
     V d = map_direction(k % LIGHT_W, k / LIGHT_W, LIGHT_W, LIGHT_H);
