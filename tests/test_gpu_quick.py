@@ -100,6 +100,22 @@ def test_world_frames_are_the_oracles(hip_backend, oracle, which, size):
     assert len(np.unique(got)) > 100
 
 
+def test_a_night_frame_is_the_oracles(hip_backend, oracle):
+    # The moon's sprite (flipped axes) near the zenith and bright stars, the camera looking up
+    # past the config-0 world's hills.
+    cfg = frames.config(0, pitch=-75.0)
+    f, atlas = frames.quick_frame(cfg, width=320, height=180)
+    moon = frames.celestial_direction(10.0)
+    d = frames.sky_description(
+        sky=(0.02, 0.03, 0.08), fog=(0.04, 0.05, 0.09), sky_fog_end=128.0, render_fog=(115.2, 128.0),
+        env_fog=(0.0, 1024.0), sky_disc=True, sunrise=(0.0, 0.0, 0.0, 0.0), sun_angle=190.0,
+        moon=(moon, atlas.sun), star_angle=190.0, star_brightness=0.9)
+    f.sky = frames.quick_sky_new(frames.camera_rays(cfg.yaw, cfg.pitch, cfg.fov_y, 320, 180), 320, 180, d)
+    got = same(hip_backend, oracle, f, "night")
+    r, b = got & 0xFF, (got >> 16) & 0xFF
+    assert np.count_nonzero((r > 150) & (b < 150)) > 50  # the moon's yellow disc
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (1000, 3), (3, 700), (33, 31)])
 def test_odd_frame_sizes(hip_backend, oracle, w, h):
     cfg = frames.config(0)
