@@ -196,6 +196,27 @@ def test_device_trace_count_is_the_oracles(hip_backend, oracle, frame0, frame1):
         assert got == c.traces
 
 
+def test_water_plane_on_the_device(hip_backend, oracle):
+    # Chunky's water plane (integrator.rs:506-534, drawn at no configuration): the device's
+    # samples equal the oracle's bit for bit at three incidences, and on the config-0 world
+    # with the plane drawn below its terrain and the camera looking out past the grid's edge.
+    from test_water_shadow_sun import water_plane_scene
+    for cos_i in (0.9, 0.4, 0.15):
+        v = water_plane_scene(cos_i).view()
+        n = 2048
+        want = oracle.render_samples(v, np.zeros(n), np.zeros(n), np.arange(n))
+        got = hip_backend.render_samples(v, np.zeros(n), np.zeros(n), np.arange(n))
+        assert np.array_equal(got.view("<u4"), want.view("<u4")), cos_i
+    fr = frames.make_frame(frames.config(0, yaw=90.0, pitch=10.0, width=160, height=90))
+    fr.params["water_plane"] = [62.0, 1.0, 0.0, 0.0]
+    v = fr.view()
+    px, py, s = sample_set(fr.params, 20000, 16, 8)
+    want = oracle.render_samples(v, px, py, s)
+    got = hip_backend.render_samples(v, px, py, s)
+    same, rel = radiance_report(got, want)
+    assert same.all(), (int((~same).sum()), rel.max())
+
+
 def test_render_hq_kat_scenes_on_the_device(hip_backend):
     # tests.rs:475-492: the lamp seen from the camera is 0.5 * 13 in every sample.
     s = scenes.lamp_scene(EMITTER_SAMPLING_ONE, False)

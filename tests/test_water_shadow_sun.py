@@ -227,3 +227,47 @@ def test_sun_texture_reads_the_sprite_texel(oracle):
     assert np.all(oracle.sun_texture(v, (-s).astype("<f4")[None]) == 0.0)
     sc0, *_ = sun_view(sprite=(3, 1, 0, 4))
     assert np.all(oracle.sun_texture(sc0.view(), s.astype("<f4")[None]) == 0.0)
+
+
+def water_plane_scene(cos_i):
+    """The reference's map test scene (tests.rs:804-831: lens map 5.0, light map 0.3) with
+    Chunky's water plane at y = 10 drawn, and a 1-pixel camera outside the grid's columns
+    (x < 0) looking down onto the plane at incidence acos(cos_i)."""
+    import scenes as sc
+    s = sc.test_scene(False)
+    p = s.params
+    p["sky_map"] = [4.0, 2.0, 2.0, 2.0]
+    s.sky = np.concatenate([np.full(24, 5.0), np.full(12, 0.3)]).astype("<f4")
+    p["water_plane"] = [10.0, 1.0, 0.0, 0.0]
+    p["water_color"] = [*SIGMA, IOR]
+    p["frame"] = [1, 1, 0, 1]
+    sin_i = np.sqrt(1 - cos_i ** 2)
+    fwd = np.array([-sin_i, -cos_i, 0.0], "<f4")
+    right = np.array([0.0, 0.0, 1.0], "<f4")
+    up = np.cross(right, fwd).astype("<f4")
+    p["cam_pos"] = [-30.0, 40.0, 8.0, 1e-4]  # tan(fov / 2): a near-pinhole 1-pixel frame
+    p["cam_fwd"] = [*fwd, 1.0]
+    p["cam_right"] = [*right, 0.0]
+    p["cam_up"] = [*up, 0.0]
+    return s
+
+
+def test_the_water_plane_reflects_at_schlicks_rate(oracle):
+    # integrator.rs:506-534 and :577-590. A camera ray meets the plane outside the grid's
+    # columns. Reflected, it leaves upward and reads the lens map (5.0). Refracted, it stays
+    # under the plane and in the water, and no sky is added. The mean over the samples is
+    # R(cos_i) * 5.0.
+    r0 = ((1 - IOR) / (1 + IOR)) ** 2
+    n = 4096
+    for cos_i in (0.9, 0.4, 0.15):
+        s = water_plane_scene(cos_i)
+        c = oracle.render_samples(s.view(), np.zeros(n), np.zeros(n), np.arange(n)).astype(np.float64)
+        assert np.all((np.abs(c - 5.0) < 1e-5) | (c == 0.0))  # reflected: the lens map; refracted: nothing
+        frac = (c[:, 0] > 0).mean()
+        want = r0 + (1 - r0) * (1 - cos_i) ** 5
+        assert abs(frac - want) < 4 * np.sqrt(want * (1 - want) / n) + 2e-3, (cos_i, frac, want)
+    # not drawn (water_plane.y = 0): the ray passes the plane and reads the lens map
+    s = water_plane_scene(0.4)
+    s.params["water_plane"] = [10.0, 0.0, 0.0, 0.0]
+    c = oracle.render_samples(s.view(), np.zeros(16), np.zeros(16), np.arange(16))
+    assert np.all(np.abs(c - 5.0) < 1e-5)
