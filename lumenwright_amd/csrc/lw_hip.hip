@@ -1246,10 +1246,6 @@ void validate(const lw_scene_view &v) {
   const int64_t aw = (int32_t)p.off2[1], ah = (int32_t)p.off2[2];
   if (aw < 1 || ah < 1 || (uint64_t)(aw * ah) > v.atlas_len) fail("atlas size does not fit the atlas");
   if ((uint64_t)p.see_through[0] + p.see_through[1] > v.quads_len) fail("see-through range past the quads");
-  for (size_t q = 0; q < v.quads_len; q++) {
-    const lw_quad &Q = v.quads[q];
-    (void)Q;
-  }
   // emitters: the grid's cells, lists and emitters stay in their buffers.
   if (p.misc_u[3] > 0 && p.emitter[0] != LW_EMITTER_SAMPLING_NONE) {
     const uint32_t *g = v.emitter_grid;
