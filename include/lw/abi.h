@@ -245,7 +245,9 @@ void lw_hip_close(lw_hip_backend *b);
 /* ---- the kernel crate's public functions, on the device (parity surface) ---- */
 
 /* trace (crates/lumenwright-kernel/src/trace.rs:137-290) for n rays: o, d are n*3
- * floats, tmin/tmax n floats each; out[i].hit says whether ray i met a surface. */
+ * floats, tmin/tmax n floats each; out[i].hit says whether ray i met a surface. The hit
+ * fields come from the walk the renderers run, which stops once every brick left toward the
+ * ray's octant is empty; out[i].cell from the whole walk. */
 int lw_hip_trace(lw_hip_backend *b, const lw_scene_view *view, size_t n, const float *o,
                  const float *d, const float *tmin, const float *tmax, lw_hit *out);
 
@@ -338,13 +340,17 @@ int lw_hip_scene_build(lw_hip_backend *b, const lw_voxel_scene *vs, lw_scene_inf
                        lw_build_stats *stats, lw_hip_scene **out);
 
 /* Copies one buffer of a resident scene to the host: LW_SCENE_GRID and
- * LW_SCENE_EMITTER_GRID (uint32), LW_SCENE_EMITTERS (lw_emitter), LW_SCENE_QUADS (lw_quad).
+ * LW_SCENE_EMITTER_GRID (uint32), LW_SCENE_EMITTERS (lw_emitter), LW_SCENE_QUADS (lw_quad),
+ * LW_SCENE_OCTANT_BOXES (uint8 per brick, numbered (y * dz + z) * dx + x: bit k, for
+ * k = [+x] | [+y] << 1 | [+z] << 2, is set when the box from the brick to the grid's end
+ * toward octant k holds a brick whose top-level or overlay entry is not EMPTY).
  * *len receives the element count; with dst null only *len is written, otherwise cap (in
  * elements) must hold the buffer. No counterpart in the reference. */
 #define LW_SCENE_GRID 0u
 #define LW_SCENE_EMITTERS 1u
 #define LW_SCENE_EMITTER_GRID 2u
 #define LW_SCENE_QUADS 3u
+#define LW_SCENE_OCTANT_BOXES 4u
 int lw_hip_scene_read(lw_hip_backend *b, const lw_hip_scene *scene, uint32_t which, void *dst,
                       size_t cap, size_t *len);
 

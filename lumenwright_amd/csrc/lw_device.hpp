@@ -49,6 +49,10 @@ struct DevScene {
   // (bit ((y/4)*4 + z/4)*4 + x/4). A voxel of an empty sub-block is model 0 without a
   // load of its brick word.
   const uint2 *__restrict__ occupancy;
+  // Per brick, bit k (k = [+x] | [+y] << 1 | [+z] << 2) is set when the box of bricks from
+  // this one to the grid's end toward octant k (this brick included) holds a brick or an
+  // overlay that is not EMPTY.
+  const uint8_t *__restrict__ octant_boxes;
 };
 
 struct Hit {
@@ -221,7 +225,8 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
   const uint32_t *ov_top = s.grid + p.off[2];
   const float best_t = t_end + 1e-3f;
 
-  uint32_t cached_b = UINT32_MAX, entry = EMPTY, ov = EMPTY;
+  uint32_t cached_b = UINT32_MAX, entry = EMPTY, ov = EMPTY, boxes = 0xFFu;
+  const uint32_t octant = (step[0] >= 0 ? 1u : 0u) | (step[1] >= 0 ? 2u : 0u) | (step[2] >= 0 ? 4u : 0u);
   uint2 occ = make_uint2(0u, 0u);
   for (uint32_t it = 0; it < MAX_STEPS; it++) {
     const uint32_t bx = (uint32_t)(cell[0] >> 4);
@@ -231,6 +236,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
     if (b != cached_b) {
       entry = top[b];
       ov = ov_top[b];
+      if (cell_out == nullptr) boxes = s.octant_boxes[b];
       cached_b = b;
       if (entry != EMPTY && (entry & UNIFORM) == 0) occ = s.occupancy[entry];
     }
@@ -241,6 +247,15 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
     }
 
     if (entry == EMPTY && ov == EMPTY) {
+      // Each brick coordinate of the walk only ever moves in its axis's step direction (a
+      // cell step moves by the step, a brick skip crosses one face in the step's direction
+      // and clamps the other coordinates into the brick, a sub-block jump moves cells by the
+      // step), and not at all on an axis with step 0. So every later brick lies in the box
+      // toward the ray's octant (step 0 counted as +). When that box holds only EMPTY bricks
+      // without overlays, no quad will be tested and the result is "no hit" wherever the
+      // walk would end (t_end, the grid's faces or MAX_STEPS). Only the final cell differs,
+      // so a caller asking for it (cell_out) gets the whole walk.
+      if (cell_out == nullptr && ((boxes >> octant) & 1u) == 0u) return false;
       // Skip the whole brick: jump to its exit face.
       const int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
       float t_out = INFINITY;

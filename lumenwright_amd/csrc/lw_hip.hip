@@ -150,10 +150,14 @@ __global__ void __launch_bounds__(kBlock) k_trace(const lw_params p, const DevSc
                                                   lw_hit *__restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Hit h;
+  // The hit fields come from the walk the renderers run (no cell report, so it stops once
+  // every brick left toward the ray's octant is empty); the final cell from a second, whole
+  // walk.
+  const V3 ro = V3{o[3 * i], o[3 * i + 1], o[3 * i + 2]}, rd = V3{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+  Hit h, h_whole;
   int32_t cell[3];
-  const bool hit = trace(s, p, V3{o[3 * i], o[3 * i + 1], o[3 * i + 2]},
-                         V3{d[3 * i], d[3 * i + 1], d[3 * i + 2]}, tmin[i], tmax[i], h, cell);
+  const bool hit = trace(s, p, ro, rd, tmin[i], tmax[i], h, nullptr);
+  (void)trace(s, p, ro, rd, tmin[i], tmax[i], h_whole, cell);
   lw_hit r;
   r.t = hit ? h.t : INFINITY;
   r.quad = hit ? h.quad : UINT32_MAX;
@@ -638,6 +642,51 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *__restrict__ p, u
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// DevScene::octant_boxes, in three kernels: k_octants_init marks each brick that is not
+// EMPTY (a top-level or overlay entry) with all eight bits; k_octants_scan then ORs the bits
+// along every line of bricks on one axis, toward +axis for the octants whose bit for that
+// axis is set and toward -axis for the others. After the x, y and z passes, bit k of a brick
+// is the OR over the box toward octant k. Bricks are numbered (y * dz + z) * dx + x.
+__global__ void __launch_bounds__(kBlock) k_octants_init(const uint32_t *__restrict__ grid, uint32_t off_top,
+                                                        uint32_t off_ov, uint64_t nb,
+                                                        uint8_t *__restrict__ boxes) {
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (uint64_t)gridDim.x * blockDim.x)
+    boxes[b] = (grid[off_top + b] != LW_EMPTY || grid[off_ov + b] != LW_EMPTY) ? 0xFFu : 0u;
+}
+
+// One lane per line along `axis` (0 x, 1 y, 2 z).
+__global__ void __launch_bounds__(kBlock) k_octants_scan(uint8_t *__restrict__ boxes, uint32_t dx, uint32_t dy,
+                                                        uint32_t dz, uint32_t axis) {
+  const uint64_t line = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t base, stride, len;
+  if (axis == 0) {  // lines (y, z): base (y * dz + z) * dx
+    if (line >= (uint64_t)dy * dz) return;
+    base = line * dx, stride = 1, len = dx;
+  } else if (axis == 1) {  // lines (z, x): base z * dx + x
+    if (line >= (uint64_t)dz * dx) return;
+    base = line, stride = (uint64_t)dx * dz, len = dy;
+  } else {  // lines (y, x): base y * dz * dx + x
+    if (line >= (uint64_t)dy * dx) return;
+    base = (line / dx) * dz * dx + line % dx, stride = dx, len = dz;
+  }
+  uint32_t plus = 0;
+  for (int k = 0; k < 8; k++)
+    if ((k >> axis) & 1) plus |= 1u << k;
+  const uint32_t minus = 0xFFu & ~plus;
+  uint32_t run = 0;
+  for (uint64_t i = len; i-- > 0;) {  // toward +axis: the OR of this brick and those after it
+    uint8_t &v = boxes[base + i * stride];
+    run |= v & plus;
+    v = (uint8_t)((v & minus) | run);
+  }
+  run = 0;
+  for (uint64_t i = 0; i < len; i++) {  // toward -axis: the OR of this brick and those before it
+    uint8_t &v = boxes[base + i * stride];
+    run |= v & minus;
+    v = (uint8_t)((v & plus) | run);
+  }
+}
+
 // Persistent path tracing: each lane runs one path at a time as a PathState; the wave
 // traces one ray per lane per iteration, whichever kind of ray each lane needs. A lane
 // whose path ends writes its radiance and takes the next work item: the wave's idle lanes
@@ -1095,6 +1144,7 @@ struct lw_hip_scene {
   DevBuf<float> sky;
   DevBuf<float> lut;
   DevBuf<uint2> occupancy;
+  DevBuf<uint8_t> octant_boxes;
   std::vector<uint32_t> quad_flags;  // host copy of each quad's flags (p0.w)
   // The params layout the scene was validated with. A scene built on the device
   // (lw_hip_scene_build) takes it from the first render's params, checked against `built`.
@@ -1106,7 +1156,7 @@ struct lw_hip_scene {
   uint32_t max_emitter_rays = 1;  // emitter face rays one bounce can trace
   DevScene dev() const {
     return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
-                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr, occupancy.ptr};
+                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr, occupancy.ptr, octant_boxes.ptr};
   }
 };
 
@@ -1335,6 +1385,25 @@ std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
   return occ;
 }
 
+// DevScene::octant_boxes of a scene whose grid is on the device.
+void build_octant_boxes(lw_hip_backend *b, lw_hip_scene &sc, const uint32_t dims[3], uint32_t off_top,
+                        uint32_t off_ov) {
+  const uint64_t nb = (uint64_t)dims[0] * dims[1] * dims[2];
+  sc.octant_boxes.alloc(std::max<uint64_t>(nb, 1));
+  const auto blocks = [](uint64_t n) {
+    return dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>((n + kBlock - 1) / kBlock, 1), 1u << 20));
+  };
+  k_octants_init<<<blocks(nb), dim3(kBlock), 0, b->stream>>>(sc.grid.ptr, off_top, off_ov, nb, sc.octant_boxes.ptr);
+  HIP_TRY(hipGetLastError());
+  const uint64_t lines[3] = {(uint64_t)dims[1] * dims[2], (uint64_t)dims[2] * dims[0], (uint64_t)dims[1] * dims[0]};
+  for (uint32_t axis = 0; axis < 3; axis++) {
+    k_octants_scan<<<blocks(lines[axis]), dim3(kBlock), 0, b->stream>>>(sc.octant_boxes.ptr, dims[0], dims[1],
+                                                                         dims[2], axis);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));  // a scene is whole when its call returns
+}
+
 void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   validate(v);
   HIP_TRY(hipSetDevice(b->device));
@@ -1356,6 +1425,7 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   const std::vector<uint2> occ = occupancy_masks(v);
   sc.occupancy.upload(occ.data(), occ.size(), b->stream);
   HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors above die with this scope
+  build_octant_boxes(b, sc, v.params->grid_dims, v.params->off[0], v.params->off[2]);
   sc.key = layout_key(*v.params);
   sc.key_set = true;
   // The most face rays one bounce's emitter sampling can trace (integrator.rs:441-469).
@@ -1476,6 +1546,7 @@ void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, l
   sc.emitters.adopt(o.emitters, o.emitters_len);
   sc.egrid.adopt(o.egrid, o.egrid_len);
   sc.occupancy.adopt(o.occupancy, o.occupancy_len);
+  build_octant_boxes(b, sc, v.dims, o.offsets[0], o.offsets[2]);
   uint64_t max_faces = 0;
   for (size_t m = 0; m < v.models_len; m++)
     if (v.model_flags[m] & LW_MODEL_EMITS) max_faces = std::max<uint64_t>(max_faces, v.models[m].quad_count);
@@ -2432,6 +2503,7 @@ int lw_hip_scene_read(lw_hip_backend *b, const lw_hip_scene *scene, uint32_t whi
       case LW_SCENE_EMITTERS: src = scene->emitters.ptr; n = scene->emitters.n; size = sizeof(lw_emitter); break;
       case LW_SCENE_EMITTER_GRID: src = scene->egrid.ptr; n = scene->egrid.n; size = 4; break;
       case LW_SCENE_QUADS: src = scene->quads.ptr; n = scene->quads.n; size = sizeof(lw_quad); break;
+      case LW_SCENE_OCTANT_BOXES: src = scene->octant_boxes.ptr; n = scene->octant_boxes.n; size = 1; break;
       default: throw InvalidScene{"unknown scene buffer"};
     }
     *len = n;

@@ -72,6 +72,8 @@ def lib():
         L.lwo_render_hq.restype = i32
         L.lwo_render_hq_counted.argtypes = [vp, u32, ctypes.c_uint, vp, vp]
         L.lwo_render_hq_counted.restype = i32
+        L.lwo_octant_boxes.argtypes = [vp, vp]
+        L.lwo_octant_boxes.restype = i32
         L.lwo_render_diff.argtypes = [vp, u32, vp, ctypes.c_uint, vp]
         L.lwo_render_diff.restype = i32
         L.lwo_adjoint.argtypes = [vp, u32, vp, vp, ctypes.c_uint, vp]
@@ -364,6 +366,16 @@ def render_hq_counted(view, spp: int, threads: int = 0):
     threads = threads or os.cpu_count() or 1
     _check(lib().lwo_render_hq_counted(view.ref, spp, threads, _addr(out), ctypes.byref(c)))
     return out, c
+
+
+def octant_boxes(view):
+    """Per brick, the octant bits the counted walk (and the device's walk) stops by: bit k
+    (k = [+x] | [+y] << 1 | [+z] << 2) set when the box toward octant k holds a brick that is
+    not EMPTY. Bricks are numbered (y * dz + z) * dx + x."""
+    dims = [int(x) for x in view.params["grid_dims"][:3]]
+    out = np.zeros(dims[0] * dims[1] * dims[2], dtype="u1")
+    _check(lib().lwo_octant_boxes(view.ref, _addr(out)))
+    return out
 
 
 def preetham_radiance(view, d):

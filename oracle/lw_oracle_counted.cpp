@@ -24,4 +24,16 @@ int lwo_render_hq_counted(const View *view, uint32_t spp, unsigned threads, floa
   }
 }
 
+// The per-brick octant bits the counted walk stops by (tl_octant_boxes), into out (one byte
+// per brick).
+int lwo_octant_boxes(const View *view, uint8_t *out) {
+  try {
+    const std::vector<uint8_t> boxes = octant_boxes(*view);
+    std::copy(boxes.begin(), boxes.end(), out);
+    return 0;
+  } catch (const std::exception &) {
+    return -1;
+  }
+}
+
 }  // extern "C"

@@ -112,6 +112,15 @@ static thread_local Counters *tl_counters = nullptr;
   } while (0)
 #endif
 
+#if LWO_COUNTING
+// The device's walk stops once every brick left toward the ray's octant is EMPTY with no
+// overlay (no quad can be tested after that, so its result is the same "no hit"), and the
+// counted walk stops there too, so that the counts are those of the walk the device runs.
+// Per brick, bit k (k = [+x] | [+y] << 1 | [+z] << 2, step 0 counted as +) is set when the
+// box from the brick to the grid's end toward octant k holds a brick that is not EMPTY.
+static thread_local const std::vector<uint8_t> *tl_octant_boxes = nullptr;
+#endif
+
 template <typename T>
 static inline const T &at(const T *buf, size_t len, size_t i, const char *what) {
   if (i >= len) throw OracleError(std::string("index out of bounds in ") + what);
@@ -517,13 +526,19 @@ static bool trace(const View &s, V3 o, V3 d, real tmin, real tmax, Hit &out,
     uint32_t b = (by * dims[2] + bz) * dims[0] + bx;
     uint32_t entry = grid_at(s, (size_t)(top_off + b));
     uint32_t ov = grid_at(s, (size_t)(ov_top_off + b));
-    LWO_COUNT(bytes, 8);
+    LWO_COUNT(bytes, 9);  // the top-level and overlay entries and the octant byte
     LWO_COUNT(dda_steps, 1);
     cell_out[0] = cell[0];
     cell_out[1] = cell[1];
     cell_out[2] = cell[2];
 
     if (entry == EMPTY && ov == EMPTY) {
+#if LWO_COUNTING
+      if (tl_octant_boxes) {
+        const uint32_t octant = (step[0] >= 0 ? 1u : 0u) | (step[1] >= 0 ? 2u : 0u) | (step[2] >= 0 ? 4u : 0u);
+        if ((((*tl_octant_boxes)[b] >> octant) & 1u) == 0u) return false;
+      }
+#endif
       // Skip the whole brick: jump to its exit face.
       int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
       real t_out = INFINITY;
@@ -1138,6 +1153,35 @@ static inline bool finite3(V3 c) {
   return std::isfinite(c.x) && std::isfinite(c.y) && std::isfinite(c.z);
 }
 
+#if LWO_COUNTING
+// tl_octant_boxes for a scene: each brick's bits by brute force over the bricks that are
+// not EMPTY, each of which sets the octant bits of every brick whose box it lies in.
+static std::vector<uint8_t> octant_boxes(const View &s) {
+  const Params &p = *s.params;
+  const int64_t dx = p.grid_dims[0], dy = p.grid_dims[1], dz = p.grid_dims[2];
+  std::vector<uint8_t> boxes((size_t)(dx * dy * dz), 0);
+  for (int64_t y = 0; y < dy; y++)
+    for (int64_t z = 0; z < dz; z++)
+      for (int64_t x = 0; x < dx; x++) {
+        const size_t b = (size_t)((y * dz + z) * dx + x);
+        if (grid_at(s, (size_t)p.off[0] + b) == EMPTY && grid_at(s, (size_t)p.off[2] + b) == EMPTY) continue;
+        // brick (x, y, z) lies in the +x box of bricks with bx <= x, the -x box of bx >= x, ...
+        for (int64_t by = 0; by < dy; by++)
+          for (int64_t bz = 0; bz < dz; bz++)
+            for (int64_t bx = 0; bx < dx; bx++) {
+              uint8_t &v = boxes[(size_t)((by * dz + bz) * dx + bx)];
+              for (uint32_t k = 0; k < 8; k++) {
+                const bool in_x = (k & 1u) ? bx <= x : bx >= x;
+                const bool in_y = (k & 2u) ? by <= y : by >= y;
+                const bool in_z = (k & 4u) ? bz <= z : bz >= z;
+                if (in_x && in_y && in_z) v |= (uint8_t)(1u << k);
+              }
+            }
+      }
+  return boxes;
+}
+#endif
+
 // CpuBackend::render_hq (crates/lumenwright/src/render/cpu/mod.rs:39-87): rows in
 // parallel (one row per task, taken dynamically as rayon's work stealing does), f64
 // per-pixel accumulation of the finite samples, divided by spp.
@@ -1152,10 +1196,14 @@ static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, re
   std::atomic<uint32_t> next_row{0};
   std::vector<std::string> errors(threads);
   std::vector<Counters> per_thread(threads);
+#if LWO_COUNTING
+  const std::vector<uint8_t> boxes = octant_boxes(view);
+#endif
   auto worker = [&](unsigned tid) {
 #if LWO_COUNTING
     per_thread[tid] = Counters{};
     tl_counters = counters ? &per_thread[tid] : nullptr;
+    tl_octant_boxes = counters ? &boxes : nullptr;
 #endif
     try {
       for (;;) {
@@ -1183,6 +1231,7 @@ static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, re
     }
 #if LWO_COUNTING
     tl_counters = nullptr;
+    tl_octant_boxes = nullptr;
 #endif
   };
   std::vector<std::thread> pool;

@@ -97,6 +97,48 @@ def test_scene(entity: bool) -> Scene:
                  np.zeros(EMITTER_GRID_HEADER, "<u4"), [])
 
 
+def tower_scene(filled: bool = True) -> Scene:
+    """3x6x3 bricks with three non-empty bricks in different rows and corners, for the walk
+    that stops once every brick left toward the ray's octant is empty: a uniform brick of
+    cubes at brick (1, 0, 1), an overlay-only brick at (0, 3, 2) holding a cutout quad in cell
+    (8, 53, 37), and a stored brick at (2, 4, 0) with one cube in cell (36, 69, 5). Without
+    `filled` every brick is EMPTY."""
+    quads = cube(ALPHA_OPAQUE)
+    models = [model(0, 0), model(0, 6)]
+    dx, dy, dz = 3, 6, 3
+    nb = dx * dy * dz
+    brick_of = lambda bx, by, bz: (by * dz + bz) * dx + bx
+    top = [EMPTY] * nb
+    ov_top = [EMPTY] * nb
+    brick = np.zeros(BRICK_WORDS, dtype="<u4")
+    ov_cells, ov_quads = [], []
+    if filled:
+        top[brick_of(1, 0, 1)] = UNIFORM | 1
+        top[brick_of(2, 4, 0)] = 0
+        local = (5 * 16 + 5) * 16 + 4
+        brick[local >> 1] |= 1 << ((local & 1) * 16)
+        quads.append(quad((8.5, 54.0, 37.0), (0, 0, 1.0), (0, -1.0, 0), ALPHA_CUTOUT))
+        ov_top[brick_of(0, 3, 2)] = 0
+        ov_cells = np.zeros(OV_CELL_WORDS, dtype="<u4")
+        cell = (5 * 16 + 5) * 16 + 8
+        ov_cells[cell * 2 + 1] = 1
+        ov_quads = [6]
+    grid = []
+    off_top = len(grid); grid += top
+    off_bricks = len(grid); grid += list(brick)
+    off_ov_top = len(grid); grid += ov_top
+    off_ov_cells = len(grid); grid += list(ov_cells)
+    off_ov_quads = len(grid); grid += ov_quads
+    grid.append(0)
+    p = new_params()
+    p["grid_dims"] = [dx, dy, dz, nb]
+    p["off"] = [off_top, off_bricks, off_ov_top, off_ov_cells]
+    p["off2"] = [off_ov_quads, 2, 1, len(quads)]
+    p["misc_f"] = [1.0, 1e-3, 0.0, 0.0]
+    return Scene(p, grid, models, quads, [0xFFFFFFFF, 0x00000000], np.zeros(1, EMITTER),
+                 np.zeros(EMITTER_GRID_HEADER, "<u4"), [])
+
+
 LAMP_EMITTANCE = 0.5   # tests.rs:301
 LAMP_INTENSITY = 13.0  # tests.rs:303
 

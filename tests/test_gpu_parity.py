@@ -73,8 +73,14 @@ def test_probe_and_name(hip_backend):
 
 @pytest.mark.parametrize("which", [0, 1])
 def test_trace_is_bit_exact_on_the_config_worlds(hip_backend, oracle, frame0, frame1, which):
+    import lumenwright_amd as lw
     fr = frame0 if which == 0 else frame1
     v = fr.view()
+    sc = hip_backend.upload(fr)
+    try:  # the walk's per-brick octant bits, made on the device from the uploaded grid
+        assert np.array_equal(hip_backend.scene_read(sc, lw.abi.SCENE_OCTANT_BOXES), oracle.octant_boxes(v))
+    finally:
+        sc.free()
 
     def otrace(o, d, tmin, tmax):
         return oracle.trace(v, o, d, tmin, tmax)
@@ -123,6 +129,41 @@ def test_trace_on_the_test_scenes_matches_the_oracle(hip_backend, oracle):
         want = oracle.trace(v, o, d)
         got = hip_backend.trace(v, o, d)
         assert trace_fields_equal(got, want).all()
+
+
+def test_the_renderers_walk_stops_when_its_octant_is_empty(hip_backend, oracle):
+    # lw_hip_trace reports hits from the walk the renderers run, which gives up on a ray once
+    # every brick left toward its octant is empty, and the final cell from the whole walk. The
+    # tower's three non-empty bricks sit in different rows and corners (one an overlay only);
+    # a third of the rays aim at the cutout quad and the lone cube on the two upper rows, a
+    # third rise or stay level, a third go anywhere.
+    import lumenwright_amd as lw
+    rng = np.random.default_rng(17)
+    targets = np.array([[8.5, 53.5, 37.5], [36.5, 69.5, 5.5]])
+    for filled in (True, False):
+        v = scenes.tower_scene(filled).view()
+        sc = hip_backend.upload(scenes.tower_scene(filled))
+        try:
+            assert np.array_equal(hip_backend.scene_read(sc, lw.abi.SCENE_OCTANT_BOXES), oracle.octant_boxes(v))
+        finally:
+            sc.free()
+        n = 30000
+        k = n // 3
+        o = rng.random((n, 3)) * [60.0, 120.0, 60.0] - [6.0, 12.0, 6.0]
+        d = rng.normal(size=(n, 3))
+        d[:k] = targets[rng.integers(0, 2, k)] + rng.uniform(-0.6, 0.6, (k, 3)) - o[:k]
+        d[k:2 * k, 1] = np.abs(d[k:2 * k, 1])
+        d[k:k + k // 4, 1] = 0.0
+        o, d = o.astype("<f4"), rays._unit(d).astype("<f4")
+        want = oracle.trace(v, o, d)
+        got = hip_backend.trace(v, o, d)
+        assert trace_fields_equal(got, want).all(), filled
+        if filled:
+            hit = got["hit"] == 1
+            assert np.count_nonzero(hit & (got["quad"] == 6)) > 100  # the overlay's quad
+            assert np.count_nonzero(hit & (got["cell"][:, 1] == 69)) > 100  # the lone cube
+        else:
+            assert not got["hit"].any()
 
 
 def sample_set(params, n, spp, seed):

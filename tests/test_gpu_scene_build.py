@@ -10,7 +10,8 @@ import pytest
 
 from lumenwright_amd import frames
 from lumenwright_amd.abi import (MODEL, MODEL_EMITS, MODEL_OCCLUDES, QUAD, SCENE_EMITTER_GRID,
-                                 SCENE_EMITTERS, SCENE_GRID, SCENE_QUADS, VoxelVolume)
+                                 SCENE_EMITTERS, SCENE_GRID, SCENE_OCTANT_BOXES, SCENE_QUADS,
+                                 VoxelVolume)
 from test_scene_build import (DEEP, emitter_volume, kernel_test_volume, random_entity_quads,
                               view_of)
 
@@ -24,9 +25,12 @@ def built_equals_oracle(b, oracle, vol, sky=None):
         em = b.scene_read(sc, SCENE_EMITTERS)
         eg = b.scene_read(sc, SCENE_EMITTER_GRID)
         qd = b.scene_read(sc, SCENE_QUADS)
+        boxes = b.scene_read(sc, SCENE_OCTANT_BOXES)
     finally:
         sc.free()
     og, oem, oeg, info = oracle.build_layout(vol)
+    # the walk's per-brick octant bits, made on the device from the built grid
+    assert np.array_equal(boxes, oracle.octant_boxes(view_of(vol, (og, oem, oeg, info))))
     assert g.size == og.size and np.array_equal(g, og), np.flatnonzero(g[:og.size] != og[:g.size])[:8]
     assert np.array_equal(em.view("u1"), oem.view("u1"))
     assert np.array_equal(eg, oeg)
