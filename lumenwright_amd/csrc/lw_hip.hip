@@ -654,36 +654,36 @@ __global__ void __launch_bounds__(kBlock) k_octants_init(const uint32_t *__restr
     boxes[b] = (grid[off_top + b] != LW_EMPTY || grid[off_ov + b] != LW_EMPTY) ? 0xFFu : 0u;
 }
 
-// One lane per line along `axis` (0 x, 1 y, 2 z).
+// Lines along `axis` (0 x, 1 y, 2 z), one per lane, grid-stride.
 __global__ void __launch_bounds__(kBlock) k_octants_scan(uint8_t *__restrict__ boxes, uint32_t dx, uint32_t dy,
                                                         uint32_t dz, uint32_t axis) {
-  const uint64_t line = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t base, stride, len;
-  if (axis == 0) {  // lines (y, z): base (y * dz + z) * dx
-    if (line >= (uint64_t)dy * dz) return;
-    base = line * dx, stride = 1, len = dx;
-  } else if (axis == 1) {  // lines (z, x): base z * dx + x
-    if (line >= (uint64_t)dz * dx) return;
-    base = line, stride = (uint64_t)dx * dz, len = dy;
-  } else {  // lines (y, x): base y * dz * dx + x
-    if (line >= (uint64_t)dy * dx) return;
-    base = (line / dx) * dz * dx + line % dx, stride = dx, len = dz;
-  }
   uint32_t plus = 0;
   for (int k = 0; k < 8; k++)
     if ((k >> axis) & 1) plus |= 1u << k;
   const uint32_t minus = 0xFFu & ~plus;
-  uint32_t run = 0;
-  for (uint64_t i = len; i-- > 0;) {  // toward +axis: the OR of this brick and those after it
-    uint8_t &v = boxes[base + i * stride];
-    run |= v & plus;
-    v = (uint8_t)((v & minus) | run);
-  }
-  run = 0;
-  for (uint64_t i = 0; i < len; i++) {  // toward -axis: the OR of this brick and those before it
-    uint8_t &v = boxes[base + i * stride];
-    run |= v & minus;
-    v = (uint8_t)((v & plus) | run);
+  const uint64_t lines = axis == 0 ? (uint64_t)dy * dz : (axis == 1 ? (uint64_t)dz * dx : (uint64_t)dy * dx);
+  for (uint64_t line = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; line < lines;
+       line += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t base, stride, len;
+    if (axis == 0) {  // lines (y, z): base (y * dz + z) * dx
+      base = line * dx, stride = 1, len = dx;
+    } else if (axis == 1) {  // lines (z, x): base z * dx + x
+      base = line, stride = (uint64_t)dx * dz, len = dy;
+    } else {  // lines (y, x): base y * dz * dx + x
+      base = (line / dx) * dz * dx + line % dx, stride = dx, len = dz;
+    }
+    uint32_t run = 0;
+    for (uint64_t i = len; i-- > 0;) {  // toward +axis: the OR of this brick and those after it
+      uint8_t &v = boxes[base + i * stride];
+      run |= v & plus;
+      v = (uint8_t)((v & minus) | run);
+    }
+    run = 0;
+    for (uint64_t i = 0; i < len; i++) {  // toward -axis: the OR of this brick and those before it
+      uint8_t &v = boxes[base + i * stride];
+      run |= v & minus;
+      v = (uint8_t)((v & plus) | run);
+    }
   }
 }
 
