@@ -303,11 +303,17 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
                           (((uint32_t)cell[0] & 15u) >> 2);
       const uint32_t bits = sb < 32u ? occ.x : occ.y;
       if (((bits >> (sb & 31u)) & 1u) == 0u) {
+        // When the 8^3 block holding the sub-block is empty too (its eight sub-blocks are the
+        // occupancy bits 0x330033 shifted by 2 x8 + 8 z8 + 32 y8), it is crossed the same way:
+        // the argument holds for any aligned box of empty cells inside one brick.
+        const uint32_t x8 = ((uint32_t)cell[0] & 15u) >> 3, y8 = ((uint32_t)cell[1] & 15u) >> 3,
+                       z8 = ((uint32_t)cell[2] & 15u) >> 3;
+        const int32_t span = ((y8 ? occ.y : occ.x) & (0x330033u << (2u * x8 + 8u * z8))) == 0u ? 8 : 4;
         int32_t far[3];
         float tf[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-          far[a] = (cell[a] & ~3) + (step[a] > 0 ? 3 : 0);
+          far[a] = (cell[a] & ~(span - 1)) + (step[a] > 0 ? span - 1 : 0);
           tf[a] = next_boundary(get(o, a), get(inv, a), far[a], step[a]);
         }
         int x;
