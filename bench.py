@@ -117,7 +117,7 @@ def roofline(config, samples, trace_ms, launches):
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)",
             "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"],
-            "note": "VALU-issue bound: 92% VALU busy at 13% lane utilisation, L2 hit 85% "
+            "note": "VALU-issue bound: 93% VALU busy at 13% lane utilisation, L2 hit 84% "
                     "(profiles/k_render_cfg1_720p_valu_l2.md)" if config == 1 else None}
 
 
