@@ -241,11 +241,11 @@ def test_an_empty_mesh_is_the_sky(oracle):
 
 def test_coverage_is_the_pixel_centres_inside(oracle):
     # Random front-facing triangles against an exact point-in-triangle test of the pixel
-    # centres (pixels within 1e-3 px of an edge are not judged); ragged 75 x 41 frame.
+    # centres (pixels within 1e-3 px of an edge are not checked); ragged 75 x 41 frame.
     w, h = 75, 41
     rng = np.random.default_rng(11)
     rays = pixel_rays(w, h)
-    judged = 0
+    checked = 0
     for _ in range(12):
         z = -rng.uniform(2, 20)
         pts = np.concatenate([rng.uniform(-1.2, 1.2, (3, 2)) * -z, np.full((3, 1), z)], 1)
@@ -273,8 +273,8 @@ def test_coverage_is_the_pixel_centres_inside(oracle):
         inside = np.all(dist <= 0, axis=0)
         clear = np.all(np.abs(dist) > 1e-3, axis=0)
         assert np.array_equal(drawn[clear], inside[clear])
-        judged += int(inside[clear].sum())
-    assert judged > 500
+        checked += int(inside[clear].sum())
+    assert checked > 500
 
 
 def test_back_faces_are_culled(oracle):
@@ -332,7 +332,7 @@ def test_translucent_surfaces_blend_over_what_is_behind(oracle):
     r = blend(q8(blend(0.0, 1.0, a)), 1.0, a2)
     # The quads' shared diagonal runs through pixel centres here (one a row): where a rounded
     # edge value is exactly zero both triangles cover the pixel (the edge test keeps zero) and
-    # blend the layer twice, so the diagonal is not judged here.
+    # blend the layer twice, so the diagonal is not checked here.
     Y, X = np.mgrid[0:48, 0:64]
     off = np.abs(X - (55 - Y)) > 1
     assert np.all(np.abs(img[off, 1] - g * 255) <= 1) and np.all(np.abs(img[off, 0] - r * 255) <= 1)

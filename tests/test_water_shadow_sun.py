@@ -201,7 +201,7 @@ def test_sun_texture_reads_the_sprite_texel(oracle):
     v = sc.view()
     lin = oracle.gamma_to_linear(np.arange(10, 190, 3, dtype="<f4") / np.float32(255.0)).astype(np.float64)
     rng = np.random.default_rng(3)
-    judged = 0
+    checked = 0
     for _ in range(2000):
         # angles a, b in [0, 2 width): the direction makes angle a - width with the plane
         # perpendicular to su (asin(d . su) = a - width), and likewise b with sv
@@ -215,14 +215,14 @@ def test_sun_texture_reads_the_sprite_texel(oracle):
             abs(a), abs(a - 2 * width), abs(b), abs(b - 2 * width)) < 1e-5
         if near_edge:
             continue
-        judged += 1
+        checked += 1
         if not inside:
             assert np.all(got == 0.0), (a, b, got)
             continue
         x, y = int(np.floor(fx)), int(np.floor(fy))
         want = lin[(1 + y) * 10 + 3 + x] * brightness
         assert abs(got[0] - want) < 1e-6 * max(1.0, want) and got[1] == 0.0, (a, b, x, y, got)
-    assert judged > 1500
+    assert checked > 1500
     # facing away (d . sun < 0.5) and a zero-width sprite: no texture
     assert np.all(oracle.sun_texture(v, (-s).astype("<f4")[None]) == 0.0)
     sc0, *_ = sun_view(sprite=(3, 1, 0, 4))
