@@ -214,10 +214,25 @@ class HipBackend:
                                                            len(views), ctypes.byref(desc), ptrs))
         return outs
 
+    @staticmethod
+    def _per_view_images(views, images, what: str):
+        """Each view's w*h*3 array as contiguous f32, checked against the view's size: the
+        library copies w*h*3 floats from every pointer."""
+        if len(images) != len(views):
+            raise ValueError(f"{len(images)} {what} arrays for {len(views)} views")
+        out = []
+        for i, ((params, _), a) in enumerate(zip(views, images)):
+            a = np.ascontiguousarray(a, dtype="<f4").reshape(-1)
+            want = int(params["frame"][0]) * int(params["frame"][1]) * 3
+            if a.size != want:
+                raise ValueError(f"view {i}: {what} has {a.size} values, the view needs {want}")
+            out.append(a)
+        return out
+
     def adjoint_batch(self, scene: HipScene, views, desc, n_grads: int, dldis) -> np.ndarray:
         """lw_hip_adjoint_batch: the gradient of sum over views of dldis[i] . I_i."""
         arr = self._views(views)
-        ds = [np.ascontiguousarray(d, dtype="<f4").reshape(-1) for d in dldis]
+        ds = self._per_view_images(views, dldis, "dL/dI")
         ptrs = (ctypes.c_void_p * len(views))(*[d.ctypes.data for d in ds])
         out = np.zeros(n_grads, dtype="<f8")
         self._check(_native.lib().lw_hip_adjoint_batch(self.handle, scene.handle, _addr(arr),
@@ -229,7 +244,7 @@ class HipBackend:
             beta2: float = 0.999, eps: float = 1e-8, min_factor: float = 1e-3) -> "HipFit":
         """lw_hip_fit_new: an inverse fit of desc's factors to the views' target images."""
         arr = self._views(views)
-        ts = [np.ascontiguousarray(t, dtype="<f4").reshape(-1) for t in targets]
+        ts = self._per_view_images(views, targets, "target")
         ptrs = (ctypes.c_void_p * len(views))(*[t.ctypes.data for t in ts])
         opt = FitOptionsC(lr, beta1, beta2, eps, min_factor)
         h = ctypes.c_void_p()
@@ -243,7 +258,9 @@ class HipBackend:
         """lw_hip_adjoint: sum over pixels of dL/dI . dI/dtheta, (n_grads,) f64."""
         p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
         g = np.ascontiguousarray(d_loss_d_image, dtype="<f4").reshape(-1)
-        assert g.size == int(p["frame"][0]) * int(p["frame"][1]) * 3
+        if g.size != int(p["frame"][0]) * int(p["frame"][1]) * 3:
+            raise ValueError(f"dL/dI has {g.size} values, the frame needs "
+                             f"{int(p['frame'][0]) * int(p['frame'][1]) * 3}")
         out = np.zeros(n_grads, dtype="<f8")
         self._check(_native.lib().lw_hip_adjoint(self.handle, scene.handle, _addr(p), spp,
                                                  ctypes.byref(desc), _addr(g), _addr(out)))

@@ -10,7 +10,10 @@ bench.py reads for its roofline figure. Counting rules (per sample):
   at (integrator.rs:326,630), 48 B per sky cache or sky map lookup (integrator.rs:183-185,
   210-216), and for emitter sampling 28 B of grid header, 8 B per cell pair, 4 B per list
   index, 32 B per emitter and 80 B per face (integrator.rs:359-372,391,454-455).
-Params (constant) and the per-sample output are not counted.
+Params (constant) and the per-sample output are not counted. The walk counted is the
+reference algorithm's whole walk (trace.rs:137-290); the device's walk, which stops once
+every brick left toward the ray's octant is empty, is counted beside it (its steps and bytes
+under "device_walk") but is not the roofline's numerator.
 
     python oracle/count_bytes.py [--spp N]
 """
@@ -32,12 +35,14 @@ def main():
     from lumenwright_amd import frames
     from oracle import ffi
     ffi.build()
-    out = {"rules": __doc__.split("Counting rules (per sample):")[1].split("Params")[0].strip(),
+    out = {"rules": __doc__.split("Counting rules (per sample):")[1].split("    python")[0].strip(),
            "counted_with": f"oracle render_hq over the first {args.spp} samples of every pixel"}
     for n in (0, 1):
         cfg = frames.CONFIGS[n]
         fr = frames.make_frame(cfg)
-        _, c = ffi.render_hq_counted(fr.view(), args.spp, threads=os.cpu_count() or 1)
+        _, c = ffi.render_hq_counted(fr.view(), args.spp, threads=os.cpu_count() or 1,
+                                     device_walk=False)
+        _, cd = ffi.render_hq_counted(fr.view(), args.spp, threads=os.cpu_count() or 1)
         s = c.samples
         out[f"config{n}"] = {
             "workload": cfg.name,
@@ -47,6 +52,8 @@ def main():
             "dda_steps_per_sample": c.dda_steps / s,
             "quads_tested_per_sample": c.quads_tested / s,
             "sky_lookups_per_sample": c.sky_lookups / s,
+            "device_walk": {"bytes_per_sample": cd.bytes / s,
+                            "dda_steps_per_sample": cd.dda_steps / s},
         }
         print(n, out[f"config{n}"], flush=True)
     path = ROOT / "profiles" / "b_alg.json"

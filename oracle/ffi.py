@@ -31,7 +31,10 @@ class Counters(ctypes.Structure):
     _fields_ = [("bytes", ctypes.c_uint64), ("dda_steps", ctypes.c_uint64),
                 ("quads_tested", ctypes.c_uint64), ("traces", ctypes.c_uint64),
                 ("samples", ctypes.c_uint64), ("sky_lookups", ctypes.c_uint64),
-                ("pad", ctypes.c_uint64 * 2)]
+                ("see_through_drawn", ctypes.c_uint64), ("see_through_forced", ctypes.c_uint64),
+                ("translucent_passes", ctypes.c_uint64), ("alpha_zero_rejects", ctypes.c_uint64),
+                ("shadow_water_exits", ctypes.c_uint64), ("shadow_translucent", ctypes.c_uint64),
+                ("water_interfaces", ctypes.c_uint64), ("pad", ctypes.c_uint64)]
 
 
 def lib():
@@ -70,7 +73,7 @@ def lib():
         L.lwo_render_samples.restype = i32
         L.lwo_render_hq.argtypes = [vp, u32, ctypes.c_uint, vp]
         L.lwo_render_hq.restype = i32
-        L.lwo_render_hq_counted.argtypes = [vp, u32, ctypes.c_uint, vp, vp]
+        L.lwo_render_hq_counted.argtypes = [vp, u32, ctypes.c_uint, vp, vp, i32]
         L.lwo_render_hq_counted.restype = i32
         L.lwo_octant_boxes.argtypes = [vp, vp]
         L.lwo_octant_boxes.restype = i32
@@ -358,13 +361,16 @@ def render_samples_f64(view, px, py, sample):
     return out
 
 
-def render_hq_counted(view, spp: int, threads: int = 0):
-    """render_hq with the algorithmic-byte counters; returns (image, Counters)."""
+def render_hq_counted(view, spp: int, threads: int = 0, device_walk: bool = True):
+    """render_hq with the algorithmic-byte and branch counters; returns (image, Counters).
+    device_walk: count the walk the device runs (stopped once every brick left toward the
+    ray's octant is empty); False: the reference's whole walk."""
     w, h = int(view.params["frame"][0]), int(view.params["frame"][1])
     out = np.zeros((h, w, 3), dtype="<f4")
     c = Counters()
     threads = threads or os.cpu_count() or 1
-    _check(lib().lwo_render_hq_counted(view.ref, spp, threads, _addr(out), ctypes.byref(c)))
+    _check(lib().lwo_render_hq_counted(view.ref, spp, threads, _addr(out), ctypes.byref(c),
+                                       1 if device_walk else 0))
     return out, c
 
 

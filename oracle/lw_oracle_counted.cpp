@@ -11,13 +11,15 @@ using namespace lwo_counted;
 extern "C" {
 
 // CpuBackend::render_hq (render/cpu/mod.rs:39-87) with the scene-buffer bytes, DDA
-// steps, quads tested, trace calls and samples summed into *counters.
+// steps, quads tested, trace calls, samples and branch counts summed into *counters, over
+// the device's walk (device_walk != 0: it stops once every brick left toward the ray's
+// octant is EMPTY) or the reference's whole walk (0).
 int lwo_render_hq_counted(const View *view, uint32_t spp, unsigned threads, float *out_rgb,
-                          Counters *counters) {
+                          Counters *counters, int device_walk) {
   if (threads == 0) threads = 1;
   try {
     *counters = Counters{};
-    render_hq(*view, spp, threads, out_rgb, counters);
+    render_hq(*view, spp, threads, out_rgb, counters, device_walk != 0);
     return 0;
   } catch (const std::exception &) {
     return -1;

@@ -89,7 +89,9 @@ struct HitOut {
   uint32_t hit, pad;
 };
 
-// Algorithmic scene-buffer bytes and trace calls (the measurement rules).
+// Algorithmic scene-buffer bytes and trace calls (the measurement rules), and how often the
+// rarer branches of render_sample and shadow were taken (so that a test can show a scene
+// reaches them).
 struct Counters {
   uint64_t bytes;
   uint64_t dda_steps;
@@ -97,7 +99,14 @@ struct Counters {
   uint64_t traces;
   uint64_t samples;
   uint64_t sky_lookups;
-  uint64_t pad[2];
+  uint64_t see_through_drawn;   // a see-through quad drawn instead of the surface met
+  uint64_t see_through_forced;  // ... of which translucent, taken by its alpha draw
+  uint64_t translucent_passes;  // a path passing through a translucent surface
+  uint64_t alpha_zero_rejects;  // a translucent quad met at a texel of alpha 0
+  uint64_t shadow_water_exits;  // a shadow segment leaving water (Beer-Lambert)
+  uint64_t shadow_translucent;  // a shadow segment filtered by a translucent surface
+  uint64_t water_interfaces;    // a path event at a water quad
+  uint64_t pad;
 };
 
 #if LWO_COUNTING
@@ -447,6 +456,7 @@ static inline void test_quad(const View &s, uint32_t qi, V3 o, V3 d, real tmin, 
     uint32_t tx = quad_texel(s, q, ss, tt);
     uint32_t alpha = tx >> 24;
     uint32_t mode = q_flags(q) & FLAG_ALPHA_MASK;
+    if (mode == ALPHA_TRANSLUCENT && alpha == 0u) LWO_COUNT(alpha_zero_rejects, 1);
     if ((mode == ALPHA_CUTOUT && alpha < 128u) || (mode == ALPHA_TRANSLUCENT && alpha == 0u)) return;
     best = Hit{t, qi, tx, ss, tt};
   }
@@ -526,7 +536,7 @@ static bool trace(const View &s, V3 o, V3 d, real tmin, real tmax, Hit &out,
     uint32_t b = (by * dims[2] + bz) * dims[0] + bx;
     uint32_t entry = grid_at(s, (size_t)(top_off + b));
     uint32_t ov = grid_at(s, (size_t)(ov_top_off + b));
-    LWO_COUNT(bytes, 9);  // the top-level and overlay entries and the octant byte
+    LWO_COUNT(bytes, 8);  // the top-level and overlay entries
     LWO_COUNT(dda_steps, 1);
     cell_out[0] = cell[0];
     cell_out[1] = cell[1];
@@ -865,8 +875,12 @@ static V3 shadow(const View &s, V3 o, V3 d, real tmax) {
     LWO_COUNT(bytes, 80);
     uint32_t flags = q_flags(q);
     if ((flags & FLAG_WATER) != 0) {
-      if (dot(q_normal(q), d) > 0.0f) tr = mul(tr, transmittance(from4(s.params->water_color), h.t));
+      if (dot(q_normal(q), d) > 0.0f) {
+        LWO_COUNT(shadow_water_exits, 1);
+        tr = mul(tr, transmittance(from4(s.params->water_color), h.t));
+      }
     } else if ((flags & FLAG_ALPHA_MASK) == ALPHA_TRANSLUCENT) {
+      LWO_COUNT(shadow_translucent, 1);
       real a = (real)(h.texel >> 24) / 255.0f;
       V3 col = mul(linear_rgb(h.texel), linear_rgb(q_tint(q)));
       tr = mul(tr, lerp(ONE3, col, a));
@@ -1060,9 +1074,12 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
           if (rng.next() < (real)(st.texel >> 24) / 255.0f) {
             hit = st;
             forced = true;
+            LWO_COUNT(see_through_drawn, 1);
+            LWO_COUNT(see_through_forced, 1);
           }
         } else {
           hit = st;
+          LWO_COUNT(see_through_drawn, 1);
         }
       }
     }
@@ -1099,6 +1116,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
     V3 n = shading_normal(q, d, front);
 
     if ((flags & FLAG_WATER) != 0) {
+      LWO_COUNT(water_interfaces, 1);
       water_interface(p, pos, d, n, front, in_water, eps, rng, d, o, in_water);
       specular = true;
       tmin = 0.0f;
@@ -1106,6 +1124,7 @@ static V3 render_sample(const View &s, uint32_t px, uint32_t py, uint32_t sample
     }
 
     if (!forced && (flags & FLAG_ALPHA_MASK) == ALPHA_TRANSLUCENT && rng.next() >= alpha) {
+      LWO_COUNT(translucent_passes, 1);
       beta = mul(beta, lerp(ONE3, albedo, alpha));
       o = pos + d * eps;
       tmin = 0.0f;
@@ -1185,8 +1204,10 @@ static std::vector<uint8_t> octant_boxes(const View &s) {
 // CpuBackend::render_hq (crates/lumenwright/src/render/cpu/mod.rs:39-87): rows in
 // parallel (one row per task, taken dynamically as rayon's work stealing does), f64
 // per-pixel accumulation of the finite samples, divided by spp.
+// With counters, device_walk chooses the walk counted: the device's, which stops once every
+// brick left toward the ray's octant is EMPTY, or (false) the reference's whole walk.
 static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, real *out,
-                      Counters *counters) {
+                      Counters *counters, bool device_walk = true) {
   Params params = *view_in.params;
   params.frame[2] = 0;
   View view = view_in;
@@ -1203,7 +1224,7 @@ static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, re
 #if LWO_COUNTING
     per_thread[tid] = Counters{};
     tl_counters = counters ? &per_thread[tid] : nullptr;
-    tl_octant_boxes = counters ? &boxes : nullptr;
+    tl_octant_boxes = counters && device_walk ? &boxes : nullptr;
 #endif
     try {
       for (;;) {
@@ -1249,10 +1270,18 @@ static void render_hq(const View &view_in, uint32_t spp_in, unsigned threads, re
       counters->traces += c.traces;
       counters->samples += c.samples;
       counters->sky_lookups += c.sky_lookups;
+      counters->see_through_drawn += c.see_through_drawn;
+      counters->see_through_forced += c.see_through_forced;
+      counters->translucent_passes += c.translucent_passes;
+      counters->alpha_zero_rejects += c.alpha_zero_rejects;
+      counters->shadow_water_exits += c.shadow_water_exits;
+      counters->shadow_translucent += c.shadow_translucent;
+      counters->water_interfaces += c.water_interfaces;
     }
   }
 #else
   (void)counters;
+  (void)device_walk;
 #endif
 }
 

@@ -171,3 +171,54 @@ def test_sun_angle_gradient_meets_the_closed_form(oracle, alt, az):
     assert abs(g[-2] - want) <= 0.02 * abs(want)
     assert abs(g[-1]) <= 0.02 * abs(want)
     assert g[-3] == 0.0
+
+
+@pytest.fixture(scope="module")
+def fit_problem():
+    """The inverse fit's problem (config 4) on a small world of its kind: water, cutout plants,
+    emitters sampled with ONE and the game look's dusk sky maps."""
+    return diff.config4_problem(n_views=2, size=(40, 24), spp=4, n_groups=12,
+                                world=frames.WorldSpec(6, 6, 24, cube_types=20, cutout_types=4,
+                                                       water_level=62, emitter_fraction=0.01,
+                                                       cutout_fraction=0.05))
+
+
+def test_fit_world_gradient_matches_central_differences(oracle, fit_problem):
+    # The gradient the fit follows, through water (integrator.rs:627-643), ONE emitter sampling
+    # (:688-693) and the sky maps (:196-243), against central differences of the oracle's own
+    # forward with Russian roulette off (palette and intensity). With roulette off the forward
+    # is a polynomial in the factors, so a central difference errs only by its cubic terms
+    # (h^2/6 times the third derivative, about 1e-4 relative at h = 1e-2). At h = 1e-3 on a
+    # frame this small the f32 rounding of the samples moves sum(G * I) by about 1e-6 between
+    # the two sides, 2e-3 of noise in a difference; the device test of the same gradient takes
+    # h = 1e-3 on a larger frame.
+    bp = fit_problem
+    p0, spp = bp.views[0]
+    fr = frames.HqFrame(bp.base.scene, p0, spp, bp.base.sky)
+    v = fr.view()
+    assert p0["misc_u"][3] > 0 and p0["sky_map"][0] == 2048.0 and bp.scene.stats["water_cells"] > 0
+    start = smooth(bp.start)
+    G = np.random.default_rng(3).standard_normal(int(p0["frame"][0]) * int(p0["frame"][1]) * 3).astype("<f4")
+    adj = oracle.adjoint(v, spp, start.desc(), start.n_grads, G)
+    h = 1e-2
+    fd = []
+    for i in range(3 * start.n_palette):
+        pp, pm = start.palette.ravel().copy(), start.palette.ravel().copy()
+        pp[i] += h
+        pm[i] -= h
+        fd.append((oracle.render_diff_dot(v, spp, smooth(start.with_values(palette=pp)).desc(), G)
+                   - oracle.render_diff_dot(v, spp, smooth(start.with_values(palette=pm)).desc(), G)) / (2 * h))
+    for j in range(start.n_intensity):
+        ip, im = start.intensity.copy(), start.intensity.copy()
+        ip[j] += h
+        im[j] -= h
+        fd.append((oracle.render_diff_dot(v, spp, smooth(start.with_values(intensity=ip)).desc(), G)
+                   - oracle.render_diff_dot(v, spp, smooth(start.with_values(intensity=im)).desc(), G)) / (2 * h))
+    fd = np.array(fd)
+    a = adj[:len(fd)]
+    rel = np.abs(a - fd) / np.maximum(np.abs(fd), 1e-2 * np.abs(fd).max())
+    k = int(np.argmax(rel))
+    print("fit world: worst rel", rel.max(), "at", k, a[k], fd[k], "max |fd|", np.abs(fd).max(),
+          "nonzero intensity fd", int(np.count_nonzero(fd[3 * start.n_palette:])))
+    assert np.count_nonzero(fd[3 * start.n_palette:]) > 0  # emitters are seen
+    assert rel.max() <= 1e-3

@@ -2,10 +2,10 @@
 
 Bars (the kernel crate's own outputs, restated by the oracle):
   * trace: bit-exact -- hit or miss, quad, texel, t bits and the final DDA cell;
-  * render_sample: bit-exact (the device's e^x follows libm's expf method; the two f32
-    inputs where they part are one ulp apart), and in any case within 1e-4 relative;
-  * render_hq: the image equals the CPU backend's bit for bit where the samples do; image
-    MAE < 1e-3 in linear radiance in any case.
+  * render_sample: bit-exact in every kernel form (the device's e^x follows libm's expf
+    method; the two f32 inputs of 2^32 where they part, one ulp apart, are not met here);
+  * render_hq: the image equals the CPU backend's bit for bit. The relative error and the
+    MAE are printed as diagnostics only.
 """
 
 import os
@@ -196,9 +196,7 @@ def test_render_sample_parity_on_config_1(hip_backend, oracle, frame1):
     got = hip_backend.render_samples(v, px, py, s)
     same, rel = radiance_report(got, want)
     print(f"cfg1: {int(same.sum())}/{len(same)} samples bit-exact, worst rel {rel.max():.2e}")
-    assert np.all(np.isfinite(got) == np.isfinite(want))
-    assert rel.max() <= 1e-4
-    assert same.mean() > 0.99
+    assert same.all()
 
 
 def test_render_hq_matches_the_cpu_backend(hip_backend, oracle, frame0, frame1):
@@ -209,9 +207,7 @@ def test_render_hq_matches_the_cpu_backend(hip_backend, oracle, frame0, frame1):
         same = np.all(got.view("<u4") == want.view("<u4"), axis=1)
         mae = float(np.abs(got.astype(np.float64) - want).mean())
         print(f"{name}: {int(same.sum())}/{len(same)} pixels bit-exact, MAE {mae:.3e}")
-        assert mae < 1e-3
-        if name == "cfg0":
-            assert same.all()
+        assert same.all()
 
 
 def test_device_image_error_against_the_f64_render(hip_backend, oracle, frame0):
@@ -307,10 +303,10 @@ def test_every_kernel_form_matches_the_oracle(mode_backends, oracle, frame0, fra
         got = b.render_samples(v, px, py, s)
         same, rel = radiance_report(got, want)
         print(f"{mode}: {int(same.sum())}/{len(same)} bit-exact, worst rel {rel.max():.2e}")
-        assert rel.max() <= 1e-4 and same.mean() > 0.999
+        assert same.all()
     img = b.render_hq(frames.HqFrame(frame1.scene, frame1.params, 3, frame1.sky))
     want = oracle.render_hq(frame1.view(), 3).reshape(-1, 3)
-    assert float(np.abs(img.astype(np.float64) - want).mean()) < 1e-3
+    assert np.array_equal(img.view("<u4"), want.view("<u4"))
 
 
 def lamp_camera(s):

@@ -69,3 +69,21 @@ def test_oracle_cpu_backend_is_thread_count_invariant(oracle):
     imgs = [oracle.render_hq(v, 4, threads=t) for t in (1, 3, 8)]
     assert all(np.array_equal(imgs[0].view("<u4"), im.view("<u4")) for im in imgs[1:])
     assert np.isfinite(imgs[0]).all() and imgs[0].mean() > 0
+
+
+def test_per_view_arrays_are_checked_before_the_call():
+    # lw_hip_adjoint_batch and lw_hip_fit_new copy w*h*3 floats from each view's pointer, so
+    # the binding refuses an array of another size before it reaches the library.
+    import numpy as np
+    import pytest
+    from lumenwright_amd.abi import new_params
+    from lumenwright_amd.backend import HipBackend
+    p = new_params()
+    p["frame"][:2] = [4, 3]
+    views = [(p, 1), (p, 2)]
+    ok = HipBackend._per_view_images(views, [np.zeros(36), np.zeros((12, 3))], "dL/dI")
+    assert [a.dtype.str for a in ok] == ["<f4", "<f4"] and all(a.size == 36 for a in ok)
+    with pytest.raises(ValueError, match="view 1"):
+        HipBackend._per_view_images(views, [np.zeros(36), np.zeros(35)], "dL/dI")
+    with pytest.raises(ValueError, match="2 views"):
+        HipBackend._per_view_images(views, [np.zeros(36)], "target")
