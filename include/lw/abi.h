@@ -271,6 +271,20 @@ int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene,
 
 void lw_hip_scene_free(lw_hip_backend *b, lw_hip_scene *scene);
 
+/* lw_hip_render_hq_scene that allocates nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory mid-run (tight serving loops, code under
+ * stream capture, harnesses that forbid hipMalloc). The frame's f64 accumulator and staged
+ * samples live in a scratch array of LW_NOALLOC_SCRATCH_BYTES that each device holds from the
+ * library's load to the process's end (36 bytes a pixel, rounded: frames up to about 3.7 M
+ * pixels; larger ones fail with LW_EINVAL and a message). It renders in the direct form
+ * whatever LW_KERNEL says, and gives lw_hip_render_hq_scene's image bit for bit. Every call
+ * on one device shares that device's array: calls that overlap on one device race (the
+ * accumulator is cleared at the start of each call). */
+#define LW_NOALLOC_SCRATCH_BYTES (128u << 20)
+int lw_hip_render_hq_scene_noalloc(lw_hip_backend *b, const lw_hip_scene *scene,
+                                   const lw_params *params, uint32_t spp,
+                                   lw_progress_fn progress, void *user, float *out_rgb);
+
 /* ---- the scene builder on the device (crates/lumenwright/src/scene/voxel.rs) ---- */
 
 #define LW_MODEL_OCCLUDES 1u /* the model's block occludes (BlockInfo::occludes, voxel.rs:119-130) */

@@ -33,6 +33,7 @@ SIGNATURES = {
     "lw_hip_render_samples": (i32, [vp, vp, sz, vp, vp, vp, vp]),
     "lw_hip_scene_upload": (i32, [vp, vp, ctypes.POINTER(vp)]),
     "lw_hip_render_hq_scene": (i32, [vp, vp, vp, u32, PROGRESS_FN, vp, vp]),
+    "lw_hip_render_hq_scene_noalloc": (i32, [vp, vp, vp, u32, PROGRESS_FN, vp, vp]),
     "lw_hip_count_traces": (i32, [vp, vp, vp, u32, vp]),
     "lw_hip_scene_free": (None, [vp, vp]),
     "lw_hip_scene_build": (i32, [vp, vp, vp, vp, ctypes.POINTER(vp)]),

@@ -159,6 +159,18 @@ class HipBackend:
                                                          cb, None, _addr(out)))
         return out
 
+    def render_hq_scene_noalloc(self, scene: HipScene, params: np.ndarray, spp: int,
+                                progress: Optional[Callable[[int, int], None]] = None) -> np.ndarray:
+        """lw_hip_render_hq_scene_noalloc: render_hq over a resident scene in the library's
+        fixed scratch memory, allocating nothing."""
+        p = np.ascontiguousarray(params, dtype=PARAMS).reshape(())
+        w, h = int(p["frame"][0]), int(p["frame"][1])
+        out = np.zeros((w * h, 3), dtype="<f4")
+        cb = _native.PROGRESS_FN((lambda d, t, u: progress(d, t)) if progress else 0)
+        self._check(_native.lib().lw_hip_render_hq_scene_noalloc(
+            self.handle, scene.handle, _addr(p), spp, cb, None, _addr(out)))
+        return out
+
     def count_traces(self, scene: HipScene, params: np.ndarray, spp: int) -> int:
         """lw_hip_count_traces: the trace calls a render_hq_scene of (params, spp) makes,
         counted on the device by an untimed replay of its paths."""

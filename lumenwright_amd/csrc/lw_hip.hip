@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,20 @@
 #include "lw_quick.hpp"
 
 using namespace lw;
+
+// The scratch memory of lw_hip_render_hq_scene_noalloc: a frame's f64 accumulator followed by
+// its staged samples (and, once they are summed, its f32 means). Each device that loads the
+// library holds its own copy until the process ends. In host code `scratch` names a host-side
+// stand-in, never the device memory: kernels and copies take the address that
+// hipGetSymbolAddress gives for the current device (g_scratch_ptr, set when a backend opens).
+constexpr size_t kScratchBytes = LW_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
 
 // ------------------------------------------------------------------------------------
 // kernels
@@ -1205,6 +1220,7 @@ struct lw_hip_backend {
   DevBuf<uint4> wf_hit;
   DevBuf<uint32_t> wf_queue, wf_ctr;
   uint32_t *pinned = nullptr;
+  void *scratch = nullptr;  // this device's scratch array (hipGetSymbolAddress), or null
   uint64_t last_iterations = 0;
   // Batches: the staging cap, a minimum pass count (0: as few as fit) and whether a pass
   // replays as a captured hipGraph (LW_BATCH_GRAPHS=0 launches it kernel by kernel).
@@ -1671,14 +1687,35 @@ constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
 constexpr uint64_t kFillBlocks = 2048;     // 8 workgroups of 256 lanes per CU
 constexpr uint32_t kMaxBatch = 64;         // render/cuda.rs:165
 
+// Device memory a frame renders into when the caller provides it (FrameScratch) instead of
+// the backend's grow-only buffers: the f64 accumulator and `rad_floats` floats of staging,
+// which also take the f32 means at the end. Such a frame runs the direct form.
+struct FrameScratch {
+  double *acc;
+  float *rad;
+  uint64_t rad_floats;
+};
+
+// The scratch array's split for a frame of npix pixels: the accumulator at its start, the
+// staging after it, 256-byte aligned. False when the frame does not fit one sample per launch.
+bool split_scratch(void *base, uint64_t npix, FrameScratch &fs) {
+  const uint64_t acc_bytes = (npix * 3 * sizeof(double) + 255) / 256 * 256;
+  if (acc_bytes + npix * 3 * sizeof(float) > kScratchBytes) return false;
+  fs.acc = static_cast<double *>(base);
+  fs.rad = reinterpret_cast<float *>(static_cast<unsigned char *>(base) + acc_bytes);
+  fs.rad_floats = (kScratchBytes - acc_bytes) / sizeof(float);
+  return true;
+}
+
 // With `diff`, the samples are render_sample_diff's under its parameters (direct form).
 // With `out_f64`, it also receives the per-pixel means in f64 (sum / spp before the f32
 // rounding of out_rgb).
 // With `one_launch`, all spp samples go in one launch (up to kMaxBatch), for timing probes.
+// With `fixed`, the frame allocates nothing: it renders into that memory, in the direct form.
 void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &params_in,
                   uint32_t spp_in, lw_progress_fn progress, void *user, float *out_rgb,
                   const DiffDev *diff = nullptr, double *out_f64 = nullptr,
-                  bool one_launch = false) {
+                  bool one_launch = false, const FrameScratch *fixed = nullptr) {
   auto t_wall = std::chrono::steady_clock::now();
   check_layout(sc, params_in);
   lw_params p = params_in;
@@ -1691,15 +1728,31 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   HIP_TRY(hipSetDevice(b->device));
   const DevScene ds = sc.dev();
 
-  // Staging for up to kMaxBatch samples per launch, capped at 2 GiB.
+  // Staging for up to kMaxBatch samples per launch, capped at 2 GiB (or at what the caller's
+  // memory holds).
   uint32_t max_batch = kMaxBatch;
   while (max_batch > 1 && npix * 3 * sizeof(float) * max_batch > (2ull << 30)) max_batch /= 2;
-  b->rad.ensure(npix * 3 * max_batch);
-  b->acc.ensure(npix * 3);
-  b->out.ensure(npix * 3);
+  float *rad, *out;
+  double *acc;
+  int mode = b->mode;
+  if (fixed) {
+    while (max_batch > 1 && npix * 3 * max_batch > fixed->rad_floats) max_batch /= 2;
+    if (npix * 3 > fixed->rad_floats) throw InvalidScene{"frame too large for the scratch memory"};
+    rad = fixed->rad;
+    acc = fixed->acc;
+    out = fixed->rad;  // the staging is free once the last launch is summed
+    mode = MODE_DIRECT;
+  } else {
+    b->rad.ensure(npix * 3 * max_batch);
+    b->acc.ensure(npix * 3);
+    b->out.ensure(npix * 3);
+    rad = b->rad.ptr;
+    acc = b->acc.ptr;
+    out = b->out.ptr;
+  }
   {
     const size_t n = npix * 3;
-    k_zero<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(b->acc.ptr, n);
+    k_zero<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(acc, n);
     HIP_TRY(hipGetLastError());
   }
 
@@ -1719,20 +1772,20 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (diff) {
       LW_DIFF_DISPATCH(b->waves, (k_render_diff<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
-                                     p, ds, *diff, done, b->rad.ptr)));
-    } else if (b->mode == MODE_DIRECT && b->per_lane > 1) {
+                                     p, ds, *diff, done, rad)));
+    } else if (mode == MODE_DIRECT && b->per_lane > 1) {
       const uint32_t per = std::min(b->per_lane, nb);
       LW_DIRECT_DISPATCH(b->waves, (k_render_multi<W><<<dim3(tiles, (nb + per - 1) / per),
                                                          dim3(kBlock), 0, b->stream>>>(
-                                       p, ds, done, nb, per, b->rad.ptr)));
-    } else if (b->mode == MODE_DIRECT) {
+                                       p, ds, done, nb, per, rad)));
+    } else if (mode == MODE_DIRECT) {
       LW_DIRECT_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
-                                       p, ds, done, b->xcd_remap, b->rad.ptr)));
-    } else if (b->mode == MODE_WAVEFRONT) {
-      const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
+                                       p, ds, done, b->xcd_remap, rad)));
+    } else if (mode == MODE_WAVEFRONT) {
+      const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, rad};
       run_wavefront(b, sc, p, fw, fw.total);
     } else {
-      const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, b->rad.ptr};
+      const FrameWork fw{w, h, tiles_x, tiles, done, tiles * 256u * nb, rad};
       const uint32_t grid = std::min<uint32_t>(b->blocks_frame, (fw.total + kBlock - 1) / kBlock);
       k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
       LW_WAVES_DISPATCH(b->waves, (k_paths<FrameWork, W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
@@ -1741,7 +1794,7 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     k_accumulate<<<dim3((unsigned)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
-        (uint32_t)npix, nb, b->rad.ptr, b->acc.ptr);
+        (uint32_t)npix, nb, rad, acc);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[2], b->stream));
     HIP_TRY(hipEventSynchronize(b->ev[2]));
@@ -1763,14 +1816,14 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   {
     const size_t n = npix * 3;
     k_resolve<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
-        (uint32_t)npix, spp, b->acc.ptr, b->out.ptr);
+        (uint32_t)npix, spp, acc, out);
     HIP_TRY(hipGetLastError());
   }
   if (out_rgb)  // without it the image stays in b->out for the caller's next kernel
-    HIP_TRY(hipMemcpyAsync(out_rgb, b->out.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(out_rgb, out, npix * 3 * sizeof(float), hipMemcpyDeviceToHost,
                            b->stream));
   if (out_f64)
-    HIP_TRY(hipMemcpyAsync(out_f64, b->acc.ptr, npix * 3 * sizeof(double), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(out_f64, acc, npix * 3 * sizeof(double), hipMemcpyDeviceToHost,
                            b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   if (out_f64)
@@ -2306,6 +2359,15 @@ int lw_hip_open(int device, lw_hip_backend **out) {
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto &ev : b->ev) HIP_TRY(hipEventCreate(&ev));
     b->counter.alloc(1);
+    if (device < kMaxScratchDevices) {  // once per device, with the device current
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = ptr;
+      }
+      b->scratch = g_scratch_ptr[device];
+    }
     if (const char *e = std::getenv("LW_WAVES")) b->waves = std::atoi(e);
     if (const char *e = std::getenv("LW_TRACE_WAVES")) b->trace_waves = std::atoi(e);
     int per_cu_frame = 0, per_cu_list = 0;
@@ -2518,6 +2580,22 @@ int lw_hip_render_hq_scene(lw_hip_backend *b, const lw_hip_scene *scene, const l
                            uint32_t spp, lw_progress_fn progress, void *user, float *out_rgb) {
   if (!b || !scene || !params || !out_rgb) return LW_EINVAL;
   return guarded(b, [&] { render_frame(b, *scene, *params, spp, progress, user, out_rgb); });
+}
+
+int lw_hip_render_hq_scene_noalloc(lw_hip_backend *b, const lw_hip_scene *scene,
+                                   const lw_params *params, uint32_t spp,
+                                   lw_progress_fn progress, void *user, float *out_rgb) {
+  if (!b || !scene || !params || !out_rgb) return LW_EINVAL;
+  return guarded(b, [&] {
+    if (!b->scratch) throw Unsupported{"no scratch memory on this device"};
+    const uint64_t npix = (uint64_t)params->frame[0] * params->frame[1];
+    FrameScratch fs;
+    if (!split_scratch(b->scratch, npix, fs))
+      throw InvalidScene{"a frame of " + std::to_string(npix) + " pixels needs " +
+                         std::to_string((npix * 24 + 255) / 256 * 256 + npix * 12) +
+                         " bytes of scratch memory; it has " + std::to_string(kScratchBytes)};
+    render_frame(b, *scene, *params, spp, progress, user, out_rgb, nullptr, nullptr, false, &fs);
+  });
 }
 
 int lw_hip_count_traces(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params, uint32_t spp,
