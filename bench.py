@@ -37,7 +37,15 @@ Besides the throughput the config-1 line carries:
   scene_build   the world's layout made on the device from its voxel volume
                 (lw_hip_scene_build), and the oracle's CPU restatement of it;
   sky_prep      the game's sky maps, Chunky's sky cache and their blend on the device, and
-                the oracle's CPU game_sky_maps.
+                the oracle's CPU game_sky_maps;
+  parity        the device's image of the frame at the CPU baseline's last spp against the
+                image that pass made (pixels bit for bit, MAE), and the MAE of both against
+                the f64 render on a subset of rows;
+and the metric's other two parts (leave them out with --no-metric-parts):
+  batched       config 2 in full: 64 views as one batch and as 64 plain calls, the ratio
+                batched_over_plain and whether the images are identical;
+  adjoint       config 3's adjoint / forward kernel-time ratio, and one config-4 fit step's
+                forward and adjoint times.
 """
 
 import argparse
@@ -70,7 +78,8 @@ def host_cores() -> int:
 
 def cpu_baseline(frame, seconds: float):
     """The oracle's CpuBackend::render_hq over the frame's first spp samples, doubling spp
-    until the total time reaches `seconds`."""
+    until the total time reaches `seconds`. Returns the figure, the last pass's image and its
+    spp (the parity check renders the same on the device)."""
     from oracle import ffi
     ffi.build()
     from lumenwright_amd.abi import HqFrame
@@ -80,7 +89,7 @@ def cpu_baseline(frame, seconds: float):
     spp, total_s, total_n = 1, 0.0, 0
     while True:
         t = time.perf_counter()
-        ffi.render_hq(v, spp, threads=cores)
+        img = ffi.render_hq(v, spp, threads=cores)
         dt = time.perf_counter() - t
         total_s += dt
         total_n += w * h * spp
@@ -90,7 +99,30 @@ def cpu_baseline(frame, seconds: float):
     return {"value": total_n / total_s / 1e6, "unit": "Msamples/s", "cores": cores,
             "kind": "port",
             "sample": f"{w}x{h}, passes of the first 1, 2, 4, ... {spp} samples of every "
-                      f"pixel: {total_n} samples in {total_s:.1f} s"}
+                      f"pixel: {total_n} samples in {total_s:.1f} s"}, img, spp
+
+
+def parity_block(b, scene, frame, cpu_img, spp: int, n_rows: int = 16) -> dict:
+    """The headline frame against the CPU backend: the device's image at the CPU baseline's
+    last spp, compared pixel by pixel with the oracle image that pass made (untimed), and the
+    error of both against the f64 render (oracle/lw_oracle_f64.cpp) on n_rows evenly spaced
+    rows of the frame."""
+    from oracle import ffi
+    from lumenwright_amd.abi import HqFrame
+    w, h = int(frame.params["frame"][0]), int(frame.params["frame"][1])
+    dev = b.render_hq_scene(scene, frame.params, spp).reshape(h, w, 3)
+    same = np.all(dev.view("<u4") == cpu_img.view("<u4"), axis=2)
+    rows = np.linspace(0, h - 1, n_rows).round().astype(int)
+    t = time.perf_counter()
+    f64 = ffi.render_rows_f64(HqFrame(frame.scene, frame.params, spp, frame.sky).view(), spp, rows,
+                              threads=host_cores())
+    f64_s = time.perf_counter() - t
+    err = lambda img: float(np.abs(img[rows].astype(np.float64) - f64).mean())
+    return {"spp": spp, "pixels": w * h, "bit_exact_pixels": int(same.sum()),
+            "mae_vs_cpu_backend": float(np.abs(dev.astype(np.float64) - cpu_img).mean()),
+            "mae_vs_f64": {"device": err(dev), "cpu_backend": err(cpu_img),
+                           "f64_mean": float(f64.mean()), "rows": [int(r) for r in rows],
+                           "f64_seconds": f64_s}}
 
 
 def roofline(config, samples, trace_ms, launches):
@@ -305,9 +337,20 @@ def bench_plain(args, lw, frames):
         extra["copy_probe"] = {"bytes": args.copy_probe, "ms": ms,
                                "GBps": 2 * args.copy_probe / ms / 1e6}
         log(f"copy probe: {extra['copy_probe']}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, cpu_img, cpu_spp = cpu_baseline(frame, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+        extra["parity"] = parity_block(b, scene, frame, cpu_img, cpu_spp)
+        log(f"parity: {extra['parity']}")
     scene.free()
     b.close()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(frame, args.cpu_seconds)
+    if args.config == 1 and not args.no_metric_parts:
+        extra["batched"] = batch_block(lw, frames, steps=1)
+        log(f"batched: {extra['batched']}")
+        extra["adjoint"] = {"cfg3": diff_block(lw, frames, steps=3, warmup=1),
+                            "cfg4_fit_step": fit_block(lw, frames, steps=1, warmup=1)}
+        log(f"adjoint: {extra['adjoint']}")
     line = {
         "metric": METRIC,
         "value": value,
@@ -337,66 +380,86 @@ def bench_plain(args, lw, frames):
     }
     if cpu:
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
+    if "batched" in extra:
+        line["batched_over_plain"] = extra["batched"]["batched_over_plain"]
+        line["adjoint_over_forward"] = extra["adjoint"]["cfg3"]["adjoint_over_forward"]
     return line
 
 
-def bench_batch(args, lw, frames):
+def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0) -> dict:
+    """cfg2: the 64 seeded views of the config-1 world rendered as one batch
+    (lw_hip_render_hq_batch) and as 64 plain render_hq calls, after one untimed batch; the
+    wall time of each, and whether the images are identical (same samples, same summation
+    order, so the two are compared at equal image error)."""
     cfg = frames.CONFIGS[1]
     t0 = time.perf_counter()
     scene_buf = frames.build_world(cfg.world)
     base, views = frames.batch_views(cfg, scene_buf, n_views=64, seed=3)
-    if args.spp_scale != 1.0:
-        views = [(p, max(1, int(s * args.spp_scale))) for p, s in views]
+    if spp_scale != 1.0:
+        views = [(p, max(1, int(s * spp_scale))) for p, s in views]
     samples = sum(int(p["frame"][0]) * int(p["frame"][1]) * s for p, s in views)
     log(f"64 views built in {time.perf_counter() - t0:.1f} s: {samples / 1e9:.2f} G samples")
     b = lw.HipBackend.open(0)
     scene = b.upload(base)
-    batch_imgs = b.render_hq_batch(scene, views)  # warmup
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        batch_imgs = b.render_hq_batch(scene, views)
-    t_batch = (time.perf_counter() - t) / args.steps
-    tb = b.last_timing()
-    log(f"batched: {t_batch:.2f} s per step, {tb}")
-    t = time.perf_counter()
-    plain_ms = 0.0
-    for _ in range(args.steps):
-        plain_imgs = [b.render_hq_scene(scene, p, s) for p, s in views]
-    t_plain = (time.perf_counter() - t) / args.steps
-    log(f"plain: {t_plain:.2f} s per step")
+    try:
+        batch_imgs = b.render_hq_batch(scene, views)  # warmup
+        t = time.perf_counter()
+        for _ in range(steps):
+            batch_imgs = b.render_hq_batch(scene, views)
+        t_batch = (time.perf_counter() - t) / steps
+        tb = b.last_timing()
+        log(f"batched: {t_batch:.2f} s per step, {tb}")
+        t = time.perf_counter()
+        for _ in range(steps):
+            plain_imgs = [b.render_hq_scene(scene, p, s) for p, s in views]
+        t_plain = (time.perf_counter() - t) / steps
+        log(f"plain: {t_plain:.2f} s per step")
+    finally:
+        scene.free()
+        b.close()
     same = all(np.array_equal(x.view("<u4"), y.view("<u4")) for x, y in zip(batch_imgs, plain_imgs))
-    scene.free()
-    b.close()
+    return {"workload": "cfg2: 64 seeded views of the cfg1 world, sizes 160x90..1280x720, "
+                        "spp 16..1024, 8 bounces, one batch vs 64 plain calls",
+            "views": 64, "samples": samples, "steps": steps,
+            "batched_msamples_per_s": samples / t_batch / 1e6,
+            "plain_msamples_per_s": samples / t_plain / 1e6,
+            "batched_s": t_batch, "plain_s": t_plain,
+            "batched_over_plain": t_plain / t_batch, "images_identical": same,
+            "batch_passes": tb["launches"], "batch_trace_ms": tb["trace_ms"]}
+
+
+def bench_batch(args, lw, frames):
+    blk = batch_block(lw, frames, steps=args.steps, spp_scale=args.spp_scale)
+    samples = blk["samples"]
     return {
         "metric": METRIC,
-        "value": samples / t_batch / 1e6,
+        "value": blk["batched_msamples_per_s"],
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": 1,
-        "ms_per_step": t_batch * 1e3,
+        "ms_per_step": blk["batched_s"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 path arithmetic, f64 accumulation",
         "data": "synthetic (seeded value-noise voxel world, procedural sky)",
-        "config": {"workload": "cfg2: 64 seeded views of the cfg1 world, sizes 160x90..1280x720, "
-                               "spp 16..1024, 8 bounces, one batch vs 64 plain calls",
-                   "views": 64, "samples": samples, "kernel": kernel_form(),
-                   "parallelism": "1 GPU"},
-        "plain_msamples_per_s": samples / t_plain / 1e6,
-        "batched_over_plain": t_plain / t_batch,
-        "images_identical": same,
-        "batch_passes": tb["launches"],
-        "roofline": roofline(1, samples, tb["trace_ms"], max(1, tb["launches"])),
+        "config": {"workload": blk["workload"], "views": 64, "samples": samples,
+                   "kernel": kernel_form(), "parallelism": "1 GPU"},
+        "plain_msamples_per_s": blk["plain_msamples_per_s"],
+        "batched_over_plain": blk["batched_over_plain"],
+        "images_identical": blk["images_identical"],
+        "batch_passes": blk["batch_passes"],
+        "roofline": roofline(1, samples, blk["batch_trace_ms"], max(1, blk["batch_passes"])),
         "cpu_baseline": None,
     }
 
 
-def bench_diff(args, lw, frames):
-    """cfg3: one forward (lw_hip_render_diff) and one adjoint (lw_hip_adjoint) of the L2 loss
-    against a target rendered at perturbed parameters, per step; the adjoint covers all 115
-    parameters (96 albedo factors, 16 intensities, turbidity and the sun's two angles)."""
+def diff_block(lw, frames, steps: int, warmup: int) -> dict:
+    """cfg3: per step one forward (lw_hip_render_diff) and one adjoint (lw_hip_adjoint) of the
+    L2 loss against a target rendered at perturbed parameters; the adjoint covers all 115
+    parameters (96 albedo factors, 16 intensities, turbidity and the sun's two angles). Kernel
+    time (HIP events) and wall time of each pass."""
     from lumenwright_amd import diff
     t0 = time.perf_counter()
     prob = diff.config3_problem()
@@ -408,129 +471,155 @@ def bench_diff(args, lw, frames):
     b = lw.HipBackend.open(0)
     sc = b.upload(fr)
     tsc = b.upload(tfr)
-    target = b.render_diff(tsc, tfr.params, cfg.spp, prob.target.desc())
-    desc_f = prob.start.desc()
-    start_a = prob.start.with_values()
-    desc_a = start_a.desc(derivs)
+    try:
+        target = b.render_diff(tsc, tfr.params, cfg.spp, prob.target.desc())
+        desc_f = prob.start.desc()
+        start_a = prob.start.with_values()
+        desc_a = start_a.desc(derivs)
+        for _ in range(warmup):
+            img = b.render_diff(sc, fr.params, cfg.spp, desc_f)
+            _, dldi = diff.l2_loss(img, target)
+            b.adjoint(sc, fr.params, cfg.spp, desc_a, start_a.n_grads, dldi)
+        fwd_ms = adj_ms = fwd_wall = adj_wall = 0.0
+        for i in range(steps):
+            t = time.perf_counter()
+            img = b.render_diff(sc, fr.params, cfg.spp, desc_f)
+            fwd_wall += time.perf_counter() - t
+            fwd_ms += b.last_timing()["trace_ms"]
+            loss, dldi = diff.l2_loss(img, target)
+            t = time.perf_counter()
+            g = b.adjoint(sc, fr.params, cfg.spp, desc_a, start_a.n_grads, dldi)
+            adj_wall += time.perf_counter() - t
+            adj_ms += b.last_timing()["trace_ms"]
+            log(f"step {i + 1}/{steps}: loss {loss:.6g}, |grad| {np.linalg.norm(g):.4g}")
+    finally:
+        sc.free()
+        tsc.free()
+        b.close()
     samples = cfg.width * cfg.height * cfg.spp
-    for _ in range(args.warmup):
-        img = b.render_diff(sc, fr.params, cfg.spp, desc_f)
-        _, dldi = diff.l2_loss(img, target)
-        b.adjoint(sc, fr.params, cfg.spp, desc_a, start_a.n_grads, dldi)
-    fwd_ms = adj_ms = fwd_wall = adj_wall = 0.0
-    for i in range(args.steps):
-        t = time.perf_counter()
-        img = b.render_diff(sc, fr.params, cfg.spp, desc_f)
-        fwd_wall += time.perf_counter() - t
-        fwd_ms += b.last_timing()["trace_ms"]
-        loss, dldi = diff.l2_loss(img, target)
-        t = time.perf_counter()
-        g = b.adjoint(sc, fr.params, cfg.spp, desc_a, start_a.n_grads, dldi)
-        adj_wall += time.perf_counter() - t
-        adj_ms += b.last_timing()["trace_ms"]
-        log(f"step {i + 1}/{args.steps}: loss {loss:.6g}, |grad| {np.linalg.norm(g):.4g}")
-    sc.free()
-    tsc.free()
-    b.close()
-    k = args.steps
+    return {"workload": cfg.name + "; forward + adjoint of an L2 image loss over 115 parameters "
+                                   "(96 albedo factors, 16 emitter intensities, turbidity, sun "
+                                   "altitude and azimuth)",
+            "width": cfg.width, "height": cfg.height, "spp": cfg.spp,
+            "max_bounces": cfg.max_bounces, "steps": steps, "samples": samples,
+            "forward_kernel_ms": fwd_ms / steps, "adjoint_kernel_ms": adj_ms / steps,
+            "forward_wall_ms": fwd_wall / steps * 1e3, "adjoint_wall_ms": adj_wall / steps * 1e3,
+            "forward_msamples_per_s": samples * steps / (fwd_ms / 1e3) / 1e6,
+            "adjoint_msamples_per_s": samples * steps / (adj_ms / 1e3) / 1e6,
+            "adjoint_over_forward": adj_ms / fwd_ms,
+            "adjoint_over_forward_wall": adj_wall / fwd_wall}
+
+
+def bench_diff(args, lw, frames):
+    blk = diff_block(lw, frames, args.steps, args.warmup)
     return {
         "metric": METRIC,
-        "value": samples * k / (adj_ms / 1e3) / 1e6,
+        "value": blk["adjoint_msamples_per_s"],
         "unit": "Msamples/s",
         "n_gpus": 1,
-        "steps": k,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": (fwd_wall + adj_wall) / k * 1e3,
+        "ms_per_step": blk["forward_wall_ms"] + blk["adjoint_wall_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 path arithmetic, f64 accumulation and gradient sums",
         "data": "synthetic (seeded value-noise voxel world, procedural sky)",
-        "config": {"workload": cfg.name + "; forward + adjoint of an L2 image loss over 115 "
-                               "parameters (96 albedo factors, 16 emitter intensities, "
-                               "turbidity, sun altitude and azimuth)",
-                   "width": cfg.width, "height": cfg.height, "spp": cfg.spp,
-                   "max_bounces": cfg.max_bounces, "parallelism": "1 GPU"},
+        "config": {"workload": blk["workload"], "width": blk["width"], "height": blk["height"],
+                   "spp": blk["spp"], "max_bounces": blk["max_bounces"], "parallelism": "1 GPU"},
         "value_is": "adjoint pass Msamples/s (kernel time)",
-        "forward_msamples_per_s": samples * k / (fwd_ms / 1e3) / 1e6,
-        "adjoint_over_forward": adj_ms / fwd_ms,
-        "adjoint_over_forward_wall": adj_wall / fwd_wall,
-        "forward_kernel_ms": fwd_ms / k,
-        "adjoint_kernel_ms": adj_ms / k,
+        "forward_msamples_per_s": blk["forward_msamples_per_s"],
+        "adjoint_over_forward": blk["adjoint_over_forward"],
+        "adjoint_over_forward_wall": blk["adjoint_over_forward_wall"],
+        "forward_kernel_ms": blk["forward_kernel_ms"],
+        "adjoint_kernel_ms": blk["adjoint_kernel_ms"],
         "roofline": None,
         "cpu_baseline": None,
     }
 
 
-def bench_fit(args, lw, frames):
-    """cfg4: the batched inverse fit. 32 seeded views of the cfg1 world (640x360, 128 spp,
-    6 bounces); targets rendered with seeded ground-truth factors (192 albedo factors of the
-    64 block types, 128 emitter-intensity groups); Adam on the device from factors of 1. A
-    step is one batched forward, the loss and dL/dI, one batched adjoint (path replay) and the
-    update, all on the device."""
+def fit_block(lw, frames, steps: int, warmup: int, fit_steps: int = 0, spp: int = 128) -> dict:
+    """cfg4: the batched inverse fit. 32 seeded views of the cfg1 world (640x360, `spp`, 6
+    bounces); targets rendered with seeded ground-truth factors (192 albedo factors of the 64
+    block types, 128 emitter-intensity groups); Adam on the device from factors of 1. A step is
+    one batched forward, the loss and dL/dI, one batched adjoint (path replay) and the update,
+    all on the device. `warmup` untimed steps, then `steps` timed ones (of max(steps,
+    fit_steps) in all)."""
     from lumenwright_amd import diff
     t0 = time.perf_counter()
-    spp = args.spp or 128
     bp = diff.config4_problem(spp=spp)
     log(f"problem built in {time.perf_counter() - t0:.1f} s: {len(bp.views)} views, "
         f"{bp.start.n_palette} palettes, {bp.start.n_intensity} intensity groups")
     b = lw.HipBackend.open(0)
     sc = b.upload(bp.base)
-    t = time.perf_counter()
-    targets = b.render_diff_batch(sc, bp.views, bp.truth.desc())
-    log(f"targets rendered in {time.perf_counter() - t:.1f} s")
-    fit = b.fit(sc, bp.views, bp.start.desc(), targets, lr=1e-2)
-    e_start = diff.parameter_error(bp.start, bp.truth)
-    for i in range(args.warmup):
-        st = fit.step()
-        log(f"warmup {i + 1}: {st}")
-    steps = []
-    t = time.perf_counter()
-    n_steps = max(args.steps, args.fit_steps)
-    for i in range(n_steps):
-        st = fit.step()
-        steps.append(st)
-        if i < args.steps or (i + 1) % 10 == 0:
-            log(f"step {i + 1}/{n_steps}: loss {st['loss']:.6g}, forward {st['forward_ms']:.0f} ms, "
-                f"adjoint {st['adjoint_ms']:.0f} ms, wall {st['wall_ms']:.0f} ms")
-    wall = time.perf_counter() - t
-    pal, inten, _ = fit.params()
-    e_end = diff.parameter_error(bp.start.with_values(pal, inten), bp.truth)
-    fit.free()
-    sc.free()
-    b.close()
-    timed = steps[:args.steps]
+    fit = None
+    try:
+        t = time.perf_counter()
+        targets = b.render_diff_batch(sc, bp.views, bp.truth.desc())
+        log(f"targets rendered in {time.perf_counter() - t:.1f} s")
+        fit = b.fit(sc, bp.views, bp.start.desc(), targets, lr=1e-2)
+        e_start = diff.parameter_error(bp.start, bp.truth)
+        for i in range(warmup):
+            st = fit.step()
+            log(f"warmup {i + 1}: {st}")
+        done = []
+        t = time.perf_counter()
+        n_steps = max(steps, fit_steps)
+        for i in range(n_steps):
+            st = fit.step()
+            done.append(st)
+            if i < steps or (i + 1) % 10 == 0:
+                log(f"step {i + 1}/{n_steps}: loss {st['loss']:.6g}, forward {st['forward_ms']:.0f} ms, "
+                    f"adjoint {st['adjoint_ms']:.0f} ms, wall {st['wall_ms']:.0f} ms")
+        wall = time.perf_counter() - t
+        pal, inten, _ = fit.params()
+        e_end = diff.parameter_error(bp.start.with_values(pal, inten), bp.truth)
+    finally:
+        if fit is not None:
+            fit.free()
+        sc.free()
+        b.close()
+    timed = done[:steps]
     fwd = sum(s["forward_ms"] for s in timed)
     adj = sum(s["adjoint_ms"] for s in timed)
     step_wall = sum(s["wall_ms"] for s in timed) / len(timed)
     samples = timed[0]["samples"]
+    return {"workload": "cfg4: batched inverse fit, 32 seeded views of the cfg1 world, "
+                        f"640x360, {spp} spp, 6 bounces; 192 albedo factors and 128 "
+                        "emitter-intensity groups; Adam lr 1e-2 on the device",
+            "views": len(bp.views), "spp": spp, "samples": samples, "steps": steps,
+            "step_wall_ms": step_wall,
+            "forward_kernel_ms": fwd / len(timed), "adjoint_kernel_ms": adj / len(timed),
+            "forward_msamples_per_s": samples * len(timed) / (fwd / 1e3) / 1e6,
+            "adjoint_msamples_per_s": samples * len(timed) / (adj / 1e3) / 1e6,
+            "adjoint_over_forward": adj / fwd,
+            "fit_steps": n_steps, "loss_first": done[0]["loss"], "loss_last": done[-1]["loss"],
+            "param_error_start": {"albedo": e_start[0], "intensity": e_start[1]},
+            "param_error_end": {"albedo": e_end[0], "intensity": e_end[1]},
+            "fit_wall_s": wall}
+
+
+def bench_fit(args, lw, frames):
+    blk = fit_block(lw, frames, args.steps, args.warmup, args.fit_steps, args.spp or 128)
     return {
         "metric": METRIC,
-        "value": samples / (step_wall / 1e3) / 1e6,
+        "value": blk["samples"] / (blk["step_wall_ms"] / 1e3) / 1e6,
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": step_wall,
+        "ms_per_step": blk["step_wall_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 path arithmetic, f64 accumulation and gradient sums",
         "data": "synthetic (seeded value-noise voxel world, game dusk sky maps)",
-        "config": {"workload": "cfg4: batched inverse fit, 32 seeded views of the cfg1 world, "
-                               f"640x360, {spp} spp, 6 bounces; 192 albedo factors and 128 "
-                               "emitter-intensity groups; Adam lr 1e-2 on the device",
-                   "views": len(bp.views), "spp": spp, "parallelism": "1 GPU"},
+        "config": {"workload": blk["workload"], "views": blk["views"], "spp": blk["spp"],
+                   "parallelism": "1 GPU"},
         "value_is": "view samples per second of a whole fit step (forward, loss, adjoint, update)",
-        "forward_msamples_per_s": samples * len(timed) / (fwd / 1e3) / 1e6,
-        "adjoint_msamples_per_s": samples * len(timed) / (adj / 1e3) / 1e6,
-        "adjoint_over_forward": adj / fwd,
-        "fit_steps": n_steps,
-        "loss_first": steps[0]["loss"],
-        "loss_last": steps[-1]["loss"],
-        "param_error_start": {"albedo": e_start[0], "intensity": e_start[1]},
-        "param_error_end": {"albedo": e_end[0], "intensity": e_end[1]},
-        "fit_wall_s": wall,
+        **{k: blk[k] for k in ("forward_msamples_per_s", "adjoint_msamples_per_s",
+                               "adjoint_over_forward", "fit_steps", "loss_first", "loss_last",
+                               "param_error_start", "param_error_end", "fit_wall_s")},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -548,6 +637,8 @@ def main():
     ap.add_argument("--spp-scale", type=float, default=1.0, help="config 2: scale every view's spp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-metric-parts", action="store_true",
+                    help="config 1: leave out the batched (cfg2) and adjoint (cfg3, cfg4) blocks")
     ap.add_argument("--copy-probe", type=int, default=0, help="also time a copy of this many bytes")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch shape by timing probes first (lw_hip_autotune)")

@@ -132,6 +132,8 @@ def lib():
         L.lwo64_render_hq.restype = i32
         L.lwo64_render_samples.argtypes = [vp, sz, vp, vp, vp, vp]
         L.lwo64_render_samples.restype = i32
+        L.lwo64_render_rows.argtypes = [vp, u32, ctypes.c_uint, vp, sz, vp]
+        L.lwo64_render_rows.restype = i32
         L.lwo_build_layout.argtypes = [vp, vp, vp, vp, sz, vp, sz, vp, sz, u32, vp, vp, vp]
         L.lwo_build_layout.restype = i32
         L.lwo_build_fetch.argtypes = [vp, vp, vp]
@@ -349,6 +351,17 @@ def render_hq_f64(view, spp: int, threads: int = 0):
     rc = lib().lwo64_render_hq(view.ref, spp, threads or os.cpu_count() or 1, _addr(out))
     if rc != 0:
         raise RuntimeError("oracle: f64 render failed")
+    return out
+
+
+def render_rows_f64(view, spp: int, rows, threads: int = 0):
+    """render_hq_f64 on the listed rows only: (len(rows), w, 3) float64."""
+    w = int(view.params["frame"][0])
+    r = np.ascontiguousarray(rows, dtype="<u4")
+    out = np.zeros((r.size, w, 3), dtype=np.float64)
+    if lib().lwo64_render_rows(view.ref, spp, threads or os.cpu_count() or 1, _addr(r), r.size,
+                               _addr(out)) != 0:
+        raise RuntimeError("oracle: f64 row render failed")
     return out
 
 

@@ -40,4 +40,57 @@ int lwo64_render_samples(const View *view, size_t n, const uint32_t *px, const u
   }
 }
 
+// render_hq in f64 on the listed rows only: nrows * w * 3 doubles, row by row in the list's
+// order, each pixel the f64 mean of its finite samples (render/cpu/mod.rs:66-79). For
+// stating the error of a large frame on a subset of its rows.
+int lwo64_render_rows(const View *view, uint32_t spp_in, unsigned threads, const uint32_t *rows,
+                      size_t nrows, double *out_rgb) {
+  if (threads == 0) threads = 1;
+  try {
+    Params params = *view->params;
+    params.frame[2] = 0;
+    View v = *view;
+    v.params = &params;
+    const uint32_t w = params.frame[0], h = params.frame[1];
+    const uint32_t spp = spp_in > 0 ? spp_in : 1;
+    for (size_t r = 0; r < nrows; r++)
+      if (rows[r] >= h) return -1;
+    std::atomic<size_t> next{0};
+    std::vector<std::string> errors(threads);
+    auto worker = [&](unsigned tid) {
+      try {
+        for (;;) {
+          const size_t r = next.fetch_add(1);
+          if (r >= nrows) break;
+          for (uint32_t x = 0; x < w; x++) {
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (uint32_t sample = 0; sample < spp; sample++) {
+              const V3 c = render_sample(v, x, rows[r], sample);
+              if (finite3(c)) {
+                acc[0] += c.x;
+                acc[1] += c.y;
+                acc[2] += c.z;
+              }
+            }
+            double *px = out_rgb + (r * w + x) * 3;
+            for (int k = 0; k < 3; k++) px[k] = acc[k] / (double)spp;
+          }
+        }
+      } catch (const std::exception &e) {
+        errors[tid] = e.what();
+        next.store(nrows);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; t++) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto &t : pool) t.join();
+    for (auto &e : errors)
+      if (!e.empty()) return -1;
+    return 0;
+  } catch (const std::exception &) {
+    return -1;
+  }
+}
+
 }  // extern "C"

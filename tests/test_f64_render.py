@@ -70,3 +70,12 @@ def test_samples_agree_to_f32_rounding(oracle):
         x64 = oracle.render_samples_f64(v, px, py, s)
         rel = np.max(np.abs(x32 - x64) / np.maximum(np.abs(x64), 1e-3), axis=1)
         assert (rel <= 1e-4).mean() > 0.99
+
+
+def test_the_f64_row_render_is_the_f64_frames_rows(oracle):
+    # bench.py states the 720p frame's error against f64 on a subset of rows
+    from lumenwright_amd import frames
+    fr = frames.make_frame(frames.config(1, width=80, height=45, world_chunks_x=4, world_chunks_z=4))
+    rows = [0, 7, 22, 44]
+    whole = oracle.render_hq_f64(fr.view(), 3)
+    assert np.array_equal(oracle.render_rows_f64(fr.view(), 3, rows), whole[rows])
