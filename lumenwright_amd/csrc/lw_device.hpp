@@ -61,6 +61,14 @@ struct Hit {
   float s, tt;
 };
 
+// buf[i] through a 32-bit byte offset from the buffer's base, so that the load takes the
+// base from scalar registers and the offset from one vector register (every scene buffer is
+// smaller than 4 GiB, checked on the host).
+template <class T>
+__device__ __forceinline__ T ld(const T *__restrict__ buf, uint32_t i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(buf) + i * (uint32_t)sizeof(T));
+}
+
 __device__ __forceinline__ float get4(float4 a, int i) {
   return i == 0 ? a.x : (i == 1 ? a.y : a.z);
 }
@@ -72,7 +80,7 @@ __device__ __forceinline__ uint32_t texel(const DevScene &s, const lw_params &p,
   int32_t h = (int32_t)p.off2[2];
   int32_t x = clamp_i(as_i32(f_floor(u)), 0, w - 1);
   int32_t y = clamp_i(as_i32(f_floor(v)), 0, h - 1);
-  return s.atlas[(uint32_t)(y * w + x)];
+  return ld(s.atlas, (uint32_t)(y * w + x));
 }
 
 // integrator.rs:27-34 with scene.rs:299-306: (byte / 255.0) decoded by gamma_to_linear,
@@ -144,16 +152,16 @@ __device__ __forceinline__ bool intersect_quad(const float4 *q, uint32_t info, f
 __device__ __forceinline__ void test_quad(const DevScene &s, const lw_params &p, uint32_t qi,
                                           V3 o, V3 d, float tmin, Hit &best) {
   const float4 *q = s.quads + (size_t)qi * 5;
-  float4 p0 = q[0];
+  float4 p0 = ld(s.quads, qi * 5u);
   uint32_t flags = f_bits(p0.w);
-  uint32_t info = s.qinfo[qi];
+  uint32_t info = ld(s.qinfo, qi);
   float reach = (best.quad != UINT32_MAX && qi > best.quad) ? best.t + COPLANAR : best.t;
   float t, ss, tt;
   if (!intersect_quad(q, info, p0, (flags & LW_FLAG_FRONT_ONLY) != 0, o, d, tmin, reach, t, ss,
                       tt))
     return;
-  float4 uv = q[3];
-  float4 uvt = q[4];
+  float4 uv = ld(s.quads, qi * 5u + 3u);
+  float4 uvt = ld(s.quads, qi * 5u + 4u);
   float cs = f_min(f_max(ss, 0.0005f), 0.9995f);
   float ct = f_min(f_max(tt, 0.0005f), 0.9995f);
   float u = uv.x + cs * uv.z + ct * uvt.x;
@@ -225,7 +233,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
   const uint32_t *ov_top = s.grid + p.off[2];
   const float best_t = t_end + 1e-3f;
 
-  uint32_t cached_b = UINT32_MAX, entry = EMPTY, ov = EMPTY, boxes = 0xFFu;
+  uint32_t cached_b = UINT32_MAX, entry = EMPTY, ov = EMPTY;
   const uint32_t octant = (step[0] >= 0 ? 1u : 0u) | (step[1] >= 0 ? 2u : 0u) | (step[2] >= 0 ? 4u : 0u);
   uint2 occ = make_uint2(0u, 0u);
   for (uint32_t it = 0; it < MAX_STEPS; it++) {
@@ -234,11 +242,10 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
     const uint32_t bz = (uint32_t)(cell[2] >> 4);
     const uint32_t b = (by * dz + bz) * dx + bx;
     if (b != cached_b) {
-      entry = top[b];
-      ov = ov_top[b];
-      if (cell_out == nullptr) boxes = s.octant_boxes[b];
+      entry = ld(top, b);
+      ov = ld(ov_top, b);
       cached_b = b;
-      if (entry != EMPTY && (entry & UNIFORM) == 0) occ = s.occupancy[entry];
+      if (entry != EMPTY && (entry & UNIFORM) == 0) occ = ld(s.occupancy, entry);
     }
     if (cell_out) {
       cell_out[0] = cell[0];
@@ -255,7 +262,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
       // without overlays, no quad will be tested and the result is "no hit" wherever the
       // walk would end (t_end, the grid's faces or MAX_STEPS). Only the final cell differs,
       // so a caller asking for it (cell_out) gets the whole walk.
-      if (cell_out == nullptr && ((boxes >> octant) & 1u) == 0u) return false;
+      if (cell_out == nullptr && ((ld(s.octant_boxes, b) >> octant) & 1u) == 0u) return false;
       // Skip the whole brick: jump to its exit face.
       const int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
       float t_out = INFINITY;
@@ -290,7 +297,15 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
       continue;
     }
 
-    if (entry != EMPTY && (entry & UNIFORM) == 0 && ov == EMPTY) {
+    const bool stored = entry != EMPTY && (entry & UNIFORM) == 0;
+    // The occupancy bit of the cell's 4^3 sub-block (1 outside stored bricks).
+    uint32_t sb_full = 1u;
+    if (stored) {
+      const uint32_t sb = ((((uint32_t)cell[1] & 15u) >> 2) * 4u + (((uint32_t)cell[2] & 15u) >> 2)) * 4u +
+                          (((uint32_t)cell[0] & 15u) >> 2);
+      sb_full = ((sb < 32u ? occ.x : occ.y) >> (sb & 31u)) & 1u;
+    }
+    if (stored && ov == EMPTY) {
       // An empty 4^3 sub-block (no model, no overlay in any of its cells): cross it in one
       // move, landing where the cell-by-cell walk below would, with the same t, cells and
       // step count. The walk is a merge of the three axes' crossing times (each a pure
@@ -299,10 +314,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
       // then each other axis has made the crossings that come before (ts, x) in that order.
       // Taken only when that exit stays in the grid, within t_end and within MAX_STEPS;
       // otherwise the cells are walked one by one.
-      const uint32_t sb = ((((uint32_t)cell[1] & 15u) >> 2) * 4u + (((uint32_t)cell[2] & 15u) >> 2)) * 4u +
-                          (((uint32_t)cell[0] & 15u) >> 2);
-      const uint32_t bits = sb < 32u ? occ.x : occ.y;
-      if (((bits >> (sb & 31u)) & 1u) == 0u) {
+      if (sb_full == 0u) {
         // When the 8^3 block holding the sub-block is empty too (its eight sub-blocks are the
         // occupancy bits 0x330033 shifted by 2 x8 + 8 z8 + 32 y8), it is crossed the same way:
         // the argument holds for any aligned box of empty cells inside one brick.
@@ -334,15 +346,20 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
             moves += (uint32_t)((far[a] - c) * step[a]);
             c = far[a] + step[a];
           } else if (step[a] != 0) {
-            while (c != far[a]) {
-              const float tb = next_boundary(get(o, a), get(inv, a), c, step[a]);
-              if (tb < ts || (tb == ts && a > x)) {
-                c += step[a];
-                moves++;
-              } else {
-                break;
-              }
+            // The walk crosses axis a's boundaries from c toward far[a] while each comes
+            // before (ts, x). Their times never fall from one crossing to the next (f32
+            // rounding is monotone), so the crossings made are a prefix, and its length is
+            // found by binary lifting over the at most span - 1 of them, branch-free.
+            const int32_t avail = (far[a] - c) * step[a];
+            int32_t k = 0;
+#pragma unroll
+            for (int32_t st = 4; st >= 1; st >>= 1) {
+              const int32_t j = k + st;
+              const float tb = next_boundary(get(o, a), get(inv, a), c + (j - 1) * step[a], step[a]);
+              if (j <= avail && (tb < ts || (tb == ts && a > x))) k = j;
             }
+            c += k * step[a];
+            moves += (uint32_t)k;
           }
           nc[a] = c;
         }
@@ -372,29 +389,24 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
       model = 0;
     } else if ((entry & UNIFORM) != 0) {
       model = entry & ~UNIFORM;
+    } else if (sb_full) {
+      const uint32_t w = ld(s.grid, p.off[1] + entry * LW_BRICK_WORDS + (local >> 1));
+      model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
     } else {
-      const uint32_t sb = ((((uint32_t)cell[1] & 15u) >> 2) * 4u + (((uint32_t)cell[2] & 15u) >> 2)) * 4u +
-                          (((uint32_t)cell[0] & 15u) >> 2);
-      const uint32_t bits = sb < 32u ? occ.x : occ.y;
-      if ((bits >> (sb & 31u)) & 1u) {
-        const uint32_t w = s.grid[p.off[1] + entry * LW_BRICK_WORDS + (local >> 1)];
-        model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
-      } else {
-        model = 0;
-      }
+      model = 0;
     }
     Hit cand{f_min(t_hi, best_t), UINT32_MAX, 0u, 0.0f, 0.0f};
     if (model != 0) {
-      const lw_model m = s.models[model];
+      const lw_model m = ld(s.models, model);
       const V3 lo = o - v3((float)cell[0], (float)cell[1], (float)cell[2]);
       for (uint32_t i = 0; i < m.quad_count; i++) test_quad(s, p, m.quad_offset + i, lo, d, t_lo, cand);
     }
     if (ov != EMPTY) {
       const uint32_t base = p.off[3] + ov * LW_OV_CELL_WORDS + local * 2u;
-      const uint32_t off = s.grid[base];
-      const uint32_t count = s.grid[base + 1];
+      const uint32_t off = ld(s.grid, base);
+      const uint32_t count = ld(s.grid, base + 1);
       for (uint32_t i = 0; i < count; i++) {
-        const uint32_t qi = s.grid[p.off2[0] + off + i];
+        const uint32_t qi = ld(s.grid, p.off2[0] + off + i);
         test_quad(s, p, qi, o, d, t_lo, cand);
       }
     }

@@ -1256,6 +1256,15 @@ void set_err(lw_hip_backend *b, const std::string &msg) {
   g_last_error = msg;
 }
 
+// Whether every scene buffer is smaller than 4 GiB (element counts of the grid, models, quads,
+// atlas, emitters, emitter grid and sky), as the kernels' 32-bit byte offsets need.
+bool buffers_fit_32bit(uint64_t grid, uint64_t models, uint64_t quads, uint64_t atlas,
+                       uint64_t emitters, uint64_t egrid, uint64_t sky) {
+  const uint64_t lim = 0xFFFFFFFFull;
+  return grid * 4 <= lim && models * sizeof(lw_model) <= lim && quads * sizeof(lw_quad) <= lim &&
+         atlas * 4 <= lim && emitters * sizeof(lw_emitter) <= lim && egrid * 4 <= lim && sky * 4 <= lim;
+}
+
 // Every index the kernels compute from the scene buffers stays in bounds: the checks the
 // reference's slice indexing makes at run time (a panic there), made once here.
 void validate(const lw_scene_view &v) {
@@ -1267,6 +1276,10 @@ void validate(const lw_scene_view &v) {
     fail("a scene buffer is null");
   if (v.models_len == 0 || v.quads_len == 0 || v.emitters_len == 0 || v.atlas_len == 0)
     fail("models, quads, emitters and atlas must not be empty");
+  // The kernels address every scene buffer by a 32-bit byte offset from its base.
+  if (!buffers_fit_32bit(v.grid_len, v.models_len, v.quads_len, v.atlas_len, v.emitters_len,
+                         v.emitter_grid_len, v.sky ? v.sky_len : 0))
+    throw Unsupported{"a scene buffer of 4 GiB or more"};
   const uint64_t dx = p.grid_dims[0], dy = p.grid_dims[1], dz = p.grid_dims[2];
   if (dx == 0 || dy == 0 || dz == 0 || dx * 16 > 0x7FFFFFF || dy * 16 > 0x7FFFFFF ||
       dz * 16 > 0x7FFFFFF)
@@ -1562,6 +1575,9 @@ void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, l
   sc.emitters.adopt(o.emitters, o.emitters_len);
   sc.egrid.adopt(o.egrid, o.egrid_len);
   sc.occupancy.adopt(o.occupancy, o.occupancy_len);
+  if (!buffers_fit_32bit(o.grid_len, v.models_len, quads.size(), v.atlas_len, o.emitters_len, o.egrid_len,
+                         v.sky ? v.sky_len : 0))
+    throw Unsupported{"a scene buffer of 4 GiB or more"};
   build_octant_boxes(b, sc, v.dims, o.offsets[0], o.offsets[2]);
   uint64_t max_faces = 0;
   for (size_t m = 0; m < v.models_len; m++)
