@@ -263,7 +263,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
       // walk would end (t_end, the grid's faces or MAX_STEPS). Only the final cell differs,
       // so a caller asking for it (cell_out) gets the whole walk.
       if (cell_out == nullptr && ((ld(s.octant_boxes, b) >> octant) & 1u) == 0u) return false;
-      // Skip the whole brick: jump to its exit face.
+      // An empty brick is crossed in one move, to the face where the ray leaves it.
       const int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
       float t_out = INFINITY;
       int axis = 0;

@@ -136,7 +136,8 @@ __device__ bool setup(const ClipVert *v, float w, float h, uint32_t flags, Tri &
                           v[i].color.w, v[i].pos[0], v[i].pos[1], v[i].pos[2]};
     for (int k = 0; k < 9; k++) attrs[i][k] = src[k] * iw;
   }
-  // Screen space is y-down, so counter-clockwise (front) triangles have negative area.
+  // Rows grow downward on screen: a front-facing (counter-clockwise) triangle comes out with a
+  // negative signed area here, and anything else is culled.
   const float area = (sv[1].x - sv[0].x) * (sv[2].y - sv[0].y) - (sv[1].y - sv[0].y) * (sv[2].x - sv[0].x);
   if (area >= 0.0f) return false;
   float min_x = 3.40282347e38f, max_x = -3.40282347e38f, min_y = 3.40282347e38f, max_y = -3.40282347e38f;
@@ -152,7 +153,8 @@ __device__ bool setup(const ClipVert *v, float w, float h, uint32_t flags, Tri &
   t.bbox[2] = min(lw::as_i32(ceilf(max_x)), wi - 1);
   t.bbox[3] = min(lw::as_i32(ceilf(max_y)), hi - 1);
   if (t.bbox[0] > t.bbox[2] || t.bbox[1] > t.bbox[3]) return false;
-  // Reorder to positive area for the edge functions.
+  // Swapping the last two vertices makes the signed area positive, as the edge functions
+  // below expect.
   const int order[3] = {0, 2, 1};
   for (int i = 0; i < 3; i++) {
     t.v[i] = sv[order[i]];
@@ -177,7 +179,7 @@ __global__ void __launch_bounds__(kB) k_tris(const uint32_t *__restrict__ idx, u
                          : (g < n_opaque + n_translucent ? TRI_TRANSLUCENT : (TRI_TRANSLUCENT | TRI_SEE_THROUGH));
   ClipVert tv[3];
   for (int k = 0; k < 3; k++) tv[k] = verts[idx[3 * (size_t)g + k]];
-  // Trivial reject if all vertices are outside one side.
+  // A triangle whose three vertices all lie beyond the same clip plane is dropped at once.
   bool out[5] = {true, true, true, true, true};
   for (int k = 0; k < 3; k++) {
     const float4 c = tv[k].clip;
@@ -415,7 +417,8 @@ __device__ void shade(const Tri &t, uint32_t px, uint32_t py, const Atlas &A, co
   const float pw = 1.0f / iw;
   float at[9];
   for (int a = 0; a < 9; a++) at[a] = (b0 * t.a[0][a] + b1 * t.a[1][a] + b2 * t.a[2][a]) * pw;
-  // The plane of the UVs at another pixel centre (a GPU helper lane).
+  // The UVs the triangle's plane gives at a neighbouring pixel centre, as a quad of GPU
+  // fragments would see them in its helper lanes.
   auto uv_at = [&](float x, float y, float &u, float &v) {
     const float c0 = edge(t.v[1], t.v[2], x, y) * t.area_inv;
     const float c1 = edge(t.v[2], t.v[0], x, y) * t.area_inv;

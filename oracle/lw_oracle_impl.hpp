@@ -549,7 +549,7 @@ static bool trace(const View &s, V3 o, V3 d, real tmin, real tmax, Hit &out,
         if ((((*tl_octant_boxes)[b] >> octant) & 1u) == 0u) return false;
       }
 #endif
-      // Skip the whole brick: jump to its exit face.
+      // An empty brick is crossed in one move, to the face where the ray leaves it.
       int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
       real t_out = INFINITY;
       int axis = 0;
