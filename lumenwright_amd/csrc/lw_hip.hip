@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -421,6 +422,92 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
   o[0] = c.x;
   o[1] = c.y;
   o[2] = c.z;
+}
+
+// The direct form in one persistent launch, for a frame or a batch of views. A work item is one
+// 4 x 4 pixel tile of one view with all of its samples (view << 24 | tile, tiles numbered row
+// by row; without `items`, item k is tile k of view 0). Whole waves take items from a device
+// counter. A wave's lanes are 16 pixels x 4 sample slots: in round r, slot j renders sample
+// 4 r + j of its pixel, and the pixel's slot-0 lane gathers the four results by shuffles and
+// adds the finite ones to the pixel's f64 sums in sample order (the CPU backend's order, so the
+// image is render_hq's bit for bit, render/cpu/mod.rs:66-79), kept in LDS; it writes the
+// pixel's mean at the end. No staging and no accumulation pass. A wave leaves once the counter
+// has passed the last item, so every wave ends. With `progress_host`, the index of each item
+// taken is stored to host memory for the caller's progress reports.
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_tiles(const lw_params p, const DevScene s,
+                                                  const BatchDev *__restrict__ bd,
+                                                  const uint32_t *__restrict__ items, uint32_t n_items,
+                                                  uint32_t *__restrict__ counter, float *__restrict__ out,
+                                                  uint32_t *progress_host) {
+  __shared__ float lut_s[256];
+  __shared__ double sums[(kBlock / 4) * 3];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t lane = threadIdx.x & 63u, pix = lane & 15u, slot = lane >> 4;
+  double *acc = sums + (size_t)((threadIdx.x >> 6) * 16u + pix) * 3;
+  for (;;) {
+    uint32_t k = 0;
+    if (lane == 0) {
+      k = atomicAdd(counter, 1u);
+      if (progress_host && k < n_items)
+        __hip_atomic_store(progress_host, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    k = __shfl(k, 0);
+    if (k >= n_items) break;
+    const uint32_t item = items ? items[k] : k;
+    const uint32_t v = item >> 24, tile = item & 0xFFFFFFu;
+    const ViewCam cam = bd->cam[v];
+    const uint32_t tiles_x = (cam.width + 3u) / 4u;
+    const uint32_t px = (tile % tiles_x) * 4u + (pix & 3u);
+    const uint32_t py = (tile / tiles_x) * 4u + (pix >> 2);
+    const bool inside = px < cam.width && py < cam.height;
+    if (slot == 0u) {
+      acc[0] = 0.0;
+      acc[1] = 0.0;
+      acc[2] = 0.0;
+    }
+    for (uint32_t base = 0; base < cam.spp; base += 4u) {
+      const uint32_t smp = base + slot;
+      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+      uint32_t ok = 0u;
+      if (inside && smp < cam.spp) {
+        const V3 c = render_sample(s, p, cam, lut_s, px, py, smp);
+        cx = c.x;
+        cy = c.y;
+        cz = c.z;
+        ok = (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) ? 1u : 0u;
+      }
+      float gx[4], gy[4], gz[4];
+      uint32_t gok[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        gx[j] = __shfl(cx, (int)(pix + 16u * j));
+        gy[j] = __shfl(cy, (int)(pix + 16u * j));
+        gz[j] = __shfl(cz, (int)(pix + 16u * j));
+        gok[j] = __shfl(ok, (int)(pix + 16u * j));
+      }
+      if (slot == 0u && inside) {
+        double a0 = acc[0], a1 = acc[1], a2 = acc[2];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (gok[j]) {
+            a0 += (double)gx[j];
+            a1 += (double)gy[j];
+            a2 += (double)gz[j];
+          }
+        }
+        acc[0] = a0;
+        acc[1] = a1;
+        acc[2] = a2;
+      }
+    }
+    if (slot == 0u && inside) {
+      float *o = out + ((size_t)bd->pix_prefix[v] + (size_t)py * cam.width + px) * 3;
+      o[0] = (float)(acc[0] / (double)cam.spp);
+      o[1] = (float)(acc[1] / (double)cam.spp);
+      o[2] = (float)(acc[2] / (double)cam.spp);
+    }
+  }
 }
 
 // ---- sky preparation (render/sky.rs, integrator.rs:133-148) ----
@@ -1227,6 +1314,17 @@ struct lw_hip_backend {
   uint64_t batch_staging = 2ull << 30;
   uint32_t batch_passes = 0;
   bool use_graphs = true;
+  // LW_BATCH_FORM=passes: the direct form's batches in passes (k_render_batch) rather than in
+  // one persistent launch over (view, tile) items (k_tiles).
+  bool batch_passes_form = false;
+  // LW_FRAME_FORM=launches: frames in launches of up to 64 samples per pixel with staging and
+  // an f64 accumulation pass (render/cuda.rs's ramp) rather than in one persistent k_tiles
+  // launch.
+  bool frame_launches = false;
+  uint32_t *progress_host = nullptr;  // host memory a k_tiles launch reports its items to
+  uint32_t *progress_dev = nullptr;   // the same memory's device address
+  DevBuf<uint32_t> batch_items;
+  uint32_t blocks_tiles = 0;  // resident workgroups of k_tiles
   DevBuf<BatchDev> batch_dev;
   DevBuf<float> tone_in;
   DevBuf<uint32_t> tone_out;
@@ -1700,6 +1798,80 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
 }
 
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
+// Frames with at least this many samples per pixel render in one persistent k_tiles launch
+// (a wave's 4 sample slots are then busy in most rounds).
+constexpr uint32_t kTilesMinSpp = 16;
+constexpr uint32_t kTilesPerWave = 4;
+
+// A frame in one persistent launch (k_tiles over its 4 x 4 tiles, all samples of a tile in one
+// wave, summed in sample order in f64): the image equals the launch ramp's bit for bit. With
+// `progress`, the host reports (samples done, spp) while the launch runs, from the item index
+// the kernel last stored to host memory, and (spp, spp) at the end.
+void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, uint32_t spp,
+                        lw_progress_fn progress, void *user, float *out_rgb,
+                        std::chrono::steady_clock::time_point t_wall) {
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint64_t npix = (uint64_t)w * h;
+  if (npix > (1ull << 26)) throw InvalidScene{"frame too large"};
+  const uint32_t n_items = ((w + 3) / 4) * ((h + 3) / 4);
+  BatchPlan plan;
+  BatchDev &bd = plan.bd();
+  std::memset(&bd, 0, sizeof(bd));
+  ViewCam &c = bd.cam[0];
+  std::memcpy(c.cam_pos, p.cam_pos, sizeof(c.cam_pos));
+  std::memcpy(c.cam_fwd, p.cam_fwd, sizeof(c.cam_fwd));
+  std::memcpy(c.cam_right, p.cam_right, sizeof(c.cam_right));
+  std::memcpy(c.cam_up, p.cam_up, sizeof(c.cam_up));
+  c.width = w;
+  c.height = h;
+  c.seed = p.misc_u[0];
+  c.spp = spp;
+  bd.nviews = 1;
+  bd.pix_prefix[1] = (uint32_t)npix;
+  bd.npix_total = (uint32_t)npix;
+  b->batch_dev.ensure(1);
+  b->out.ensure(npix * 3);
+  HIP_TRY(hipMemcpyAsync(b->batch_dev.ptr, &bd, sizeof(BatchDev), hipMemcpyHostToDevice, b->stream));
+  k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
+  HIP_TRY(hipGetLastError());
+  if (progress) __atomic_store_n(b->progress_host, 0u, __ATOMIC_RELAXED);
+  const DevScene ds = sc.dev();
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (n_items + 3) / 4));
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  LW_DIRECT_DISPATCH(b->waves, (k_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+                                   p, ds, b->batch_dev.ptr, nullptr, n_items, b->counter.ptr, b->out.ptr,
+                                   progress ? b->progress_dev : nullptr)));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  if (progress) {
+    uint32_t reported = 0;
+    for (;;) {
+      const hipError_t q = hipEventQuery(b->ev[1]);
+      if (q == hipSuccess) break;
+      if (q != hipErrorNotReady) throw HipError{q, "hipEventQuery"};
+      std::this_thread::sleep_for(std::chrono::milliseconds(20));
+      // every item before the one last stored has been taken; count those samples as done
+      const uint64_t taken = __atomic_load_n(b->progress_host, __ATOMIC_RELAXED);
+      const uint32_t done = (uint32_t)std::min<uint64_t>(spp - 1, taken * spp / n_items);
+      if (done > reported) {
+        reported = done;
+        progress(done, spp, user);
+      }
+    }
+  }
+  if (out_rgb)  // without it the image stays in b->out for the caller's next kernel
+    HIP_TRY(hipMemcpyAsync(out_rgb, b->out.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (progress) progress(spp, spp, user);
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  lw_timing tm{};
+  tm.trace_ms = ms;
+  tm.launches = 1;
+  tm.samples = npix * spp;
+  tm.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
+  b->timing = tm;
+}
 constexpr uint64_t kFillBlocks = 2048;     // 8 workgroups of 256 lanes per CU
 constexpr uint32_t kMaxBatch = 64;         // render/cuda.rs:165
 
@@ -1743,6 +1915,17 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   const uint32_t spp = std::max(spp_in, 1u);
   HIP_TRY(hipSetDevice(b->device));
   const DevScene ds = sc.dev();
+
+  // The persistent form needs at least kTilesPerWave items for each resident wave, so that the
+  // last items to start are a small part of the launch; smaller frames fill the device
+  // through the launch ramp's many samples per launch.
+  const uint64_t tile_items = (uint64_t)((w + 3) / 4) * ((h + 3) / 4);
+  if (!diff && !out_f64 && !one_launch && !fixed && b->mode == MODE_DIRECT && b->per_lane == 1 &&
+      !b->frame_launches && spp >= kTilesMinSpp &&
+      tile_items >= (uint64_t)kTilesPerWave * b->blocks_tiles * (kBlock / 64)) {
+    render_frame_tiles(b, sc, p, spp, progress, user, out_rgb, t_wall);
+    return;
+  }
 
   // Staging for up to kMaxBatch samples per launch, capped at 2 GiB (or at what the caller's
   // memory holds).
@@ -1948,14 +2131,19 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
                  samples = plan.samples;
   HIP_TRY(hipSetDevice(b->device));
   const DevScene ds = sc.dev();
+  const bool tiles_form = !diff && outs && b->mode == MODE_DIRECT && !b->batch_passes_form;
   b->batch_dev.ensure(1);
-  b->rad.ensure(staging / sizeof(float) + 1);
-  b->acc.ensure(npix_total * 3);
+  if (!tiles_form) {
+    b->rad.ensure(staging / sizeof(float) + 1);
+    b->acc.ensure(npix_total * 3);
+  }
   b->out.ensure(npix_total * 3);
   HIP_TRY(hipMemcpyAsync(b->batch_dev.ptr, &bd, sizeof(BatchDev), hipMemcpyHostToDevice, b->stream));
-  k_zero<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
-      b->acc.ptr, npix_total * 3);
-  HIP_TRY(hipGetLastError());
+  if (!tiles_form) {
+    k_zero<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(
+        b->acc.ptr, npix_total * 3);
+    HIP_TRY(hipGetLastError());
+  }
   const BatchWork work{b->batch_dev.ptr, b->rad.ptr};
   const dim3 acc_grid((unsigned)((npix_total + kBlock - 1) / kBlock));
   // Pass 0 has the most items (a view's samples per pass only shrink at its last pass).
@@ -1977,7 +2165,30 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   };
   lw_timing tm{};
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-  if (b->mode == MODE_WAVEFRONT && !diff) {
+  if (tiles_form) {
+    // One persistent launch over (view, 8 x 8 tile) items, the views with the most samples
+    // per pixel first so that the last items to start are the shortest.
+    std::vector<uint32_t> order(nv);
+    for (size_t v = 0; v < nv; v++) order[v] = (uint32_t)v;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t c) { return bd.cam[a].spp > bd.cam[c].spp; });
+    std::vector<uint32_t> list;
+    for (uint32_t v : order) {
+      const uint32_t n = ((bd.cam[v].width + 3) / 4) * ((bd.cam[v].height + 3) / 4);
+      for (uint32_t t = 0; t < n; t++) list.push_back(v << 24 | t);
+    }
+    b->batch_items.upload(list.data(), list.size(), b->stream);
+    k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (uint32_t)((list.size() + 3) / 4)));
+    LW_DIRECT_DISPATCH(b->waves, (k_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+                                     p0, ds, b->batch_dev.ptr, b->batch_items.ptr, (uint32_t)list.size(),
+                                     b->counter.ptr, b->out.ptr, nullptr)));
+    HIP_TRY(hipGetLastError());
+    if (progress) {
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      progress(1, 1, user);
+    }
+  } else if (b->mode == MODE_WAVEFRONT && !diff) {
     // The wavefront loop's length depends on the pass, so each pass is launched from here.
     for (uint32_t k = 0; k < npass; k++) {
       k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, b->stream>>>(b->batch_dev.ptr);
@@ -2025,9 +2236,11 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   }
   HIP_TRY(hipEventRecord(b->ev[1], b->stream));
   if (outs) {
-    k_resolve_batch<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                      b->stream>>>(b->batch_dev.ptr, b->acc.ptr, b->out.ptr);
-    HIP_TRY(hipGetLastError());
+    if (!tiles_form) {
+      k_resolve_batch<<<dim3((unsigned)((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                        b->stream>>>(b->batch_dev.ptr, b->acc.ptr, b->out.ptr);
+      HIP_TRY(hipGetLastError());
+    }
     for (size_t v = 0; v < nv; v++) {
       const size_t n3 = (size_t)bd.cam[v].width * bd.cam[v].height * 3;
       HIP_TRY(hipMemcpyAsync(outs[v], b->out.ptr + (size_t)bd.pix_prefix[v] * 3, n3 * sizeof(float),
@@ -2038,7 +2251,7 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
   tm.trace_ms = ms;
-  tm.launches = npass;
+  tm.launches = tiles_form ? 1u : npass;
   tm.samples = samples;
   tm.wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall).count();
@@ -2412,8 +2625,20 @@ int lw_hip_open(int device, lw_hip_backend **out) {
       if (v >= kBlock) b->wf_slots = (uint32_t)std::min<long>(v, 1l << 24) / kBlock * kBlock;
     }
     HIP_TRY(hipHostMalloc((void **)&b->pinned, 64, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&b->progress_host, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    HIP_TRY(hipHostGetDevicePointer((void **)&b->progress_dev, b->progress_host, 0));
+    if (const char *e = std::getenv("LW_FRAME_FORM")) b->frame_launches = std::strcmp(e, "launches") == 0;
     if (const char *e = std::getenv("LW_BATCH_GRAPHS")) b->use_graphs = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("LW_BATCH_PASSES")) b->batch_passes = (uint32_t)std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("LW_BATCH_FORM")) b->batch_passes_form = std::strcmp(e, "passes") == 0;
+    {
+      int per_cu = 0;
+      LW_DIRECT_DISPATCH(b->waves, {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&k_tiles<W>), kBlock, 0));
+      });
+      b->blocks_tiles = cus * (uint32_t)std::max(1, per_cu);
+    }
   });
   if (rc != LW_OK) {
     lw_hip_close(b);
@@ -2532,6 +2757,7 @@ void lw_hip_close(lw_hip_backend *b) {
   b->batch_dev.free();
   if (b->quick_ws) lwq::workspace_free(b->quick_ws);
   if (b->pinned) (void)hipHostFree(b->pinned);
+  if (b->progress_host) (void)hipHostFree(b->progress_host);
   for (auto &ev : b->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);

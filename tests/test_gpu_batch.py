@@ -1,7 +1,8 @@
 """Batched rendering (lw_hip_render_hq_batch) against per-view render_hq on the device and
 against the oracle's CPU backend: each view's image must equal the single-view render bit
-for bit (same samples, same f64 summation order), with and without hipGraph replay and in
-both path-kernel forms."""
+for bit (same samples, same f64 summation order), in the one persistent launch over
+(view, tile) items, in passes with and without hipGraph replay, and in every path-kernel
+form."""
 
 import os
 
@@ -37,11 +38,13 @@ def open_backend(env):
                 os.environ[k] = v
 
 
-@pytest.mark.parametrize("env", [{}, {"LW_BATCH_GRAPHS": "0"}, {"LW_BATCH_PASSES": "5"},
+@pytest.mark.parametrize("env", [{}, {"LW_BATCH_FORM": "passes"},
+                                 {"LW_BATCH_FORM": "passes", "LW_BATCH_GRAPHS": "0"},
+                                 {"LW_BATCH_FORM": "passes", "LW_BATCH_PASSES": "5"},
                                  {"LW_KERNEL": "persistent"},
                                  {"LW_KERNEL": "persistent", "LW_BATCH_PASSES": "3"},
                                  {"LW_KERNEL": "wavefront"}],
-                         ids=["graphs", "no-graphs", "5-passes", "persistent",
+                         ids=["tiles", "graphs", "no-graphs", "5-passes", "persistent",
                               "persistent-3-passes", "wavefront"])
 def test_batch_equals_single_view_renders(batch_setup, env):
     base, views = batch_setup

@@ -336,3 +336,36 @@ def test_emitter_sampling_strategies_are_bit_exact(mode_backends, oracle, mode, 
     got = mode_backends[mode].render_samples(v, px, py, smp)
     assert np.array_equal(got.view("<u4"), want.view("<u4")), (strategy, decoy)
     assert n == len(got) and want.mean() > 0.0
+
+
+def test_the_persistent_frame_form_equals_the_launch_ramp(hip_backend, oracle):
+    # A frame with 16 spp or more and enough 4 x 4 tiles to give every resident wave several
+    # (the 720p frame) renders in one persistent launch, every tile's samples summed in sample
+    # order in one wave; the launch ramp with staging and an f64 accumulation pass
+    # (LW_FRAME_FORM=launches) and the CPU backend give the same image bit for bit, and
+    # progress climbs to (spp, spp).
+    import lumenwright_amd as lw
+    fr = frames.make_frame(frames.config(1, spp=20))
+    calls = []
+    got = hip_backend.render_hq(fr, progress=lambda d, t: calls.append((d, t)))
+    t = hip_backend.last_timing()
+    assert t["launches"] == 1 and t["samples"] == 1280 * 720 * 20
+    assert calls and calls[-1] == (20, 20) and all(t == 20 for _, t in calls)
+    assert all(a[0] <= b[0] for a, b in zip(calls, calls[1:]))
+    old = os.environ.get("LW_FRAME_FORM")
+    os.environ["LW_FRAME_FORM"] = "launches"
+    try:
+        ramp = lw.HipBackend.open(0)
+    finally:
+        if old is None:
+            os.environ.pop("LW_FRAME_FORM", None)
+        else:
+            os.environ["LW_FRAME_FORM"] = old
+    try:
+        want = ramp.render_hq(fr)
+        assert ramp.last_timing()["launches"] > 1
+    finally:
+        ramp.close()
+    assert np.array_equal(got.view("<u4"), want.view("<u4"))
+    cpu = oracle.render_hq(fr.view(), 20).reshape(-1, 3)
+    assert np.array_equal(got.view("<u4"), cpu.view("<u4"))
