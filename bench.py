@@ -139,18 +139,21 @@ def roofline(config, samples, trace_ms, launches):
     per_launch = samples / launches
     mean_s = trace_ms / 1e3 / launches
     achieved = b_alg["bytes_per_sample"] * per_launch / mean_s / 1e9
+    persistent = launches == 1 and kernel_form() == "direct, waves 6"
     traffic = None
     traffic_path = ROOT / "profiles" / "traffic.json"
-    if traffic_path.exists() and kernel_form() == "direct, waves 6":
+    if traffic_path.exists() and persistent:
         t = json.loads(traffic_path.read_text()).get(f"config{config}")
         if t:
             traffic = t["hbm_bytes_per_sample"] * per_launch
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)",
+            "kernel": ("k_render_tiles (one persistent launch per frame)" if persistent else
+                       "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)"),
             "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"],
-            "note": "VALU-issue bound: 93% VALU busy at 13% lane utilisation, L2 hit 84% "
-                    "(profiles/k_render_cfg1_720p_valu_l2.md)" if config == 1 else None}
+            "note": "VALU-issue bound: the VALU issues every cycle at 13% lane utilisation, L2 hit "
+                    "86%; HBM traffic 813 B/sample, mostly register spill "
+                    "(profiles/k_render_tiles_cfg1_720p_*.md)" if config == 1 else None}
 
 
 def mrays(config, msamples):

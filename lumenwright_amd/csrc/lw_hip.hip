@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
 // has passed the last item, so every wave ends. With `progress_host`, the index of each item
 // taken is stored to host memory for the caller's progress reports.
 template <int W>
-__global__ void __launch_bounds__(kBlock, W) k_tiles(const lw_params p, const DevScene s,
+__global__ void __launch_bounds__(kBlock, W) k_render_tiles(const lw_params p, const DevScene s,
                                                   const BatchDev *__restrict__ bd,
                                                   const uint32_t *__restrict__ items, uint32_t n_items,
                                                   uint32_t *__restrict__ counter, float *__restrict__ out,
@@ -452,11 +452,13 @@ __global__ void __launch_bounds__(kBlock, W) k_tiles(const lw_params p, const De
       if (progress_host && k < n_items)
         __hip_atomic_store(progress_host, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    k = __shfl(k, 0);
+    // The item and its view are the same for the whole wave: read into scalar registers, so
+    // the view's camera is read by scalar loads where it is used rather than held per lane.
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
     if (k >= n_items) break;
-    const uint32_t item = items ? items[k] : k;
+    const uint32_t item = __builtin_amdgcn_readfirstlane(items ? items[k] : k);
     const uint32_t v = item >> 24, tile = item & 0xFFFFFFu;
-    const ViewCam cam = bd->cam[v];
+    const ViewCam &cam = bd->cam[v];
     const uint32_t tiles_x = (cam.width + 3u) / 4u;
     const uint32_t px = (tile % tiles_x) * 4u + (pix & 3u);
     const uint32_t py = (tile / tiles_x) * 4u + (pix >> 2);
@@ -1315,16 +1317,16 @@ struct lw_hip_backend {
   uint32_t batch_passes = 0;
   bool use_graphs = true;
   // LW_BATCH_FORM=passes: the direct form's batches in passes (k_render_batch) rather than in
-  // one persistent launch over (view, tile) items (k_tiles).
+  // one persistent launch over (view, tile) items (k_render_tiles).
   bool batch_passes_form = false;
   // LW_FRAME_FORM=launches: frames in launches of up to 64 samples per pixel with staging and
-  // an f64 accumulation pass (render/cuda.rs's ramp) rather than in one persistent k_tiles
-  // launch.
+  // an f64 accumulation pass (render/cuda.rs's ramp) rather than in one persistent
+  // k_render_tiles launch.
   bool frame_launches = false;
-  uint32_t *progress_host = nullptr;  // host memory a k_tiles launch reports its items to
+  uint32_t *progress_host = nullptr;  // host memory k_render_tiles reports its items to
   uint32_t *progress_dev = nullptr;   // the same memory's device address
   DevBuf<uint32_t> batch_items;
-  uint32_t blocks_tiles = 0;  // resident workgroups of k_tiles
+  uint32_t blocks_tiles = 0;  // resident workgroups of k_render_tiles
   DevBuf<BatchDev> batch_dev;
   DevBuf<float> tone_in;
   DevBuf<uint32_t> tone_out;
@@ -1798,13 +1800,13 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
 }
 
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
-// Frames with at least this many samples per pixel render in one persistent k_tiles launch
-// (a wave's 4 sample slots are then busy in most rounds).
+// Frames with at least this many samples per pixel render in one persistent k_render_tiles
+// launch (a wave's 4 sample slots are then busy in most rounds).
 constexpr uint32_t kTilesMinSpp = 16;
 constexpr uint32_t kTilesPerWave = 4;
 
-// A frame in one persistent launch (k_tiles over its 4 x 4 tiles, all samples of a tile in one
-// wave, summed in sample order in f64): the image equals the launch ramp's bit for bit. With
+// A frame in one persistent launch (k_render_tiles over its 4 x 4 tiles, all samples of a tile
+// in one wave, summed in sample order in f64): the image equals the launch ramp's bit for bit. With
 // `progress`, the host reports (samples done, spp) while the launch runs, from the item index
 // the kernel last stored to host memory, and (spp, spp) at the end.
 void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, uint32_t spp,
@@ -1838,7 +1840,7 @@ void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_para
   const DevScene ds = sc.dev();
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (n_items + 3) / 4));
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-  LW_DIRECT_DISPATCH(b->waves, (k_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+  LW_DIRECT_DISPATCH(b->waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
                                    p, ds, b->batch_dev.ptr, nullptr, n_items, b->counter.ptr, b->out.ptr,
                                    progress ? b->progress_dev : nullptr)));
   HIP_TRY(hipGetLastError());
@@ -2180,7 +2182,7 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
     b->batch_items.upload(list.data(), list.size(), b->stream);
     k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (uint32_t)((list.size() + 3) / 4)));
-    LW_DIRECT_DISPATCH(b->waves, (k_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+    LW_DIRECT_DISPATCH(b->waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
                                      p0, ds, b->batch_dev.ptr, b->batch_items.ptr, (uint32_t)list.size(),
                                      b->counter.ptr, b->out.ptr, nullptr)));
     HIP_TRY(hipGetLastError());
@@ -2635,7 +2637,7 @@ int lw_hip_open(int device, lw_hip_backend **out) {
       int per_cu = 0;
       LW_DIRECT_DISPATCH(b->waves, {
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_tiles<W>), kBlock, 0));
+            &per_cu, reinterpret_cast<const void *>(&k_render_tiles<W>), kBlock, 0));
       });
       b->blocks_tiles = cus * (uint32_t)std::max(1, per_cu);
     }
