@@ -125,7 +125,7 @@ def parity_block(b, scene, frame, cpu_img, spp: int, n_rows: int = 16) -> dict:
                            "f64_seconds": f64_s}}
 
 
-def roofline(config, samples, trace_ms, launches):
+def roofline(config, samples, trace_ms, launches, steps=1):
     """The dominant kernel against HBM: achieved = algorithmic scene bytes per launch (the
     instrumented oracle's bytes per sample times the launch's samples) over the mean launch
     time; traffic = the HBM bytes per launch the FETCH_SIZE / WRITE_SIZE counters measured for
@@ -139,7 +139,7 @@ def roofline(config, samples, trace_ms, launches):
     per_launch = samples / launches
     mean_s = trace_ms / 1e3 / launches
     achieved = b_alg["bytes_per_sample"] * per_launch / mean_s / 1e9
-    persistent = launches == 1 and kernel_form() == "direct, waves 6"
+    persistent = launches == steps and kernel_form() == "direct, waves 6"
     traffic = None
     traffic_path = ROOT / "profiles" / "traffic.json"
     if traffic_path.exists() and persistent:
@@ -377,7 +377,7 @@ def bench_plain(args, lw, frames):
         "kernel_ms_per_step": trace_ms / args.steps,
         "accumulate_ms_per_step": acc_ms / args.steps,
         "launches_per_step": launches / args.steps,
-        "roofline": roofline(args.config, samples, trace_ms, launches),
+        "roofline": roofline(args.config, samples, trace_ms, launches, args.steps),
         "cpu_baseline": cpu,
         **extra,
     }
@@ -431,6 +431,15 @@ def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0) -> dict:
             "batch_passes": tb["launches"], "batch_trace_ms": tb["trace_ms"]}
 
 
+def batch_roofline(samples, blk):
+    """The batch launch against HBM with the cfg1 world's algorithmic bytes per sample; no
+    traffic figure (the counters were taken on the 720p frame, not on this view mix)."""
+    r = roofline(1, samples, blk["batch_trace_ms"], max(1, blk["batch_passes"]), steps=0)
+    if r and blk["batch_passes"] == 1:
+        r["kernel"] = "k_render_tiles (one persistent launch for the batch)"
+    return r
+
+
 def bench_batch(args, lw, frames):
     blk = batch_block(lw, frames, steps=args.steps, spp_scale=args.spp_scale)
     samples = blk["samples"]
@@ -453,7 +462,7 @@ def bench_batch(args, lw, frames):
         "batched_over_plain": blk["batched_over_plain"],
         "images_identical": blk["images_identical"],
         "batch_passes": blk["batch_passes"],
-        "roofline": roofline(1, samples, blk["batch_trace_ms"], max(1, blk["batch_passes"])),
+        "roofline": batch_roofline(samples, blk),
         "cpu_baseline": None,
     }
 

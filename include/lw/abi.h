@@ -183,7 +183,9 @@ const char *lw_hip_name(const lw_hip_backend *b);
  * mean linear radiance per pixel, row-major, top row first, to out_rgb (w*h*3 floats).
  * Non-finite samples are dropped from the sum and still count in the mean
  * (render/cpu/mod.rs:66-79). Uploads the view's buffers for the call, as the reference
- * does (render/cuda.rs:96-113). */
+ * does (render/cuda.rs:96-113). progress(done, spp) reports whole samples per pixel done,
+ * from the calling thread, as the reference's per-launch callback does (render/cuda.rs:161-163),
+ * and ends with (spp, spp). */
 int lw_hip_render_hq(lw_hip_backend *b, const lw_scene_view *view, uint32_t spp,
                      lw_progress_fn progress, void *user, float *out_rgb);
 
@@ -377,11 +379,12 @@ typedef struct lw_view {
   uint32_t pad[3];
 } lw_view;
 
-/* Renders n views (1..256) of a resident scene as one work pool, in passes; each pass is
- * one persistent launch over all views and replays as a captured hipGraph. out_rgb[i]
- * receives view i's mean radiance, w*h*3 floats, equal to what render_hq gives for the
- * view alone (same samples, same summation order). No counterpart in the reference, which
- * renders one view per call (render/mod.rs:98-102). */
+/* Renders n views (1..256) of a resident scene as one work pool: one persistent launch whose
+ * waves take (view, 4x4-pixel tile) items, the views with most samples first, and sum each
+ * pixel's samples in sample order. out_rgb[i] receives view i's mean radiance, w*h*3 floats,
+ * equal to what render_hq gives for the view alone (same samples, same summation order).
+ * progress, when given, is called once, with (1, 1), after the launch ends. No counterpart in
+ * the reference, which renders one view per call (render/mod.rs:98-102). */
 int lw_hip_render_hq_batch(lw_hip_backend *b, const lw_hip_scene *scene, const lw_view *views,
                            size_t n, lw_progress_fn progress, void *user, float *const *out_rgb);
 
