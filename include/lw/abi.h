@@ -503,6 +503,13 @@ int lw_hip_tonemap_mix(lw_hip_backend *b, const float *hdr, size_t n, float expo
  * on its first call with libm itself; setup_ms (may be null) receives that one-time cost. */
 int lw_hip_encode(lw_hip_backend *b, const float *x, size_t n, float *out, double *setup_ms);
 
+/* Rust's f32 `as i32` and `as u32` (truncation toward zero, saturation at the type's range,
+ * NaN to 0) of n values, computed the way every kernel converts: the reference's casts in
+ * trace.rs, scene.rs and integrator.rs (e.g. trace.rs:161-163, scene.rs:291,
+ * integrator.rs:351). A check of the device conversions for callers and tests; no
+ * counterpart in the reference. */
+int lw_hip_casts(lw_hip_backend *b, const float *x, size_t n, int32_t *out_i, uint32_t *out_u);
+
 /* downsample (post.rs:218-247): box-filters w x h RGBA8 by k into (w/k) x (h/k). */
 int lw_hip_downsample(lw_hip_backend *b, const uint32_t *px, uint32_t w, uint32_t h, uint32_t k,
                       uint32_t *out);

@@ -53,6 +53,7 @@ SIGNATURES = {
     "lw_hip_tonemap": (i32, [vp, vp, sz, f32, u32, vp]),
     "lw_hip_tonemap_mix": (i32, [vp, vp, sz, f32, u32, u32, f32, vp]),
     "lw_hip_encode": (i32, [vp, vp, sz, vp, vp]),
+    "lw_hip_casts": (i32, [vp, vp, sz, vp, vp]),
     "lw_hip_sky_cache": (i32, [vp, vp, vp, sz, ctypes.POINTER(sz)]),
     "lw_hip_blend_chunky_sky": (i32, [vp, vp, f32, vp, sz]),
     "lw_hip_game_sky_maps": (i32, [vp, vp, vp, f32, vp, vp, sz, u32, vp, sz, vp, sz, vp]),

@@ -316,6 +316,14 @@ class HipBackend:
         self.encode_setup_ms = ms.value
         return out
 
+    def casts(self, x: np.ndarray):
+        """lw_hip_casts: Rust's f32 `as i32` and `as u32` of each value, as the kernels convert."""
+        a = np.ascontiguousarray(x, dtype="<f4").reshape(-1)
+        oi = np.zeros(a.size, dtype="<i4")
+        ou = np.zeros(a.size, dtype="<u4")
+        self._check(_native.lib().lw_hip_casts(self.handle, _addr(a), a.size, _addr(oi), _addr(ou)))
+        return oi, ou
+
     def sky_cache(self, params: np.ndarray) -> np.ndarray:
         """lw_hip_sky_cache: build_sky_cache (integrator.rs:133-148) on the device at the
         resolution params.misc_f[2]."""

@@ -681,6 +681,15 @@ __global__ void __launch_bounds__(kBlock) k_encode(const float *__restrict__ x, 
   if (i < n) out[i] = encode_f(x[i], fix, nfix);
 }
 
+// Rust's f32 `as i32` and `as u32` of n values, as every kernel converts them.
+__global__ void __launch_bounds__(kBlock) k_casts(const float *__restrict__ x, uint32_t n,
+                                                  int32_t *__restrict__ out_i, uint32_t *__restrict__ out_u) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out_i[i] = lw::as_i32(x[i]);
+  out_u[i] = lw::as_u32(x[i]);
+}
+
 // draw_map (sky.rs:864-881) for the game's maps, one lane per texel: the client's sky along
 // the texel's direction, decoded to linear as c^2.2 with libm's powf (pow_libm).
 __global__ void __launch_bounds__(kBlock) k_quick_sky_map(float *__restrict__ out, uint32_t w, uint32_t h,
@@ -3336,6 +3345,27 @@ int lw_hip_encode(lw_hip_backend *b, const float *x, size_t n, float *out, doubl
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(out, res.ptr, n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  });
+}
+
+int lw_hip_casts(lw_hip_backend *b, const float *x, size_t n, int32_t *out_i, uint32_t *out_u) {
+  if (!b || (n && (!x || !out_i || !out_u))) return LW_EINVAL;
+  return guarded(b, [&] {
+    HIP_TRY(hipSetDevice(b->device));
+    if (n > 0xFFFFFFFFull) throw InvalidScene{"too many values"};
+    if (n == 0) return;
+    DevBuf<float> in;
+    DevBuf<int32_t> ri;
+    DevBuf<uint32_t> ru;
+    in.upload(x, n, b->stream);
+    ri.alloc(n);
+    ru.alloc(n);
+    k_casts<<<dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream>>>(in.ptr, (uint32_t)n,
+                                                                                         ri.ptr, ru.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out_i, ri.ptr, n * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpyAsync(out_u, ru.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
   });
 }
