@@ -389,11 +389,13 @@ def bench_plain(args, lw, frames):
     return line
 
 
-def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0) -> dict:
+def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0, bound_subset: bool = True) -> dict:
     """cfg2: the 64 seeded views of the config-1 world rendered as one batch
     (lw_hip_render_hq_batch) and as 64 plain render_hq calls, after one untimed batch; the
     wall time of each, and whether the images are identical (same samples, same summation
-    order, so the two are compared at equal image error)."""
+    order, so the two are compared at equal image error). With bound_subset, the views whose
+    plain call is one persistent launch are also rendered as a batch of their own, which
+    bounds what batching can gain on this view mix (full_gpu_views)."""
     cfg = frames.CONFIGS[1]
     t0 = time.perf_counter()
     scene_buf = frames.build_world(cfg.world)
@@ -412,11 +414,38 @@ def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0) -> dict:
         t_batch = (time.perf_counter() - t) / steps
         tb = b.last_timing()
         log(f"batched: {t_batch:.2f} s per step, {tb}")
+        per_view = np.zeros(len(views))
+        one_launch = np.zeros(len(views), dtype=bool)
         t = time.perf_counter()
         for _ in range(steps):
-            plain_imgs = [b.render_hq_scene(scene, p, s) for p, s in views]
+            plain_imgs = []
+            for i, (p, s) in enumerate(views):
+                tv = time.perf_counter()
+                plain_imgs.append(b.render_hq_scene(scene, p, s))
+                per_view[i] += time.perf_counter() - tv
+                one_launch[i] = b.last_timing()["launches"] == 1
         t_plain = (time.perf_counter() - t) / steps
+        per_view /= steps
         log(f"plain: {t_plain:.2f} s per step")
+        bound = None
+        if bound_subset and one_launch.any():
+            # The views whose plain call is already one persistent launch fill the GPU by
+            # themselves; as a batch of their own they show what batching can still gain on
+            # them. A batch of all 64 views does at least their work, so
+            # t_plain / (their batch's time) bounds batched_over_plain from above, as if every
+            # other view cost nothing.
+            sub = [v for v, big in zip(views, one_launch) if big]
+            b.render_hq_batch(scene, sub)  # warmup
+            t = time.perf_counter()
+            b.render_hq_batch(scene, sub)
+            t_sub = time.perf_counter() - t
+            sub_samples = sum(int(p["frame"][0]) * int(p["frame"][1]) * s for p, s in sub)
+            bound = {"views": len(sub), "samples": sub_samples,
+                     "plain_s": float(per_view[one_launch].sum()), "batched_s": t_sub,
+                     "gain_on_these_views": float(per_view[one_launch].sum()) / t_sub,
+                     "other_views_plain_s": float(per_view[~one_launch].sum()),
+                     "batched_over_plain_bound": t_plain / t_sub}
+            log(f"bound: {bound}")
     finally:
         scene.free()
         b.close()
@@ -428,7 +457,8 @@ def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0) -> dict:
             "plain_msamples_per_s": samples / t_plain / 1e6,
             "batched_s": t_batch, "plain_s": t_plain,
             "batched_over_plain": t_plain / t_batch, "images_identical": same,
-            "batch_passes": tb["launches"], "batch_trace_ms": tb["trace_ms"]}
+            "batch_passes": tb["launches"], "batch_trace_ms": tb["trace_ms"],
+            "full_gpu_views": bound}
 
 
 def batch_roofline(samples, blk):
@@ -462,6 +492,7 @@ def bench_batch(args, lw, frames):
         "batched_over_plain": blk["batched_over_plain"],
         "images_identical": blk["images_identical"],
         "batch_passes": blk["batch_passes"],
+        "full_gpu_views": blk["full_gpu_views"],
         "roofline": batch_roofline(samples, blk),
         "cpu_baseline": None,
     }

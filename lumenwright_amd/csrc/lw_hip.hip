@@ -1258,6 +1258,7 @@ struct lw_hip_scene {
   DevBuf<float> lut;
   DevBuf<uint2> occupancy;
   DevBuf<uint8_t> octant_boxes;
+  DevBuf<uint32_t> mfaces;
   std::vector<uint32_t> quad_flags;  // host copy of each quad's flags (p0.w)
   // The params layout the scene was validated with. A scene built on the device
   // (lw_hip_scene_build) takes it from the first render's params, checked against `built`.
@@ -1269,7 +1270,8 @@ struct lw_hip_scene {
   uint32_t max_emitter_rays = 1;  // emitter face rays one bounce can trace
   DevScene dev() const {
     return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
-                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr, occupancy.ptr, octant_boxes.ptr};
+                    atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr, occupancy.ptr, octant_boxes.ptr,
+                    mfaces.ptr};
   }
 };
 
@@ -1497,6 +1499,32 @@ uint32_t quad_info(const lw_quad &q) {
          (nv ? QI_NEG_V : 0u) | (nn ? QI_NEG_N : 0u);
 }
 
+// Each model's face word (DevScene::mfaces, MF_* in lw_device.hpp) from its quads' axis codes
+// and flags: set for a model whose quads are all FRONT_ONLY axis quads with distinct normals
+// (every full cube), 0 for any other.
+std::vector<uint32_t> model_faces(const lw_model *models, size_t n, const std::vector<uint32_t> &info,
+                                  const std::vector<uint32_t> &flags) {
+  std::vector<uint32_t> out(std::max<size_t>(n, 1), 0u);
+  for (size_t m = 0; m < n; m++) {
+    const lw_model &md = models[m];
+    if (md.quad_count == 0 || md.quad_count > 6) continue;
+    uint32_t fw = 0x3FFFFu;  // every direction MF_NONE
+    bool ok = true;
+    for (uint32_t i = 0; i < md.quad_count && ok; i++) {
+      const uint64_t q = (uint64_t)md.quad_offset + i;
+      if (q >= info.size() || !(info[q] & QI_AXIS) || !(flags[q] & LW_FLAG_FRONT_ONLY)) {
+        ok = false;
+        break;
+      }
+      const uint32_t f = 3u * (2u * (info[q] & 3u) + ((info[q] & QI_NEG_N) ? 1u : 0u));
+      if (((fw >> f) & 7u) != MF_NONE) ok = false;
+      fw = (fw & ~(7u << f)) | (i << f);
+    }
+    if (ok) out[m] = fw | MF_FACES;
+  }
+  return out;
+}
+
 // The sub-block occupancy of every stored brick (DevScene::occupancy), indexed by the
 // brick's top-level entry.
 std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
@@ -1553,6 +1581,8 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.quad_flags.resize(v.quads_len);
   for (size_t q = 0; q < v.quads_len; q++) std::memcpy(&sc.quad_flags[q], &v.quads[q].p0[3], 4);
   sc.qinfo.upload(info.data(), info.size(), b->stream);
+  const std::vector<uint32_t> faces = model_faces(v.models, v.models_len, info, sc.quad_flags);
+  sc.mfaces.upload(faces.data(), faces.size(), b->stream);
   sc.atlas.upload(v.atlas, v.atlas_len, b->stream);
   sc.emitters.upload(v.emitters, v.emitters_len, b->stream);
   sc.egrid.upload(v.emitter_grid, v.emitter_grid_len, b->stream);
@@ -1658,6 +1688,8 @@ void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, l
     std::memcpy(&sc.quad_flags[q], &quads[q].p0[3], 4);
   }
   sc.qinfo.upload(qi.data(), qi.size(), b->stream);
+  const std::vector<uint32_t> faces = model_faces(v.models, v.models_len, qi, sc.quad_flags);
+  sc.mfaces.upload(faces.data(), faces.size(), b->stream);
   sc.atlas.upload(v.atlas, v.atlas_len, b->stream);
   sc.sky.upload(v.sky, v.sky ? v.sky_len : 0, b->stream);
   float lut[256];
