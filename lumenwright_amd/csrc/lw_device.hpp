@@ -63,6 +63,11 @@ struct DevScene {
   const uint8_t *__restrict__ octant_boxes;
   // Per model, its face word (MF_* above), 0 for any other model.
   const uint32_t *__restrict__ mfaces;
+  // Per stored brick (by its top-level entry) and voxel (local index), for each ray octant k
+  // (bits 4k .. 4k + 3, k as DevScene::octant_boxes numbers octants) the side n (at most 8) of
+  // the largest cube of voxels from this one toward octant k, clipped to the brick, that holds
+  // no model; 0 for a voxel with a model. Null when the scene has no table.
+  const uint32_t *__restrict__ cell_ext;
 };
 
 struct Hit {
@@ -315,29 +320,57 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
                           (((uint32_t)cell[0] & 15u) >> 2);
       sb_full = ((sb < 32u ? occ.x : occ.y) >> (sb & 31u)) & 1u;
     }
+    uint32_t model;
+    const uint32_t local =
+        (uint32_t)((((cell[1] & 15) * 16 + (cell[2] & 15)) * 16) + (cell[0] & 15));
+    if (entry == EMPTY) {
+      model = 0;
+    } else if ((entry & UNIFORM) != 0) {
+      model = entry & ~UNIFORM;
+    } else if (sb_full) {
+      const uint32_t w = ld(s.grid, p.off[1] + entry * LW_BRICK_WORDS + (local >> 1));
+      model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
+    } else {
+      model = 0;
+    }
     if (stored && ov == EMPTY) {
-      // An empty 4^3 sub-block (no model, no overlay in any of its cells): cross it in one
-      // move, landing where the cell-by-cell walk below would, with the same t, cells and
-      // step count. The walk is a merge of the three axes' crossing times (each a pure
-      // function of the cell, next_boundary), taking the smallest and, on a tie, the higher
-      // axis. It leaves the sub-block at the first crossing of a far face, ts on axis x; by
-      // then each other axis has made the crossings that come before (ts, x) in that order.
+      // A box of empty cells around the current one, each with no model (and the brick has no
+      // overlay): cross it in one move, landing where the cell-by-cell walk below would, with
+      // the same t, cells and step count. The walk is a merge of the three axes' crossing times
+      // (each a pure function of the cell, next_boundary), taking the smallest and, on a tie,
+      // the higher axis. It leaves the box at the first crossing of a far face, ts on axis x;
+      // by then each other axis has made the crossings that come before (ts, x) in that order.
       // Taken only when that exit stays in the grid, within t_end and within MAX_STEPS;
-      // otherwise the cells are walked one by one.
+      // otherwise the cells are walked one by one. The box is the cell's empty 4^3 sub-block,
+      // or the empty 8^3 block holding it (aligned boxes inside the brick), or, for an empty
+      // cell of an occupied sub-block, the empty cube from the cell toward the ray's octant
+      // (cell_ext), clipped to the brick.
+      int32_t far[3];
+      bool boxed = false;
       if (sb_full == 0u) {
-        // When the 8^3 block holding the sub-block is empty too (its eight sub-blocks are the
-        // occupancy bits 0x330033 shifted by 2 x8 + 8 z8 + 32 y8), it is crossed the same way:
-        // the argument holds for any aligned box of empty cells inside one brick.
+        // The 8^3 block's eight sub-blocks are the occupancy bits 0x330033 shifted by
+        // 2 x8 + 8 z8 + 32 y8.
         const uint32_t x8 = ((uint32_t)cell[0] & 15u) >> 3, y8 = ((uint32_t)cell[1] & 15u) >> 3,
                        z8 = ((uint32_t)cell[2] & 15u) >> 3;
         const int32_t span = ((y8 ? occ.y : occ.x) & (0x330033u << (2u * x8 + 8u * z8))) == 0u ? 8 : 4;
-        int32_t far[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) far[a] = (cell[a] & ~(span - 1)) + (step[a] > 0 ? span - 1 : 0);
+        boxed = true;
+      } else if (model == 0u && s.cell_ext != nullptr) {
+        const int32_t n = (int32_t)((ld(s.cell_ext, entry * 4096u + local) >> (4u * octant)) & 15u);
+        if (n >= 2) {
+#pragma unroll
+          for (int a = 0; a < 3; a++) {
+            const int32_t lo = cell[a] & ~15;
+            far[a] = step[a] > 0 ? min(cell[a] + n - 1, lo + 15) : (step[a] < 0 ? max(cell[a] - n + 1, lo) : cell[a]);
+          }
+          boxed = true;
+        }
+      }
+      if (boxed) {
         float tf[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-          far[a] = (cell[a] & ~(span - 1)) + (step[a] > 0 ? span - 1 : 0);
-          tf[a] = next_boundary(get(o, a), get(inv, a), far[a], step[a]);
-        }
+        for (int a = 0; a < 3; a++) tf[a] = next_boundary(get(o, a), get(inv, a), far[a], step[a]);
         int x;
         if (tf[0] < tf[1]) {
           x = tf[0] < tf[2] ? 0 : 2;
@@ -359,7 +392,7 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
             // The walk crosses axis a's boundaries from c toward far[a] while each comes
             // before (ts, x). Their times never fall from one crossing to the next (f32
             // rounding is monotone), so the crossings made are a prefix, and its length is
-            // found by binary lifting over the at most span - 1 of them, branch-free.
+            // found by binary lifting over the at most 7 of them, branch-free.
             const int32_t avail = (far[a] - c) * step[a];
             int32_t k = 0;
 #pragma unroll
@@ -391,20 +424,6 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
     const float t_cell_exit = f_min(next[0], f_min(next[1], next[2]));
     const float t_lo = f_max(t - 1e-4f, tmin);
     const float t_hi = t_cell_exit + 1e-4f;
-
-    uint32_t model;
-    const uint32_t local =
-        (uint32_t)((((cell[1] & 15) * 16 + (cell[2] & 15)) * 16) + (cell[0] & 15));
-    if (entry == EMPTY) {
-      model = 0;
-    } else if ((entry & UNIFORM) != 0) {
-      model = entry & ~UNIFORM;
-    } else if (sb_full) {
-      const uint32_t w = ld(s.grid, p.off[1] + entry * LW_BRICK_WORDS + (local >> 1));
-      model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
-    } else {
-      model = 0;
-    }
     Hit cand{f_min(t_hi, best_t), UINT32_MAX, 0u, 0.0f, 0.0f};
     if (model != 0) {
       const lw_model m = ld(s.models, model);

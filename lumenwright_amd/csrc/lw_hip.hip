@@ -755,6 +755,53 @@ __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *__restrict__ p, u
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// DevScene::cell_ext, one workgroup per stored brick. For each ray octant k (sign sx, sy, sz
+// per axis), the side of the largest model-free cube from a voxel toward k, clipped to the
+// brick, follows n(v) = 0 for a voxel with a model, else min(8, 1 + min n(v + forward
+// neighbour)) over the (up to 7) forward neighbours inside the brick (8 where there are none).
+// In coordinates mirrored so that k points toward -1 on every axis, a voxel's forward
+// neighbours lie on the planes x' + y' + z' below its own, so the planes are filled in order,
+// one lane per (x, z) column, with a barrier between planes.
+__global__ void __launch_bounds__(kBlock) k_cell_ext(const uint32_t *__restrict__ grid, uint64_t off_bricks,
+                                                    uint32_t *__restrict__ ext) {
+  __shared__ uint8_t n[4096];
+  __shared__ uint32_t acc[4096];
+  const uint32_t *words = grid + off_bricks + (uint64_t)blockIdx.x * LW_BRICK_WORDS;
+  const int32_t cx = (int32_t)(threadIdx.x & 15u), cz = (int32_t)(threadIdx.x >> 4);
+  uint32_t full = 0;  // bit y: voxel (cx, y, cz) has a model
+  for (int32_t y = 0; y < 16; y++) {
+    const uint32_t local = (uint32_t)((y * 16 + cz) * 16 + cx);
+    if ((words[local >> 1] >> ((local & 1u) * 16u)) & 0xFFFFu) full |= 1u << y;
+    acc[local] = 0u;
+  }
+  for (uint32_t k = 0; k < 8; k++) {
+    const int32_t sx = (k & 1u) ? 1 : -1, sy = (k & 2u) ? 1 : -1, sz = (k & 4u) ? 1 : -1;
+    const int32_t mx = sx > 0 ? 15 - cx : cx, mz = sz > 0 ? 15 - cz : cz;
+    for (int32_t d = 0; d <= 45; d++) {
+      const int32_t my = d - mx - mz;
+      if (my >= 0 && my <= 15) {
+        const int32_t y = sy > 0 ? 15 - my : my;
+        const uint32_t local = (uint32_t)((y * 16 + cz) * 16 + cx);
+        uint32_t m = 0;
+        if (!((full >> y) & 1u)) {
+          m = 8;
+          for (int32_t c = 1; c < 8; c++) {
+            const int32_t nx = cx + ((c & 1) ? sx : 0), ny = y + ((c & 2) ? sy : 0), nz = cz + ((c & 4) ? sz : 0);
+            if (nx >= 0 && nx <= 15 && ny >= 0 && ny <= 15 && nz >= 0 && nz <= 15)
+              m = min(m, (uint32_t)n[(ny * 16 + nz) * 16 + nx]);
+          }
+          m = min(m + 1u, 8u);
+        }
+        n[local] = (uint8_t)m;
+        acc[local] |= m << (4u * k);
+      }
+      __syncthreads();
+    }
+  }
+  uint32_t *out = ext + (uint64_t)blockIdx.x * 4096u;
+  for (uint32_t i = threadIdx.x; i < 4096u; i += blockDim.x) out[i] = acc[i];
+}
+
 // DevScene::octant_boxes, in three kernels: k_octants_init marks each brick that is not
 // EMPTY (a top-level or overlay entry) with all eight bits; k_octants_scan then ORs the bits
 // along every line of bricks on one axis, toward +axis for the octants whose bit for that
@@ -1259,6 +1306,7 @@ struct lw_hip_scene {
   DevBuf<uint2> occupancy;
   DevBuf<uint8_t> octant_boxes;
   DevBuf<uint32_t> mfaces;
+  DevBuf<uint32_t> cell_ext;
   std::vector<uint32_t> quad_flags;  // host copy of each quad's flags (p0.w)
   // The params layout the scene was validated with. A scene built on the device
   // (lw_hip_scene_build) takes it from the first render's params, checked against `built`.
@@ -1271,7 +1319,7 @@ struct lw_hip_scene {
   DevScene dev() const {
     return DevScene{grid.ptr, models.ptr, reinterpret_cast<const float4 *>(quads.ptr), qinfo.ptr,
                     atlas.ptr, emitters.ptr, egrid.ptr, sky.ptr, lut.ptr, occupancy.ptr, octant_boxes.ptr,
-                    mfaces.ptr};
+                    mfaces.ptr, cell_ext.ptr};
   }
 };
 
@@ -1525,9 +1573,8 @@ std::vector<uint32_t> model_faces(const lw_model *models, size_t n, const std::v
   return out;
 }
 
-// The sub-block occupancy of every stored brick (DevScene::occupancy), indexed by the
-// brick's top-level entry.
-std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
+// The number of stored bricks a view's top-level entries name (one past the largest entry).
+uint64_t stored_bricks(const lw_scene_view &v) {
   const lw_params &p = *v.params;
   const uint64_t nb = (uint64_t)p.grid_dims[0] * p.grid_dims[1] * p.grid_dims[2];
   uint64_t stored = 0;
@@ -1535,6 +1582,14 @@ std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
     const uint32_t e = v.grid[p.off[0] + b];
     if (e != LW_EMPTY && (e & LW_UNIFORM) == 0) stored = std::max<uint64_t>(stored, (uint64_t)e + 1);
   }
+  return stored;
+}
+
+// The sub-block occupancy of every stored brick (DevScene::occupancy), indexed by the
+// brick's top-level entry.
+std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
+  const lw_params &p = *v.params;
+  const uint64_t stored = stored_bricks(v);
   std::vector<uint2> occ(std::max<uint64_t>(stored, 1), make_uint2(0u, 0u));
   for (uint64_t e = 0; e < stored; e++) {
     const uint32_t *words = v.grid + p.off[1] + e * LW_BRICK_WORDS;
@@ -1549,6 +1604,16 @@ std::vector<uint2> occupancy_masks(const lw_scene_view &v) {
     occ[e] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
   }
   return occ;
+}
+
+// DevScene::cell_ext of a scene whose grid is on the device, for its `stored` bricks; left null
+// (the walk then crosses such cells one by one) when the table would not fit 32-bit offsets.
+void build_cell_ext(lw_hip_backend *b, lw_hip_scene &sc, uint64_t off_bricks, uint64_t stored) {
+  sc.cell_ext.free();
+  if (stored == 0 || stored * 4096u * sizeof(uint32_t) >= (1ull << 32)) return;
+  sc.cell_ext.alloc(stored * 4096u);
+  k_cell_ext<<<dim3((unsigned)stored), dim3(kBlock), 0, b->stream>>>(sc.grid.ptr, off_bricks, sc.cell_ext.ptr);
+  HIP_TRY(hipGetLastError());
 }
 
 // DevScene::octant_boxes of a scene whose grid is on the device.
@@ -1592,6 +1657,7 @@ void upload_scene(lw_hip_backend *b, const lw_scene_view &v, lw_hip_scene &sc) {
   sc.lut.upload(lut, 256, b->stream);
   const std::vector<uint2> occ = occupancy_masks(v);
   sc.occupancy.upload(occ.data(), occ.size(), b->stream);
+  build_cell_ext(b, sc, v.params->off[1], stored_bricks(v));
   HIP_TRY(hipStreamSynchronize(b->stream));  // the host vectors above die with this scope
   build_octant_boxes(b, sc, v.params->grid_dims, v.params->off[0], v.params->off[2]);
   sc.key = layout_key(*v.params);
@@ -1720,6 +1786,8 @@ void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, l
                          v.sky ? v.sky_len : 0))
     throw Unsupported{"a scene buffer of 4 GiB or more"};
   build_octant_boxes(b, sc, v.dims, o.offsets[0], o.offsets[2]);
+  build_cell_ext(b, sc, o.offsets[1], o.bricks);
+  HIP_TRY(hipStreamSynchronize(b->stream));
   uint64_t max_faces = 0;
   for (size_t m = 0; m < v.models_len; m++)
     if (v.model_flags[m] & LW_MODEL_EMITS) max_faces = std::max<uint64_t>(max_faces, v.models[m].quad_count);
