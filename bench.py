@@ -168,7 +168,9 @@ def mrays(config, msamples):
 
 def kernel_form():
     form = os.environ.get("LW_KERNEL", "direct")
-    form += f", waves {os.environ.get('LW_WAVES', '6')}"
+    w = os.environ.get("LW_WAVES")
+    form += (f", persistent tiles at {os.environ.get('LW_TILES_WAVES', w or '5')} waves per SIMD, "
+             f"launch ramp at {w or '6'}")
     if os.environ.get("LW_XCD_REMAP", "0") != "0":
         form += ", xcd remap"
     return form

@@ -1356,6 +1356,11 @@ struct lw_hip_backend {
   // against 528 at 5, 516 at 7 and 412 at 8. The persistent form, which has no 6-wave variant, then
   // runs at its own best, 4.
   int waves = 6, trace_waves = 1;
+  // LW_TILES_WAVES (or LW_WAVES): the persistent tile kernel's budget. With the model-free cube
+  // jump it measured fastest at 5 waves per SIMD: 922 Msamples/s at the 720p 256-spp frame
+  // against 916 at 4, 882 at 6 and 650 at 8 (the launch ramp's k_render stays fastest at 6: 851
+  // against 815 at 5).
+  int tiles_waves = 5;
   // LW_XCD_REMAP=1: the direct kernel gives each XCD a contiguous run of tiles.
   uint32_t xcd_remap = 0;
   // LW_SPL: samples per lane of the direct form (k_render_multi when above 1). Measured
@@ -1949,7 +1954,7 @@ void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_para
   const DevScene ds = sc.dev();
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (n_items + 3) / 4));
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-  LW_DIRECT_DISPATCH(b->waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+  LW_DIRECT_DISPATCH(b->tiles_waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
                                    p, ds, b->batch_dev.ptr, nullptr, n_items, b->counter.ptr, b->out.ptr,
                                    progress ? b->progress_dev : nullptr)));
   HIP_TRY(hipGetLastError());
@@ -2291,7 +2296,7 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
     b->batch_items.upload(list.data(), list.size(), b->stream);
     k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (uint32_t)((list.size() + 3) / 4)));
-    LW_DIRECT_DISPATCH(b->waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+    LW_DIRECT_DISPATCH(b->tiles_waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
                                      p0, ds, b->batch_dev.ptr, b->batch_items.ptr, (uint32_t)list.size(),
                                      b->counter.ptr, b->out.ptr, nullptr)));
     HIP_TRY(hipGetLastError());
@@ -2708,7 +2713,8 @@ int lw_hip_open(int device, lw_hip_backend **out) {
       }
       b->scratch = g_scratch_ptr[device];
     }
-    if (const char *e = std::getenv("LW_WAVES")) b->waves = std::atoi(e);
+    if (const char *e = std::getenv("LW_WAVES")) b->waves = b->tiles_waves = std::atoi(e);
+    if (const char *e = std::getenv("LW_TILES_WAVES")) b->tiles_waves = std::atoi(e);
     if (const char *e = std::getenv("LW_TRACE_WAVES")) b->trace_waves = std::atoi(e);
     int per_cu_frame = 0, per_cu_list = 0;
     LW_WAVES_DISPATCH(b->waves, {
@@ -2744,7 +2750,7 @@ int lw_hip_open(int device, lw_hip_backend **out) {
     if (const char *e = std::getenv("LW_BATCH_FORM")) b->batch_passes_form = std::strcmp(e, "passes") == 0;
     {
       int per_cu = 0;
-      LW_DIRECT_DISPATCH(b->waves, {
+      LW_DIRECT_DISPATCH(b->tiles_waves, {
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(&k_render_tiles<W>), kBlock, 0));
       });
