@@ -359,7 +359,12 @@ int lw_hip_scene_build(lw_hip_backend *b, const lw_voxel_scene *vs, lw_scene_inf
  * LW_SCENE_EMITTER_GRID (uint32), LW_SCENE_EMITTERS (lw_emitter), LW_SCENE_QUADS (lw_quad),
  * LW_SCENE_OCTANT_BOXES (uint8 per brick, numbered (y * dz + z) * dx + x: bit k, for
  * k = [+x] | [+y] << 1 | [+z] << 2, is set when the box from the brick to the grid's end
- * toward octant k holds a brick whose top-level or overlay entry is not EMPTY).
+ * toward octant k holds a brick whose top-level or overlay entry is not EMPTY),
+ * LW_SCENE_CELL_CUBES (uint32 per voxel of each stored brick, 4096 per brick in the order of
+ * the brick words, bricks by their top-level entry: bits 4k .. 4k + 3, k numbered as above,
+ * hold the side n <= 8 of the largest cube of voxels from this one toward octant k, clipped to
+ * the brick, in which no voxel has a model; 0 for a voxel with a model; empty when the scene
+ * has no table).
  * *len receives the element count; with dst null only *len is written, otherwise cap (in
  * elements) must hold the buffer. No counterpart in the reference. */
 #define LW_SCENE_GRID 0u
@@ -367,6 +372,7 @@ int lw_hip_scene_build(lw_hip_backend *b, const lw_voxel_scene *vs, lw_scene_inf
 #define LW_SCENE_EMITTER_GRID 2u
 #define LW_SCENE_QUADS 3u
 #define LW_SCENE_OCTANT_BOXES 4u
+#define LW_SCENE_CELL_CUBES 5u
 int lw_hip_scene_read(lw_hip_backend *b, const lw_hip_scene *scene, uint32_t which, void *dst,
                       size_t cap, size_t *len);
 

@@ -158,7 +158,8 @@ def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
 
 
 MODEL_OCCLUDES, MODEL_EMITS = 1, 2  # lw_voxel_scene.model_flags
-SCENE_GRID, SCENE_EMITTERS, SCENE_EMITTER_GRID, SCENE_QUADS, SCENE_OCTANT_BOXES = 0, 1, 2, 3, 4  # lw_hip_scene_read
+SCENE_GRID, SCENE_EMITTERS, SCENE_EMITTER_GRID, SCENE_QUADS, SCENE_OCTANT_BOXES, SCENE_CELL_CUBES = (
+    0, 1, 2, 3, 4, 5)  # lw_hip_scene_read
 
 
 class VoxelSceneC(ctypes.Structure):

@@ -14,8 +14,8 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import _native
-from .abi import (EMITTER, FORMS, HIT, MODEL, PARAMS, QUAD, SCENE_EMITTER_GRID, SCENE_EMITTERS,
-                  SCENE_GRID, SCENE_OCTANT_BOXES, SCENE_QUADS, VIEW, BoundView, BuildStatsC, FitOptionsC, FitStatsC,
+from .abi import (EMITTER, FORMS, HIT, MODEL, PARAMS, QUAD, SCENE_CELL_CUBES, SCENE_EMITTER_GRID,
+                  SCENE_EMITTERS, SCENE_GRID, SCENE_OCTANT_BOXES, SCENE_QUADS, VIEW, BoundView, BuildStatsC, FitOptionsC, FitStatsC,
                   GameSkyC, HqFrame, HqScene, QuickFrame, TimingC, TuneResultC, VoxelVolume)
 from .frames import SceneInfoC
 
@@ -142,7 +142,7 @@ class HipBackend:
         n = ctypes.c_size_t()
         self._check(L.lw_hip_scene_read(self.handle, scene.handle, which, None, 0, ctypes.byref(n)))
         dtype = {SCENE_GRID: "<u4", SCENE_EMITTER_GRID: "<u4", SCENE_EMITTERS: EMITTER,
-                 SCENE_QUADS: QUAD, SCENE_OCTANT_BOXES: "u1"}[which]
+                 SCENE_QUADS: QUAD, SCENE_OCTANT_BOXES: "u1", SCENE_CELL_CUBES: "<u4"}[which]
         out = np.zeros(n.value, dtype=dtype)
         self._check(L.lw_hip_scene_read(self.handle, scene.handle, which, _addr(out), out.size,
                                         ctypes.byref(n)))

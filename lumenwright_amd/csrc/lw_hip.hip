@@ -2925,6 +2925,7 @@ int lw_hip_scene_read(lw_hip_backend *b, const lw_hip_scene *scene, uint32_t whi
       case LW_SCENE_EMITTER_GRID: src = scene->egrid.ptr; n = scene->egrid.n; size = 4; break;
       case LW_SCENE_QUADS: src = scene->quads.ptr; n = scene->quads.n; size = sizeof(lw_quad); break;
       case LW_SCENE_OCTANT_BOXES: src = scene->octant_boxes.ptr; n = scene->octant_boxes.n; size = 1; break;
+      case LW_SCENE_CELL_CUBES: src = scene->cell_ext.ptr; n = scene->cell_ext.n; size = 4; break;
       default: throw InvalidScene{"unknown scene buffer"};
     }
     *len = n;

@@ -9,6 +9,7 @@ Bars (the kernel crate's own outputs, restated by the oracle):
 """
 
 import os
+import types
 
 import numpy as np
 import pytest
@@ -164,6 +165,117 @@ def test_the_renderers_walk_stops_when_its_octant_is_empty(hip_backend, oracle):
             assert np.count_nonzero(hit & (got["cell"][:, 1] == 69)) > 100  # the lone cube
         else:
             assert not got["hit"].any()
+
+
+def sparse_volume(rng, dims, density):
+    """Voxels sprinkled at random, so that most 4^3 sub-blocks hold a model and most of their
+    voxels are empty (the walk crosses those by the model-free cube jump): model 1 a cube of
+    FRONT_ONLY opaque axis quads, model 2 the same cube with the right half of each face cut
+    out, model 3 two crossed cutout quads (not axis quads); brick 0 left empty and the last
+    brick full."""
+    from lumenwright_amd.abi import (ALPHA_CUTOUT, ALPHA_OPAQUE, FLAG_FRONT_ONLY, MODEL,
+                                     MODEL_OCCLUDES, QUAD, VoxelVolume)
+    nb = int(np.prod(dims))
+    vox = np.zeros(nb * 4096, dtype="<u2")
+    pick = rng.random(nb * 4096) < density
+    vox[pick] = rng.choice(np.array([1, 1, 1, 2, 3], "<u2"), size=int(pick.sum()))
+    vox[:4096] = 0
+    vox[4096 * (nb - 1):] = 1
+    quads = (scenes.cube(ALPHA_OPAQUE | FLAG_FRONT_ONLY) + scenes.cube(ALPHA_CUTOUT | FLAG_FRONT_ONLY)
+             + [scenes.quad((0, 1, 0), (1, 0, 1), (0, -1, 0), ALPHA_CUTOUT),
+                scenes.quad((1, 1, 0), (-1, 0, 1), (0, -1, 0), ALPHA_CUTOUT)])
+    models = [scenes.model(0, 0), scenes.model(0, 6), scenes.model(6, 6), scenes.model(12, 2)]
+    return VoxelVolume(dims=dims, voxels=vox, models=np.array(models, dtype=MODEL),
+                       model_flags=np.array([0, MODEL_OCCLUDES, 0, 0], np.uint8),
+                       quads=np.array(quads, dtype=QUAD), atlas=np.array([0xFFFFFFFF, 0], "<u4"),
+                       atlas_size=(2, 1))
+
+
+@pytest.mark.parametrize("density", [0.01, 0.04])
+def test_trace_crosses_empty_voxels_of_occupied_sub_blocks_exactly(hip_backend, oracle, density):
+    # The walk crosses an empty voxel of an occupied 4^3 sub-block by the largest model-free
+    # cube from it toward the ray's octant (clipped to its brick) in one move; the hit, the
+    # texel, the t bits and the final cell must be the cell-by-cell walk's. Rays start inside
+    # and outside the grid, a fifth of them along a plane of two axes (a zero step), and a
+    # third stop at a finite tmax inside the grid.
+    from test_scene_build import view_of
+    rng = np.random.default_rng(int(density * 1000))
+    dims = (3, 2, 3)
+    vol = sparse_volume(rng, dims, density)
+    sub = vol.voxels.reshape(-1, 4, 4, 4, 4, 4, 4).transpose(0, 1, 3, 5, 2, 4, 6).reshape(-1, 64)[:-64]
+    occupied = (sub != 0).any(axis=1)
+    # the case under test dominates: most sub-blocks hold a model, most of their voxels are empty
+    assert occupied.mean() > 0.4 and (sub[occupied] == 0).mean() > 0.9
+    v = view_of(vol, oracle.build_layout(vol))
+    size = np.array(dims, np.float64) * 16
+    n = 30000
+    o = (rng.random((n, 3)) * size * 1.2 - size * 0.1).astype("<f4")
+    d = rng.normal(size=(n, 3))
+    d[: n // 5, rng.integers(0, 3)] = 0.0
+    d = rays._unit(d)
+    tmax = np.where(rng.random(n) < 1 / 3, rng.uniform(0.0, 40.0, n), np.inf).astype("<f4")
+    want = oracle.trace(v, o, d, 0.0, tmax)
+    got = hip_backend.trace(v, o, d, 0.0, tmax)
+    ok = trace_fields_equal(got, want)
+    print(f"density {density}: {n} rays, {int(want['hit'].sum())} hits, {int((~ok).sum())} differ")
+    assert ok.all(), (np.nonzero(~ok)[0][:10], got[~ok][:3], want[~ok][:3])
+    assert 0.1 * n < int(want["hit"].sum()) < 0.95 * n
+
+
+def cell_cubes_brute_force(grid, off_bricks, stored):
+    """For each stored brick, voxel and octant k (sign +1 on axis a where bit a of k is set),
+    the largest n <= 8 such that the cube of n^3 voxels from the voxel toward k, clipped to the
+    brick, holds no voxel with a model, by counting the models in every such cube (3-d prefix
+    sums over the brick mirrored so that k points toward +1 on every axis)."""
+    words = grid[off_bricks:off_bricks + stored * 2048].reshape(stored, 2048)
+    ids = np.stack([words & 0xFFFF, words >> 16], axis=-1).reshape(stored, 16, 16, 16)  # y, z, x
+    full = (ids != 0).astype(np.int64)
+    out = np.zeros((stored, 16, 16, 16), np.uint64)
+    for k in range(8):
+        # mirror the axes (x, z, y of (e, y, z, x)) on which octant k points toward -1
+        flip = [a for a, bit in ((3, 1), (2, 4), (1, 2)) if not k & bit]
+        f = np.flip(full, axis=flip) if flip else full
+        ps = np.zeros((stored, 17, 17, 17), np.int64)
+        ps[:, 1:, 1:, 1:] = f.cumsum(1).cumsum(2).cumsum(3)
+        i = np.arange(16)
+        side = np.zeros(f.shape, np.int64)
+        for n in range(1, 9):
+            hi = np.minimum(i + n, 16)
+            y0, z0, x0 = np.ix_(i, i, i)
+            y1, z1, x1 = np.ix_(hi, hi, hi)
+            cnt = (ps[:, y1, z1, x1] - ps[:, y0, z1, x1] - ps[:, y1, z0, x1] - ps[:, y1, z1, x0]
+                   + ps[:, y0, z0, x1] + ps[:, y0, z1, x0] + ps[:, y1, z0, x0] - ps[:, y0, z0, x0])
+            side += (cnt == 0)
+        side = np.flip(side, axis=flip) if flip else side
+        out |= side.astype(np.uint64) << np.uint64(4 * k)
+    return out.astype("<u4").reshape(-1)
+
+
+def test_the_cell_cube_table_is_the_largest_model_free_cube(hip_backend, oracle, frame0):
+    # k_cell_ext against the table's definition, on an uploaded config world and on the sparse
+    # worlds both uploaded and built on the device.
+    from lumenwright_amd.abi import SCENE_CELL_CUBES
+    from test_scene_build import view_of
+    cases = [(frame0.view(), frame0.scene.grid, None)]
+    for density in (0.01, 0.04):
+        vol = sparse_volume(np.random.default_rng(7), (3, 2, 3), density)
+        built = oracle.build_layout(vol)
+        cases.append((view_of(vol, built), built[0], vol))
+    for v, grid, vol in cases:
+        off = int(v.params["off"][1])
+        top = grid[int(v.params["off"][0]):int(v.params["off"][0]) + int(v.params["grid_dims"][3])]
+        stored = int(max([e + 1 for e in top if e != 0xFFFFFFFF and not (e & 0x80000000)], default=0))
+        want = cell_cubes_brute_force(np.asarray(grid, "<u4"), off, stored)
+        made = [hip_backend.upload(types.SimpleNamespace(view=lambda v=v: v))]
+        if vol is not None:
+            made.append(hip_backend.build_scene(vol)[0])
+        for s in made:
+            try:
+                got = hip_backend.scene_read(s, SCENE_CELL_CUBES)
+            finally:
+                s.free()
+            assert got.size == want.size and np.array_equal(got, want), np.flatnonzero(got != want)[:8]
+        print(f"{stored} stored bricks: table equal")
 
 
 def sample_set(params, n, spp, seed):
