@@ -139,20 +139,23 @@ def roofline(config, samples, trace_ms, launches, steps=1):
     per_launch = samples / launches
     mean_s = trace_ms / 1e3 / launches
     achieved = b_alg["bytes_per_sample"] * per_launch / mean_s / 1e9
-    persistent = launches == steps and kernel_form() == "direct, waves 6"
+    # one launch per step: the persistent tile kernel (the default form, not the launch ramp)
+    persistent = (launches == steps and os.environ.get("LW_KERNEL", "direct") == "direct"
+                  and os.environ.get("LW_FRAME_FORM") != "launches")
     traffic = None
     traffic_path = ROOT / "profiles" / "traffic.json"
     if traffic_path.exists() and persistent:
         t = json.loads(traffic_path.read_text()).get(f"config{config}")
-        if t:
+        waves = os.environ.get("LW_TILES_WAVES", os.environ.get("LW_WAVES", "5"))
+        if t and str(t.get("tiles_waves")) == waves:
             traffic = t["hbm_bytes_per_sample"] * per_launch
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": ("k_render_tiles (one persistent launch per frame)" if persistent else
                        "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)"),
             "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"],
-            "note": "VALU-issue bound: the VALU issues every cycle at 13% lane utilisation, L2 hit "
-                    "86%; HBM traffic 813 B/sample, mostly register spill "
+            "note": "VALU-issue bound: the VALU issues on 93-95% of cycles at 14.7% lane "
+                    "utilisation, L2 hit 94%; HBM traffic 375 B/sample, mostly register spill "
                     "(profiles/k_render_tiles_cfg1_720p_*.md)" if config == 1 else None}
 
 
