@@ -314,6 +314,8 @@ def bench_plain(args, lw, frames):
         bt = tuned["best"]
         form = f"{bt['form']}, waves {bt['waves']}" + (
             f", {bt['per_lane']} samples per lane" if bt["per_lane"] > 1 else "")
+        if "best_tiles" in tuned:
+            form += f"; persistent tiles at {tuned['best_tiles']['waves']} waves per SIMD"
         log(f"autotune: {tuned}")
     for i in range(args.warmup):
         b.render_hq_scene(scene, frame.params, spp)

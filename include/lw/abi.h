@@ -569,6 +569,7 @@ int lw_hip_game_sky_maps(lw_hip_backend *b, const struct lw_quick_sky *lens,
 #define LW_FORM_PERSISTENT 0u
 #define LW_FORM_DIRECT 1u
 #define LW_FORM_WAVEFRONT 2u
+#define LW_FORM_TILES 3u  /* the persistent tile kernel of frames with >= 16 spp */
 
 typedef struct lw_tune_entry {
   uint32_t form;      /* LW_FORM_* */
@@ -580,15 +581,21 @@ typedef struct lw_tune_entry {
 
 typedef struct lw_tune_result {
   lw_tune_entry entries[16];
-  uint32_t n;         /* candidates timed */
-  uint32_t best;      /* index of the fastest, now the handle's setting */
-  uint32_t pad[2];
+  uint32_t n;          /* candidates timed */
+  uint32_t best;       /* index of the fastest launch shape, now the handle's setting */
+  uint32_t best_tiles; /* index of the fastest LW_FORM_TILES budget, now the handle's
+                          setting; UINT32_MAX when the frame does not use the tile kernel */
+  uint32_t pad;
 } lw_tune_result;
 
 /* Times a probe of params' frame (its first probe_spp samples per pixel, one launch) with
  * each launch shape the library has -- direct form at 4, 5, 6 and 8 waves per SIMD and 1 or
  * 4 samples per lane, persistent form at 3 and 4, wavefront form -- and keeps the fastest
- * for later renders and batches on this handle. Every shape gives the same images. */
+ * for later renders and batches on this handle. When the frame is large enough for the
+ * persistent tile kernel (direct-form frames of at least 16 spp with 4 tiles for every
+ * resident wave), it also times that kernel on the frame's first 16 samples per pixel at 4, 5
+ * and 6 waves per SIMD (LW_FORM_TILES entries) and keeps the fastest budget for it.
+ * Every shape gives the same images. */
 int lw_hip_autotune(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
                     uint32_t probe_spp, lw_tune_result *out);
 

@@ -136,10 +136,10 @@ class TuneResultC(ctypes.Structure):
     """lw_tune_result."""
 
     _fields_ = [("entries", TuneEntryC * 16), ("n", ctypes.c_uint32), ("best", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32 * 2)]
+                ("best_tiles", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
-FORMS = {0: "persistent", 1: "direct", 2: "wavefront"}
+FORMS = {0: "persistent", 1: "direct", 2: "wavefront", 3: "tiles"}
 
 
 class TimingC(ctypes.Structure):

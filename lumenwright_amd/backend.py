@@ -287,7 +287,10 @@ class HipBackend:
                                                   ctypes.byref(r)))
         rows = [{"form": FORMS[e.form], "waves": e.waves, "per_lane": e.per_lane, "ms": e.ms}
                 for e in r.entries[:r.n]]
-        return {"candidates": rows, "best": rows[r.best]}
+        out = {"candidates": rows, "best": rows[r.best]}
+        if r.best_tiles != 0xFFFFFFFF:
+            out["best_tiles"] = rows[r.best_tiles]
+        return out
 
     def tonemap(self, hdr: np.ndarray, exposure: float = 1.0, op: int = 0) -> np.ndarray:
         """lw_hip_tonemap (render/post.rs:148-170): (n, 3) linear to n packed RGBA8."""

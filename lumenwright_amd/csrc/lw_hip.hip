@@ -1919,6 +1919,26 @@ constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
 constexpr uint32_t kTilesMinSpp = 16;
 constexpr uint32_t kTilesPerWave = 4;
 
+// The tile kernel's budget and the resident workgroups it allows (its launch's grid).
+void set_tiles_waves(lw_hip_backend *b, int waves) {
+  b->tiles_waves = waves;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, b->device));
+  int per_cu = 0;
+  LW_DIRECT_DISPATCH(b->tiles_waves, {
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(&k_render_tiles<W>), kBlock, 0));
+  });
+  b->blocks_tiles = (uint32_t)std::max(1, prop.multiProcessorCount) * (uint32_t)std::max(1, per_cu);
+}
+
+// Whether a frame renders in one persistent tile launch (render_frame's rule).
+bool uses_tiles(const lw_hip_backend *b, uint32_t w, uint32_t h, uint32_t spp) {
+  const uint64_t tile_items = (uint64_t)((w + 3) / 4) * ((h + 3) / 4);
+  return b->mode == MODE_DIRECT && b->per_lane == 1 && !b->frame_launches && spp >= kTilesMinSpp &&
+         tile_items >= (uint64_t)kTilesPerWave * b->blocks_tiles * (kBlock / 64);
+}
+
 // A frame in one persistent launch (k_render_tiles over its 4 x 4 tiles, all samples of a tile
 // in one wave, summed in sample order in f64): the image equals the launch ramp's bit for bit. With
 // `progress`, the host reports (samples done, spp) while the launch runs, from the item index
@@ -2035,10 +2055,7 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   // The persistent form needs at least kTilesPerWave items for each resident wave, so that the
   // last items to start are a small part of the launch; smaller frames fill the device
   // through the launch ramp's many samples per launch.
-  const uint64_t tile_items = (uint64_t)((w + 3) / 4) * ((h + 3) / 4);
-  if (!diff && !out_f64 && !one_launch && !fixed && b->mode == MODE_DIRECT && b->per_lane == 1 &&
-      !b->frame_launches && spp >= kTilesMinSpp &&
-      tile_items >= (uint64_t)kTilesPerWave * b->blocks_tiles * (kBlock / 64)) {
+  if (!diff && !out_f64 && !one_launch && !fixed && uses_tiles(b, w, h, spp)) {
     render_frame_tiles(b, sc, p, spp, progress, user, out_rgb, t_wall);
     return;
   }
@@ -2748,14 +2765,7 @@ int lw_hip_open(int device, lw_hip_backend **out) {
     if (const char *e = std::getenv("LW_BATCH_GRAPHS")) b->use_graphs = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("LW_BATCH_PASSES")) b->batch_passes = (uint32_t)std::max(0, std::atoi(e));
     if (const char *e = std::getenv("LW_BATCH_FORM")) b->batch_passes_form = std::strcmp(e, "passes") == 0;
-    {
-      int per_cu = 0;
-      LW_DIRECT_DISPATCH(b->tiles_waves, {
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_render_tiles<W>), kBlock, 0));
-      });
-      b->blocks_tiles = cus * (uint32_t)std::max(1, per_cu);
-    }
+    set_tiles_waves(b, b->tiles_waves);
   });
   if (rc != LW_OK) {
     lw_hip_close(b);
@@ -3186,7 +3196,7 @@ void lw_hip_fit_free(lw_hip_backend *b, lw_hip_fit *f) {
 int lw_hip_autotune(lw_hip_backend *b, const lw_hip_scene *scene, const lw_params *params,
                     uint32_t probe_spp, lw_tune_result *out) {
   if (!b || !scene || !params || !out) return LW_EINVAL;
-  const int mode0 = b->mode, waves0 = b->waves;
+  const int mode0 = b->mode, waves0 = b->waves, tiles0 = b->tiles_waves;
   const uint32_t lane0 = b->per_lane;
   const int rc = guarded(b, [&] {
     struct Cand {
@@ -3222,11 +3232,42 @@ int lw_hip_autotune(lw_hip_backend *b, const lw_hip_scene *scene, const lw_param
     b->mode = cands[out->best].mode;
     b->waves = cands[out->best].waves;
     b->per_lane = cands[out->best].per_lane;
+    // The tile kernel's budget, on the frame's first kTilesMinSpp samples per pixel.
+    out->best_tiles = UINT32_MAX;
+    static const int tile_waves[] = {4, 5, 6};
+    double best_tiles = 1e300;
+    p.frame[2] = 0;
+    // (timed whatever launch shape won: the tile kernel serves direct-form frames)
+    const int mode_won = b->mode;
+    const uint32_t lane_won = b->per_lane;
+    b->mode = MODE_DIRECT;
+    b->per_lane = 1;
+    for (int tw : tile_waves) {
+      set_tiles_waves(b, tw);
+      if (!uses_tiles(b, p.frame[0], p.frame[1], kTilesMinSpp)) continue;
+      const auto t0 = std::chrono::steady_clock::now();
+      render_frame_tiles(b, *scene, p, kTilesMinSpp, nullptr, nullptr, img.data(), t0);  // warm
+      render_frame_tiles(b, *scene, p, kTilesMinSpp, nullptr, nullptr, img.data(), t0);
+      const double ms = b->timing.trace_ms;
+      out->entries[out->n] = lw_tune_entry{LW_FORM_TILES, (uint32_t)tw, 1u, 0u, ms};
+      if (ms < best_tiles) {
+        best_tiles = ms;
+        out->best_tiles = out->n;
+      }
+      out->n++;
+    }
+    b->mode = mode_won;
+    b->per_lane = lane_won;
+    set_tiles_waves(b, out->best_tiles != UINT32_MAX ? (int)out->entries[out->best_tiles].waves : tiles0);
   });
   if (rc != LW_OK) {
     b->mode = mode0;
     b->waves = waves0;
     b->per_lane = lane0;
+    try {
+      set_tiles_waves(b, tiles0);
+    } catch (...) {  // keep the first error's message
+    }
   }
   return rc;
 }
