@@ -58,6 +58,9 @@ void *g_scratch_ptr[kMaxScratchDevices] = {};
 namespace {
 
 constexpr int kBlock = 256;  // 4 waves of 64: a 16 x 16 pixel tile
+// The persistent tile kernel's work item: a kTileW x kTileH pixel tile with all its samples,
+// 64 / (kTileW kTileH) sample slots per pixel in a wave.
+constexpr uint32_t kTileW = 2, kTileH = 2;
 
 __device__ __forceinline__ void load_lut(float *lut_s, const float *lut) {
   for (int i = threadIdx.x; i < 256; i += blockDim.x) lut_s[i] = lut[i];
@@ -425,26 +428,31 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
 }
 
 // The direct form in one persistent launch, for a frame or a batch of views. A work item is one
-// 4 x 4 pixel tile of one view with all of its samples (view << 24 | tile, tiles numbered row
-// by row; without `items`, item k is tile k of view 0). Whole waves take items from a device
-// counter. A wave's lanes are 16 pixels x 4 sample slots: in round r, slot j renders sample
-// 4 r + j of its pixel, and the pixel's slot-0 lane gathers the four results by shuffles and
-// adds the finite ones to the pixel's f64 sums in sample order (the CPU backend's order, so the
-// image is render_hq's bit for bit, render/cpu/mod.rs:66-79), kept in LDS; it writes the
-// pixel's mean at the end. No staging and no accumulation pass. A wave leaves once the counter
-// has passed the last item, so every wave ends. With `progress_host`, the index of each item
-// taken is stored to host memory for the caller's progress reports.
+// kTileW x kTileH pixel tile of one view with all of its samples (view << 24 | tile, tiles
+// numbered row by row; without `items`, item k is tile k of view 0). Whole waves take items
+// from a device counter. A wave's lanes are the tile's 4 pixels x 16 sample slots: in round r,
+// slot j renders sample 16 r + j of its pixel, so a round's paths are 16 jittered samples of
+// each of 4 neighbouring pixels, whose first walks and first bounces stay close together. The
+// lanes leave their results in the wave's row of LDS, and the pixel's slot-0 lane adds the
+// finite ones to the pixel's f64 sums in sample order (the CPU backend's order, so the image is
+// render_hq's bit for bit, render/cpu/mod.rs:66-79), kept in LDS; it writes the pixel's mean at
+// the end. No staging and no accumulation pass. A wave leaves once the counter has passed the
+// last item, so every wave ends. With `progress_host`, the index of each item taken is stored
+// to host memory for the caller's progress reports.
 template <int W>
 __global__ void __launch_bounds__(kBlock, W) k_render_tiles(const lw_params p, const DevScene s,
                                                   const BatchDev *__restrict__ bd,
                                                   const uint32_t *__restrict__ items, uint32_t n_items,
                                                   uint32_t *__restrict__ counter, float *__restrict__ out,
                                                   uint32_t *progress_host) {
+  constexpr uint32_t NP = kTileW * kTileH, NS = 64u / NP;
   __shared__ float lut_s[256];
-  __shared__ double sums[(kBlock / 4) * 3];
+  __shared__ double sums[(kBlock / 64) * NP * 3];
+  __shared__ float4 res[kBlock];
   load_lut(lut_s, s.gamma_lut);
-  const uint32_t lane = threadIdx.x & 63u, pix = lane & 15u, slot = lane >> 4;
-  double *acc = sums + (size_t)((threadIdx.x >> 6) * 16u + pix) * 3;
+  const uint32_t lane = threadIdx.x & 63u, pix = lane % NP, slot = lane / NP;
+  double *acc = sums + (size_t)((threadIdx.x >> 6) * NP + pix) * 3;
+  float4 *row = res + (threadIdx.x & ~63u);  // this wave's results, one per lane
   for (;;) {
     uint32_t k = 0;
     if (lane == 0) {
@@ -459,49 +467,39 @@ __global__ void __launch_bounds__(kBlock, W) k_render_tiles(const lw_params p, c
     const uint32_t item = __builtin_amdgcn_readfirstlane(items ? items[k] : k);
     const uint32_t v = item >> 24, tile = item & 0xFFFFFFu;
     const ViewCam &cam = bd->cam[v];
-    const uint32_t tiles_x = (cam.width + 3u) / 4u;
-    const uint32_t px = (tile % tiles_x) * 4u + (pix & 3u);
-    const uint32_t py = (tile / tiles_x) * 4u + (pix >> 2);
+    const uint32_t tiles_x = (cam.width + kTileW - 1u) / kTileW;
+    const uint32_t px = (tile % tiles_x) * kTileW + pix % kTileW;
+    const uint32_t py = (tile / tiles_x) * kTileH + pix / kTileW;
     const bool inside = px < cam.width && py < cam.height;
     if (slot == 0u) {
       acc[0] = 0.0;
       acc[1] = 0.0;
       acc[2] = 0.0;
     }
-    for (uint32_t base = 0; base < cam.spp; base += 4u) {
+    for (uint32_t base = 0; base < cam.spp; base += NS) {
       const uint32_t smp = base + slot;
-      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-      uint32_t ok = 0u;
+      float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // w: 1 for a finite sample
       if (inside && smp < cam.spp) {
         const V3 c = render_sample(s, p, cam, lut_s, px, py, smp);
-        cx = c.x;
-        cy = c.y;
-        cz = c.z;
-        ok = (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) ? 1u : 0u;
+        if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) r = make_float4(c.x, c.y, c.z, 1.0f);
       }
-      float gx[4], gy[4], gz[4];
-      uint32_t gok[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        gx[j] = __shfl(cx, (int)(pix + 16u * j));
-        gy[j] = __shfl(cy, (int)(pix + 16u * j));
-        gz[j] = __shfl(cz, (int)(pix + 16u * j));
-        gok[j] = __shfl(ok, (int)(pix + 16u * j));
-      }
+      row[lane] = r;
+      __builtin_amdgcn_wave_barrier();
       if (slot == 0u && inside) {
         double a0 = acc[0], a1 = acc[1], a2 = acc[2];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (gok[j]) {
-            a0 += (double)gx[j];
-            a1 += (double)gy[j];
-            a2 += (double)gz[j];
+        for (uint32_t j = 0; j < NS; j++) {
+          const float4 q = row[pix + NP * j];
+          if (q.w != 0.0f) {
+            a0 += (double)q.x;
+            a1 += (double)q.y;
+            a2 += (double)q.z;
           }
         }
         acc[0] = a0;
         acc[1] = a1;
         acc[2] = a2;
       }
+      __builtin_amdgcn_wave_barrier();
     }
     if (slot == 0u && inside) {
       float *o = out + ((size_t)bd->pix_prefix[v] + (size_t)py * cam.width + px) * 3;
@@ -1915,7 +1913,7 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
 
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
 // Frames with at least this many samples per pixel render in one persistent k_render_tiles
-// launch (a wave's 4 sample slots are then busy in most rounds).
+// launch (a wave's 16 sample slots are then busy in every round).
 constexpr uint32_t kTilesMinSpp = 16;
 constexpr uint32_t kTilesPerWave = 4;
 
@@ -1934,7 +1932,7 @@ void set_tiles_waves(lw_hip_backend *b, int waves) {
 
 // Whether a frame renders in one persistent tile launch (render_frame's rule).
 bool uses_tiles(const lw_hip_backend *b, uint32_t w, uint32_t h, uint32_t spp) {
-  const uint64_t tile_items = (uint64_t)((w + 3) / 4) * ((h + 3) / 4);
+  const uint64_t tile_items = (uint64_t)((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH);
   return b->mode == MODE_DIRECT && b->per_lane == 1 && !b->frame_launches && spp >= kTilesMinSpp &&
          tile_items >= (uint64_t)kTilesPerWave * b->blocks_tiles * (kBlock / 64);
 }
@@ -1949,7 +1947,7 @@ void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_para
   const uint32_t w = p.frame[0], h = p.frame[1];
   const uint64_t npix = (uint64_t)w * h;
   if (npix > (1ull << 26)) throw InvalidScene{"frame too large"};
-  const uint32_t n_items = ((w + 3) / 4) * ((h + 3) / 4);
+  const uint32_t n_items = ((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH);
   BatchPlan plan;
   BatchDev &bd = plan.bd();
   std::memset(&bd, 0, sizeof(bd));
@@ -2299,16 +2297,17 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   lw_timing tm{};
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
   if (tiles_form) {
-    // One persistent launch over (view, 8 x 8 tile) items, the views with the most samples
-    // per pixel first so that the last items to start are the shortest.
+    // One persistent launch over (view, kTileW x kTileH tile) items, the views with the most
+    // samples per pixel first so that the last items to start are the shortest.
     std::vector<uint32_t> order(nv);
     for (size_t v = 0; v < nv; v++) order[v] = (uint32_t)v;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t c) { return bd.cam[a].spp > bd.cam[c].spp; });
     std::vector<uint32_t> list;
     for (uint32_t v : order) {
-      const uint32_t n = ((bd.cam[v].width + 3) / 4) * ((bd.cam[v].height + 3) / 4);
-      for (uint32_t t = 0; t < n; t++) list.push_back(v << 24 | t);
+      const uint64_t n = (uint64_t)((bd.cam[v].width + kTileW - 1) / kTileW) * ((bd.cam[v].height + kTileH - 1) / kTileH);
+      if (n >= (1u << 24)) throw InvalidScene{"a view of the batch has 2^24 tiles or more"};
+      for (uint32_t t = 0; t < (uint32_t)n; t++) list.push_back(v << 24 | t);
     }
     b->batch_items.upload(list.data(), list.size(), b->stream);
     k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
