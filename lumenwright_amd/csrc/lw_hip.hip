@@ -96,6 +96,31 @@ __global__ void __launch_bounds__(kBlock, W) k_render(const lw_params p, const D
   o[2] = c.z;
 }
 
+// k_render with 64-lane waves of 2 x 2 pixels x 16 samples, for launches of a multiple of 16
+// samples per pixel: a workgroup is a 4 x 4 pixel tile (2 x 2 waves) at 16 consecutive samples
+// of the launch (blockIdx.y picks which 16), so a wave's paths are 16 jittered samples of each
+// of 4 neighbouring pixels, whose first walks and bounces stay close together (as in
+// k_render_tiles). The staging is indexed as k_render's, so the launch's accumulation is
+// unchanged.
+template <int W>
+__global__ void __launch_bounds__(kBlock, W) k_render_slots(const lw_params p, const DevScene s,
+                                                         uint32_t sample_start, float *__restrict__ rad) {
+  __shared__ float lut_s[256];
+  load_lut(lut_s, s.gamma_lut);
+  const uint32_t w = p.frame[0], h = p.frame[1];
+  const uint32_t tiles_x = (w + 3u) / 4u;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, pix = lane & 3u, slot = lane >> 2;
+  const uint32_t px = (blockIdx.x % tiles_x) * 4u + (wave & 1u) * 2u + (pix & 1u);
+  const uint32_t py = (blockIdx.x / tiles_x) * 4u + (wave >> 1) * 2u + (pix >> 1);
+  if (px >= w || py >= h) return;
+  const uint32_t sl = blockIdx.y * 16u + slot;
+  const V3 c = render_sample(s, p, lut_s, px, py, sample_start + sl);
+  float *o = rad + ((size_t)sl * h * w + (size_t)py * w + px) * 3;
+  o[0] = c.x;
+  o[1] = c.y;
+  o[2] = c.z;
+}
+
 // The trace calls of samples [sample_start, sample_start + gridDim.y) of every pixel, as
 // k_render renders them (the same lanes, the same paths): summed per workgroup in LDS, one
 // 64-bit atomic a workgroup.
@@ -2108,6 +2133,10 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
       LW_DIRECT_DISPATCH(b->waves, (k_render_multi<W><<<dim3(tiles, (nb + per - 1) / per),
                                                          dim3(kBlock), 0, b->stream>>>(
                                        p, ds, done, nb, per, rad)));
+    } else if (mode == MODE_DIRECT && nb % 16u == 0 && !b->xcd_remap) {
+      const uint32_t tiles4 = ((w + 3) / 4) * ((h + 3) / 4);
+      LW_DIRECT_DISPATCH(b->waves, (k_render_slots<W><<<dim3(tiles4, nb / 16u), dim3(kBlock), 0, b->stream>>>(
+                                       p, ds, done, rad)));
     } else if (mode == MODE_DIRECT) {
       LW_DIRECT_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
                                        p, ds, done, b->xcd_remap, rad)));
