@@ -96,6 +96,25 @@ __global__ void __launch_bounds__(kBlock, W) k_render(const lw_params p, const D
   o[2] = c.z;
 }
 
+// A launch lane's pixel and sample within the launch. Without `slots`, a workgroup is a
+// 16 x 16 pixel tile (2 x 2 waves of 8 x 8) at sample blockIdx.y; with it, a workgroup is a
+// 4 x 4 pixel tile (2 x 2 waves of 2 x 2 pixels x 16 samples) at samples 16 blockIdx.y .. + 15,
+// for launches of a multiple of 16 samples per pixel (see k_render_slots).
+struct LaunchLane {
+  uint32_t px, py, sl;
+};
+__device__ __forceinline__ LaunchLane launch_lane(uint32_t w, uint32_t slots) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  if (slots) {
+    const uint32_t tx = (w + 3u) / 4u, pix = lane & 3u;
+    return LaunchLane{(blockIdx.x % tx) * 4u + (wave & 1u) * 2u + (pix & 1u),
+                      (blockIdx.x / tx) * 4u + (wave >> 1) * 2u + (pix >> 1), blockIdx.y * 16u + (lane >> 2)};
+  }
+  const uint32_t tx = (w + 15u) / 16u;
+  return LaunchLane{(blockIdx.x % tx) * 16u + (wave & 1u) * 8u + (lane & 7u),
+                    (blockIdx.x / tx) * 16u + (wave >> 1) * 8u + (lane >> 3), blockIdx.y};
+}
+
 // k_render with 64-lane waves of 2 x 2 pixels x 16 samples, for launches of a multiple of 16
 // samples per pixel: a workgroup is a 4 x 4 pixel tile (2 x 2 waves) at 16 consecutive samples
 // of the launch (blockIdx.y picks which 16), so a wave's paths are 16 jittered samples of each
@@ -1015,17 +1034,13 @@ __global__ void __launch_bounds__(kBlock, W) k_wf_trace(const lw_params p, const
 template <int W>
 __global__ void __launch_bounds__(kBlock, W) k_render_diff(const lw_params p, const DevScene s,
                                                         const DiffDev D, uint32_t sample_start,
-                                                        float *__restrict__ rad) {
+                                                        uint32_t slots, float *__restrict__ rad) {
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
   const uint32_t w = p.frame[0], h = p.frame[1];
-  const uint32_t tiles_x = (w + 15u) / 16u;
-  const uint32_t tile = blockIdx.x;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
-  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  const LaunchLane L = launch_lane(w, slots);
+  const uint32_t px = L.px, py = L.py, sl = L.sl;
   if (px >= w || py >= h) return;
-  const uint32_t sl = blockIdx.y;
   const GradSink G{nullptr, 0u, 0u};
   const SuffixList sfx{nullptr, 0u};
   const V3 c = render_sample_diff<false>(s, p, p, lut_s, D, px, py, sample_start + sl, G, sfx,
@@ -1042,7 +1057,7 @@ __global__ void __launch_bounds__(kBlock, W) k_render_diff(const lw_params p, co
 template <int W>
 __global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const DevScene s,
                                                     const DiffDev D, uint32_t sample_start,
-                                                    float inv_spp) {
+                                                    uint32_t slots, float inv_spp) {
   __shared__ float lut_s[256];
   __shared__ uint32_t sfx_s[kMaxSuffix * kBlock];
   extern __shared__ double grad_s[];
@@ -1050,18 +1065,15 @@ __global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const 
   for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) grad_s[i] = 0.0;
   load_lut(lut_s, s.gamma_lut);  // ends with a barrier
   const uint32_t w = p.frame[0], h = p.frame[1];
-  const uint32_t tiles_x = (w + 15u) / 16u;
-  const uint32_t tile = blockIdx.x;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  const uint32_t px = (tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u);
-  const uint32_t py = (tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  const LaunchLane L = launch_lane(w, slots);
+  const uint32_t px = L.px, py = L.py;
   if (px < w && py < h) {
     const float *g = D.dldi + ((size_t)py * w + px) * 3;
     const V3 dl = V3{g[0], g[1], g[2]} * inv_spp;
     if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
       const GradSink G{grad_s, D.n_pal, D.n_int};
       const SuffixList sfx{sfx_s + threadIdx.x, (uint32_t)kBlock};
-      render_sample_diff<true>(s, p, p, lut_s, D, px, py, sample_start + blockIdx.y, G, sfx, dl);
+      render_sample_diff<true>(s, p, p, lut_s, D, px, py, sample_start + L.sl, G, sfx, dl);
     }
   }
   __syncthreads();
@@ -2126,8 +2138,10 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     const auto start = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (diff) {
-      LW_DIFF_DISPATCH(b->waves, (k_render_diff<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
-                                     p, ds, *diff, done, rad)));
+      const uint32_t by16 = nb % 16u == 0 ? 1u : 0u;  // 2 x 2 pixels x 16 samples a wave
+      const dim3 grid = by16 ? dim3(((w + 3) / 4) * ((h + 3) / 4), nb / 16u) : dim3(tiles, nb);
+      LW_DIFF_DISPATCH(b->waves, (k_render_diff<W><<<grid, dim3(kBlock), 0, b->stream>>>(
+                                     p, ds, *diff, done, by16, rad)));
     } else if (mode == MODE_DIRECT && b->per_lane > 1) {
       const uint32_t per = std::min(b->per_lane, nb);
       LW_DIRECT_DISPATCH(b->waves, (k_render_multi<W><<<dim3(tiles, (nb + per - 1) / per),
@@ -2513,8 +2527,10 @@ void adjoint_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   for (uint32_t done = 0; done < spp;) {
     const uint32_t nb = std::min(spp - done, kMaxBatch);
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    LW_DIFF_DISPATCH(b->waves, (k_adjoint<W><<<dim3(tiles, nb), dim3(kBlock), lds, b->stream>>>(
-                                   p, ds, D, done, inv_spp)));
+    const uint32_t by16 = nb % 16u == 0 ? 1u : 0u;  // 2 x 2 pixels x 16 samples a wave
+    const dim3 grid = by16 ? dim3(((w + 3) / 4) * ((h + 3) / 4), nb / 16u) : dim3(tiles, nb);
+    LW_DIFF_DISPATCH(b->waves, (k_adjoint<W><<<grid, dim3(kBlock), lds, b->stream>>>(
+                                   p, ds, D, done, by16, inv_spp)));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(hipEventSynchronize(b->ev[1]));
