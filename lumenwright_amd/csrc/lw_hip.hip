@@ -1122,7 +1122,12 @@ __global__ void __launch_bounds__(kBlock, W) k_render_multi(const lw_params p, c
   }
 }
 
-// A batch pass's view, tile and sample slice of workgroup `base` (k_render_batch's layout).
+// A batch pass's view, pixel and sample slice of workgroup `base`. The pass gives each view
+// tiles x n workgroups (n = the view's samples this pass), workgroup q tiles + t standing for
+// 16 x 16 tile t at sample q (k_render_batch's layout). When n is a multiple of 16, the 16
+// workgroups 16 g .. 16 g + 15 of a tile are instead its sixteen 4 x 4 sub-tiles at samples
+// 16 g .. 16 g + 15, with waves of 2 x 2 pixels x 16 samples (see k_render_slots): the same
+// (pixel, sample) pairs, so the staging and its accumulation are unchanged.
 struct BatchItem {
   uint32_t v, sl, px, py;
   bool in_frame;
@@ -1132,11 +1137,18 @@ __device__ __forceinline__ BatchItem batch_item(const BatchDev *__restrict__ bd,
   it.v = find_view(bd->item_prefix, bd->nviews, base);
   const uint32_t rest = (base - bd->item_prefix[it.v]) >> 8;
   const uint32_t tiles = bd->tiles[it.v], tx = bd->tiles_x[it.v];
-  const uint32_t tile = rest % tiles;
-  it.sl = rest / tiles;
+  const uint32_t tile = rest % tiles, q = rest / tiles;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  it.px = (tile % tx) * 16u + (wave & 1u) * 8u + (lane & 7u);
-  it.py = (tile / tx) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  if (bd->n[it.v] % 16u == 0) {
+    const uint32_t sub = q & 15u, pix = lane & 3u;
+    it.sl = (q & ~15u) + (lane >> 2);
+    it.px = (tile % tx) * 16u + (sub & 3u) * 4u + (wave & 1u) * 2u + (pix & 1u);
+    it.py = (tile / tx) * 16u + (sub >> 2) * 4u + (wave >> 1) * 2u + (pix >> 1);
+  } else {
+    it.sl = q;
+    it.px = (tile % tx) * 16u + (wave & 1u) * 8u + (lane & 7u);
+    it.py = (tile / tx) * 16u + (wave >> 1) * 8u + (lane >> 3);
+  }
   it.in_frame = it.px < bd->cam[it.v].width && it.py < bd->cam[it.v].height;
   return it;
 }
@@ -2266,7 +2278,9 @@ BatchPlan plan_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *v
     staging = 0;
     items = 0;
     for (size_t v = 0; v < nv; v++) {
-      const uint32_t m = (uint32_t)((bd.cam[v].spp + passes - 1) / passes);
+      // a multiple of 16 where the view has that many, for 2 x 2-pixel x 16-sample waves
+      const uint64_t spp_v = bd.cam[v].spp, m0 = (spp_v + passes - 1) / passes;
+      const uint32_t m = (uint32_t)(m0 >= 16 ? std::min<uint64_t>((m0 + 15) / 16 * 16, spp_v) : m0);
       bd.per_pass[v] = m;
       bd.rad_base[v] = staging / sizeof(float);
       staging += (uint64_t)bd.cam[v].width * bd.cam[v].height * m * 3 * sizeof(float);
