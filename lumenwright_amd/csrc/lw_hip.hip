@@ -1964,6 +1964,10 @@ constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
 // Frames with at least this many samples per pixel render in one persistent k_render_tiles
 // launch (a wave's 16 sample slots are then busy in every round).
 constexpr uint32_t kTilesMinSpp = 16;
+// ... when it has an item for every resident wave and at least kTilesPerWave rounds (an item's
+// 16 samples per pixel) for each: then the last rounds to start are a small part of the
+// launch (measured: 256 x 144 at 64 spp, 7.2 rounds a wave, 712 Msamples/s persistent against
+// 664 in launches; 320 x 180 at 16 spp, 2.8 rounds a wave, 940 against 986).
 constexpr uint32_t kTilesPerWave = 4;
 
 // The tile kernel's budget and the resident workgroups it allows (its launch's grid).
@@ -1982,8 +1986,10 @@ void set_tiles_waves(lw_hip_backend *b, int waves) {
 // Whether a frame renders in one persistent tile launch (render_frame's rule).
 bool uses_tiles(const lw_hip_backend *b, uint32_t w, uint32_t h, uint32_t spp) {
   const uint64_t tile_items = (uint64_t)((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH);
+  const uint64_t waves = (uint64_t)b->blocks_tiles * (kBlock / 64);
+  const uint64_t rounds = tile_items * ((spp + 64u / (kTileW * kTileH) - 1) / (64u / (kTileW * kTileH)));
   return b->mode == MODE_DIRECT && b->per_lane == 1 && !b->frame_launches && spp >= kTilesMinSpp &&
-         tile_items >= (uint64_t)kTilesPerWave * b->blocks_tiles * (kBlock / 64);
+         tile_items >= waves && rounds >= (uint64_t)kTilesPerWave * waves;
 }
 
 // A frame in one persistent launch (k_render_tiles over its 4 x 4 tiles, all samples of a tile
@@ -2099,9 +2105,9 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
   HIP_TRY(hipSetDevice(b->device));
   const DevScene ds = sc.dev();
 
-  // The persistent form needs at least kTilesPerWave items for each resident wave, so that the
-  // last items to start are a small part of the launch; smaller frames fill the device
-  // through the launch ramp's many samples per launch.
+  // The persistent form needs an item for each resident wave and kTilesPerWave rounds for each,
+  // so that the last rounds to start are a small part of the launch; smaller frames fill the
+  // device through the launch ramp's many samples per launch.
   if (!diff && !out_f64 && !one_launch && !fixed && uses_tiles(b, w, h, spp)) {
     render_frame_tiles(b, sc, p, spp, progress, user, out_rgb, t_wall);
     return;
