@@ -1992,10 +1992,10 @@ bool uses_tiles(const lw_hip_backend *b, uint32_t w, uint32_t h, uint32_t spp) {
          tile_items >= waves && rounds >= (uint64_t)kTilesPerWave * waves;
 }
 
-// A frame in one persistent launch (k_render_tiles over its 4 x 4 tiles, all samples of a tile
-// in one wave, summed in sample order in f64): the image equals the launch ramp's bit for bit. With
-// `progress`, the host reports (samples done, spp) while the launch runs, from the item index
-// the kernel last stored to host memory, and (spp, spp) at the end.
+// A frame in one persistent launch (k_render_tiles over its 2 x 2-pixel items, all samples of an
+// item in one wave, summed in sample order in f64): the image equals the launch ramp's bit for
+// bit. With `progress`, the host reports (samples done, spp) while the launch runs, from the item
+// index the kernel last stored to host memory, and (spp, spp) at the end.
 void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, uint32_t spp,
                         lw_progress_fn progress, void *user, float *out_rgb,
                         std::chrono::steady_clock::time_point t_wall) {

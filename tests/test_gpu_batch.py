@@ -166,3 +166,50 @@ def test_autotune_picks_the_fastest_shape_and_keeps_the_images(batch_setup):
         sc.free()
     finally:
         b.close()
+
+
+def test_tile_kernel_frame_with_ragged_edges_and_a_partial_sample_group():
+    # 641 x 201 pixels at 84 spp: the 2 x 2-pixel items along the right and bottom edges hold
+    # one or two pixels, and each item's last round of 16 sample slots holds 4 samples. The
+    # persistent tile kernel (one launch) gives the launch ramp's image bit for bit.
+    cfg = frames.config(1, width=641, height=201, spp=84, world_chunks_x=6, world_chunks_z=6)
+    frame = frames.make_frame(cfg)
+    tiles = open_backend({})
+    ramp = open_backend({"LW_FRAME_FORM": "launches"})
+    try:
+        s1, s2 = tiles.upload(frame), ramp.upload(frame)
+        a = tiles.render_hq_scene(s1, frame.params, cfg.spp)
+        assert tiles.last_timing()["launches"] == 1
+        b = ramp.render_hq_scene(s2, frame.params, cfg.spp)
+        assert ramp.last_timing()["launches"] > 1
+        assert a.shape == (641 * 201, 3) and np.isfinite(a).all() and (a > 0).any()
+        assert np.array_equal(a.view("<u4"), b.view("<u4"))
+        s1.free()
+        s2.free()
+    finally:
+        tiles.close()
+        ramp.close()
+
+
+@pytest.mark.parametrize("env", [{}, {"LW_BATCH_FORM": "passes", "LW_BATCH_PASSES": "2"},
+                                 {"LW_BATCH_FORM": "passes", "LW_BATCH_PASSES": "3"}],
+                         ids=["tiles", "2-passes", "3-passes"])
+def test_batch_passes_with_partial_sample_groups(batch_setup, env):
+    # Views of 20, 40 and 50 spp: in passes, a view's samples per pass round up to a multiple
+    # of 16 (32 + 8, 32 + 18 for two passes), so passes mix whole 16-sample waves with a
+    # remainder; in the one persistent launch an item's last round is partial. Either way
+    # every view equals its single-view render bit for bit.
+    base, views = batch_setup
+    picked = [(views[0][0], 20), (views[1][0], 40), (views[3][0], 50), (views[2][0], 17)]
+    b = open_backend(env)
+    try:
+        scene = b.upload(base)
+        outs = b.render_hq_batch(scene, picked)
+        if env:
+            assert b.last_timing()["launches"] == int(env["LW_BATCH_PASSES"])
+        for (p, spp), img in zip(picked, outs):
+            single = b.render_hq_scene(scene, p, spp)
+            assert np.array_equal(img.view("<u4"), single.view("<u4"))
+        scene.free()
+    finally:
+        b.close()
