@@ -218,9 +218,25 @@ __device__ __forceinline__ float next_boundary(float o, float inv, int32_t cell,
   return (bound - o) * inv;
 }
 
-// trace.rs:137-290. cell_out (when non-null) receives the cell the traversal stopped in.
-__device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 o, V3 d, float tmin,
-                                   float tmax, Hit &out, int32_t *cell_out) {
+// trace.rs:137-290 as a resumable walk: walk_begin sets up the ray (false: it misses the grid
+// or starts beyond tmax) and walk_step runs one iteration of the traversal loop, ending it with
+// a hit (WALK_HIT, the hit in `out`) or a miss (WALK_MISS), or leaving it to go on (WALK_ON).
+// trace() runs a walk to its end; a caller can also run two walks one after the other in one
+// loop, so that its lanes' walks of both share the loop's iterations. cell_out (when non-null)
+// receives the cell the traversal stopped in.
+constexpr int WALK_ON = 0, WALK_HIT = 1, WALK_MISS = 2;
+
+struct Walk {
+  V3 o, d, inv;
+  float tmin, t_end, best_t, t;
+  float next[3];
+  int32_t cell[3], step[3];
+  uint32_t it, cached_b, entry, ov, octant;
+  uint2 occ;
+};
+
+__device__ __forceinline__ bool walk_begin(const lw_params &p, V3 o, V3 d, float tmin, float tmax,
+                                           Walk &w, int32_t *cell_out) {
   const uint32_t dx = p.grid_dims[0], dy = p.grid_dims[1], dz = p.grid_dims[2];
   const V3 size = V3{(float)(dx * 16u), (float)(dy * 16u), (float)(dz * 16u)};
 
@@ -229,263 +245,302 @@ __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 
   const V3 tb = mul(size - o, inv);
   const float t_enter = f_max(f_max(f_min(ta.x, tb.x), f_min(ta.y, tb.y)), f_min(ta.z, tb.z));
   const float t_exit = f_min(f_min(f_max(ta.x, tb.x), f_max(ta.y, tb.y)), f_max(ta.z, tb.z));
-  float t = f_max(t_enter, 0.0f);
+  const float t = f_max(t_enter, 0.0f);
   const float t_end = f_min(t_exit, tmax);
   if (cell_out) cell_out[0] = cell_out[1] = cell_out[2] = -1;
   if (t > t_end) return false;
 
   const V3 start = o + d * (t + 1e-4f);
   const int32_t gs[3] = {(int32_t)dx * 16, (int32_t)dy * 16, (int32_t)dz * 16};
-  int32_t cell[3] = {clamp_i(as_i32(f_floor(start.x)), 0, gs[0] - 1),
-                     clamp_i(as_i32(f_floor(start.y)), 0, gs[1] - 1),
-                     clamp_i(as_i32(f_floor(start.z)), 0, gs[2] - 1)};
-  const int32_t step[3] = {sign_i(d.x), sign_i(d.y), sign_i(d.z)};
-  float next[3];
+  w.cell[0] = clamp_i(as_i32(f_floor(start.x)), 0, gs[0] - 1);
+  w.cell[1] = clamp_i(as_i32(f_floor(start.y)), 0, gs[1] - 1);
+  w.cell[2] = clamp_i(as_i32(f_floor(start.z)), 0, gs[2] - 1);
+  w.step[0] = sign_i(d.x);
+  w.step[1] = sign_i(d.y);
+  w.step[2] = sign_i(d.z);
 #pragma unroll
-  for (int a = 0; a < 3; a++) next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
+  for (int a = 0; a < 3; a++) w.next[a] = next_boundary(get(o, a), get(inv, a), w.cell[a], w.step[a]);
+  w.o = o;
+  w.d = d;
+  w.inv = inv;
+  w.tmin = tmin;
+  w.t_end = t_end;
+  w.best_t = t_end + 1e-3f;
+  w.t = t;
+  w.it = 0;
+  w.cached_b = UINT32_MAX;
+  w.entry = EMPTY;
+  w.ov = EMPTY;
+  w.octant = (w.step[0] >= 0 ? 1u : 0u) | (w.step[1] >= 0 ? 2u : 0u) | (w.step[2] >= 0 ? 4u : 0u);
+  w.occ = make_uint2(0u, 0u);
+  return true;
+}
 
+// The loop's increment and bound: MAX_STEPS iterations end the walk without a hit.
+__device__ __forceinline__ int walk_next(Walk &w) {
+  w.it += 1u;
+  return w.it < MAX_STEPS ? WALK_ON : WALK_MISS;
+}
+
+__device__ __forceinline__ int walk_step(const DevScene &s, const lw_params &p, Walk &w, Hit &out,
+                                         int32_t *cell_out) {
+  const uint32_t dx = p.grid_dims[0], dz = p.grid_dims[2];
+  const int32_t gs[3] = {(int32_t)dx * 16, (int32_t)p.grid_dims[1] * 16, (int32_t)dz * 16};
   const uint32_t *top = s.grid + p.off[0];
   const uint32_t *ov_top = s.grid + p.off[2];
-  const float best_t = t_end + 1e-3f;
+  const V3 o = w.o, d = w.d, inv = w.inv;
+  const float tmin = w.tmin, t_end = w.t_end, best_t = w.best_t;
+  const int32_t *step = w.step;
+  const uint32_t octant = w.octant;
+  int32_t *cell = w.cell;
+  float *next = w.next;
+  float &t = w.t;
+  uint32_t &it = w.it, &cached_b = w.cached_b, &entry = w.entry, &ov = w.ov;
+  uint2 &occ = w.occ;
+  const uint32_t bx = (uint32_t)(cell[0] >> 4);
+  const uint32_t by = (uint32_t)(cell[1] >> 4);
+  const uint32_t bz = (uint32_t)(cell[2] >> 4);
+  const uint32_t b = (by * dz + bz) * dx + bx;
+  if (b != cached_b) {
+    entry = ld(top, b);
+    ov = ld(ov_top, b);
+    cached_b = b;
+    if (entry != EMPTY && (entry & UNIFORM) == 0) occ = ld(s.occupancy, entry);
+  }
+  if (cell_out) {
+    cell_out[0] = cell[0];
+    cell_out[1] = cell[1];
+    cell_out[2] = cell[2];
+  }
 
-  uint32_t cached_b = UINT32_MAX, entry = EMPTY, ov = EMPTY;
-  const uint32_t octant = (step[0] >= 0 ? 1u : 0u) | (step[1] >= 0 ? 2u : 0u) | (step[2] >= 0 ? 4u : 0u);
-  uint2 occ = make_uint2(0u, 0u);
-  for (uint32_t it = 0; it < MAX_STEPS; it++) {
-    const uint32_t bx = (uint32_t)(cell[0] >> 4);
-    const uint32_t by = (uint32_t)(cell[1] >> 4);
-    const uint32_t bz = (uint32_t)(cell[2] >> 4);
-    const uint32_t b = (by * dz + bz) * dx + bx;
-    if (b != cached_b) {
-      entry = ld(top, b);
-      ov = ld(ov_top, b);
-      cached_b = b;
-      if (entry != EMPTY && (entry & UNIFORM) == 0) occ = ld(s.occupancy, entry);
-    }
-    if (cell_out) {
-      cell_out[0] = cell[0];
-      cell_out[1] = cell[1];
-      cell_out[2] = cell[2];
-    }
-
-    if (entry == EMPTY && ov == EMPTY) {
-      // Each brick coordinate of the walk only ever moves in its axis's step direction (a
-      // cell step moves by the step, a brick skip crosses one face in the step's direction
-      // and clamps the other coordinates into the brick, a sub-block jump moves cells by the
-      // step), and not at all on an axis with step 0. So every later brick lies in the box
-      // toward the ray's octant (step 0 counted as +). When that box holds only EMPTY bricks
-      // without overlays, no quad will be tested and the result is "no hit" wherever the
-      // walk would end (t_end, the grid's faces or MAX_STEPS). Only the final cell differs,
-      // so a caller asking for it (cell_out) gets the whole walk.
-      if (cell_out == nullptr && ((ld(s.octant_boxes, b) >> octant) & 1u) == 0u) return false;
-      // An empty brick is crossed in one move, to the face where the ray leaves it.
-      const int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
-      float t_out = INFINITY;
-      int axis = 0;
+  if (entry == EMPTY && ov == EMPTY) {
+    // Each brick coordinate of the walk only ever moves in its axis's step direction (a
+    // cell step moves by the step, a brick skip crosses one face in the step's direction
+    // and clamps the other coordinates into the brick, a sub-block jump moves cells by the
+    // step), and not at all on an axis with step 0. So every later brick lies in the box
+    // toward the ray's octant (step 0 counted as +). When that box holds only EMPTY bricks
+    // without overlays, no quad will be tested and the result is "no hit" wherever the
+    // walk would end (t_end, the grid's faces or MAX_STEPS). Only the final cell differs,
+    // so a caller asking for it (cell_out) gets the whole walk.
+    if (cell_out == nullptr && ((ld(s.octant_boxes, b) >> octant) & 1u) == 0u) return WALK_MISS;
+    // An empty brick is crossed in one move, to the face where the ray leaves it.
+    const int32_t bc[3] = {(int32_t)bx, (int32_t)by, (int32_t)bz};
+    float t_out = INFINITY;
+    int axis = 0;
 #pragma unroll
-      for (int a = 0; a < 3; a++) {
-        if (step[a] != 0) {
-          int32_t bound = step[a] > 0 ? (bc[a] + 1) * 16 : bc[a] * 16;
-          float tt = ((float)bound - get(o, a)) * get(inv, a);
-          if (tt < t_out) {
-            t_out = tt;
-            axis = a;
-          }
-        }
-      }
-      if (t_out > t_end) break;
-      const V3 pos = o + d * t_out;
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        if (a == axis) {
-          cell[a] = step[a] > 0 ? (bc[a] + 1) * 16 : bc[a] * 16 - 1;
-        } else {
-          cell[a] = clamp_i(as_i32(f_floor(get(pos, a))), bc[a] * 16, bc[a] * 16 + 15);
-        }
-      }
-      const int32_t ca = axis == 0 ? cell[0] : (axis == 1 ? cell[1] : cell[2]);
-      const int32_t ga = axis == 0 ? gs[0] : (axis == 1 ? gs[1] : gs[2]);
-      if (ca < 0 || ca >= ga) break;
-#pragma unroll
-      for (int a = 0; a < 3; a++) next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
-      t = t_out;
-      continue;
-    }
-
-    const bool stored = entry != EMPTY && (entry & UNIFORM) == 0;
-    // The occupancy bit of the cell's 4^3 sub-block (1 outside stored bricks).
-    uint32_t sb_full = 1u;
-    if (stored) {
-      const uint32_t sb = ((((uint32_t)cell[1] & 15u) >> 2) * 4u + (((uint32_t)cell[2] & 15u) >> 2)) * 4u +
-                          (((uint32_t)cell[0] & 15u) >> 2);
-      sb_full = ((sb < 32u ? occ.x : occ.y) >> (sb & 31u)) & 1u;
-    }
-    uint32_t model;
-    const uint32_t local =
-        (uint32_t)((((cell[1] & 15) * 16 + (cell[2] & 15)) * 16) + (cell[0] & 15));
-    if (entry == EMPTY) {
-      model = 0;
-    } else if ((entry & UNIFORM) != 0) {
-      model = entry & ~UNIFORM;
-    } else if (sb_full) {
-      const uint32_t w = ld(s.grid, p.off[1] + entry * LW_BRICK_WORDS + (local >> 1));
-      model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
-    } else {
-      model = 0;
-    }
-    if (stored && ov == EMPTY) {
-      // A box of empty cells around the current one, each with no model (and the brick has no
-      // overlay): cross it in one move, landing where the cell-by-cell walk below would, with
-      // the same t, cells and step count. The walk is a merge of the three axes' crossing times
-      // (each a pure function of the cell, next_boundary), taking the smallest and, on a tie,
-      // the higher axis. It leaves the box at the first crossing of a far face, ts on axis x;
-      // by then each other axis has made the crossings that come before (ts, x) in that order.
-      // Taken only when that exit stays in the grid, within t_end and within MAX_STEPS;
-      // otherwise the cells are walked one by one. The box is the cell's empty 4^3 sub-block,
-      // or the empty 8^3 block holding it (aligned boxes inside the brick), or, for an empty
-      // cell of an occupied sub-block, the empty cube from the cell toward the ray's octant
-      // (cell_ext), clipped to the brick.
-      int32_t far[3];
-      bool boxed = false;
-      if (sb_full == 0u) {
-        // The 8^3 block's eight sub-blocks are the occupancy bits 0x330033 shifted by
-        // 2 x8 + 8 z8 + 32 y8.
-        const uint32_t x8 = ((uint32_t)cell[0] & 15u) >> 3, y8 = ((uint32_t)cell[1] & 15u) >> 3,
-                       z8 = ((uint32_t)cell[2] & 15u) >> 3;
-        const int32_t span = ((y8 ? occ.y : occ.x) & (0x330033u << (2u * x8 + 8u * z8))) == 0u ? 8 : 4;
-#pragma unroll
-        for (int a = 0; a < 3; a++) far[a] = (cell[a] & ~(span - 1)) + (step[a] > 0 ? span - 1 : 0);
-        boxed = true;
-      } else if (model == 0u && s.cell_ext != nullptr) {
-        const int32_t n = (int32_t)((ld(s.cell_ext, entry * 4096u + local) >> (4u * octant)) & 15u);
-        if (n >= 2) {
-#pragma unroll
-          for (int a = 0; a < 3; a++) {
-            const int32_t lo = cell[a] & ~15;
-            far[a] = step[a] > 0 ? min(cell[a] + n - 1, lo + 15) : (step[a] < 0 ? max(cell[a] - n + 1, lo) : cell[a]);
-          }
-          boxed = true;
-        }
-      }
-      if (boxed) {
-        float tf[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) tf[a] = next_boundary(get(o, a), get(inv, a), far[a], step[a]);
-        int x;
-        if (tf[0] < tf[1]) {
-          x = tf[0] < tf[2] ? 0 : 2;
-        } else if (tf[1] < tf[2]) {
-          x = 1;
-        } else {
-          x = 2;
-        }
-        const float ts = x == 0 ? tf[0] : (x == 1 ? tf[1] : tf[2]);
-        int32_t nc[3];
-        uint32_t moves = 1;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-          int32_t c = cell[a];
-          if (a == x) {
-            moves += (uint32_t)((far[a] - c) * step[a]);
-            c = far[a] + step[a];
-          } else if (step[a] != 0) {
-            // The walk crosses axis a's boundaries from c toward far[a] while each comes
-            // before (ts, x). Their times never fall from one crossing to the next (f32
-            // rounding is monotone), so the crossings made are a prefix, and its length is
-            // found by binary lifting over the at most 7 of them, branch-free.
-            const int32_t avail = (far[a] - c) * step[a];
-            int32_t k = 0;
-#pragma unroll
-            for (int32_t st = 4; st >= 1; st >>= 1) {
-              const int32_t j = k + st;
-              const float tb = next_boundary(get(o, a), get(inv, a), c + (j - 1) * step[a], step[a]);
-              if (j <= avail && (tb < ts || (tb == ts && a > x))) k = j;
-            }
-            c += k * step[a];
-            moves += (uint32_t)k;
-          }
-          nc[a] = c;
-        }
-        const int32_t cx = x == 0 ? nc[0] : (x == 1 ? nc[1] : nc[2]);
-        const int32_t gx = x == 0 ? gs[0] : (x == 1 ? gs[1] : gs[2]);
-        if (ts <= t_end && cx >= 0 && cx < gx && it + moves - 1u < MAX_STEPS) {
-#pragma unroll
-          for (int a = 0; a < 3; a++) {
-            cell[a] = nc[a];
-            next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
-          }
-          t = ts;
-          it += moves - 1u;
-          continue;
+    for (int a = 0; a < 3; a++) {
+      if (step[a] != 0) {
+        int32_t bound = step[a] > 0 ? (bc[a] + 1) * 16 : bc[a] * 16;
+        float tt = ((float)bound - get(o, a)) * get(inv, a);
+        if (tt < t_out) {
+          t_out = tt;
+          axis = a;
         }
       }
     }
-
-    const float t_cell_exit = f_min(next[0], f_min(next[1], next[2]));
-    const float t_lo = f_max(t - 1e-4f, tmin);
-    const float t_hi = t_cell_exit + 1e-4f;
-    Hit cand{f_min(t_hi, best_t), UINT32_MAX, 0u, 0.0f, 0.0f};
-    if (model != 0) {
-      const lw_model m = ld(s.models, model);
-      const uint32_t fw = ld(s.mfaces, model);
-      const V3 lo = o - v3((float)cell[0], (float)cell[1], (float)cell[2]);
-      // A face model's quads that can pass this ray's front-only test, one per axis with a
-      // non-zero step (the quad whose normal has the step's sign), tested in quad order. The
-      // others would be culled by test_quad without a candidate, so skipping them leaves the
-      // same best hit: a wave of cube visits runs 3 quad tests instead of 6.
-      uint32_t n = m.quad_count, order = 0u;
-      if (fw & MF_FACES) {
-        uint32_t c[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-          c[a] = step[a] == 0 ? MF_NONE : (fw >> (3u * (2u * (uint32_t)a + (step[a] < 0 ? 1u : 0u)))) & 7u;
-        const uint32_t s0 = min(min(c[0], c[1]), c[2]), s2 = max(max(c[0], c[1]), c[2]);
-        const uint32_t s1 = c[0] ^ c[1] ^ c[2] ^ s0 ^ s2;
-        n = (c[0] != MF_NONE ? 1u : 0u) + (c[1] != MF_NONE ? 1u : 0u) + (c[2] != MF_NONE ? 1u : 0u);
-        order = s0 | (s1 << 3) | (s2 << 6);
-      }
-      for (uint32_t i = 0; i < n; i++) {
-        const uint32_t k = (fw & MF_FACES) ? (order >> (3u * i)) & 7u : i;
-        test_quad(s, p, m.quad_offset + k, lo, d, t_lo, cand);
-      }
-    }
-    if (ov != EMPTY) {
-      const uint32_t base = p.off[3] + ov * LW_OV_CELL_WORDS + local * 2u;
-      const uint32_t off = ld(s.grid, base);
-      const uint32_t count = ld(s.grid, base + 1);
-      for (uint32_t i = 0; i < count; i++) {
-        const uint32_t qi = ld(s.grid, p.off2[0] + off + i);
-        test_quad(s, p, qi, o, d, t_lo, cand);
-      }
-    }
-    if (cand.quad != UINT32_MAX) {
-      out = cand;
-      return true;
-    }
-
-    int axis;
-    if (next[0] < next[1]) {
-      axis = next[0] < next[2] ? 0 : 2;
-    } else if (next[1] < next[2]) {
-      axis = 1;
-    } else {
-      axis = 2;
-    }
-    t = axis == 0 ? next[0] : (axis == 1 ? next[1] : next[2]);
-    if (t > t_end) break;
-    bool out_of_grid = false;
+    if (t_out > t_end) return WALK_MISS;
+    const V3 pos = o + d * t_out;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       if (a == axis) {
-        cell[a] += step[a];
-        out_of_grid = cell[a] < 0 || cell[a] >= gs[a];
-        next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
+        cell[a] = step[a] > 0 ? (bc[a] + 1) * 16 : bc[a] * 16 - 1;
+      } else {
+        cell[a] = clamp_i(as_i32(f_floor(get(pos, a))), bc[a] * 16, bc[a] * 16 + 15);
       }
     }
-    if (out_of_grid) break;
+    const int32_t ca = axis == 0 ? cell[0] : (axis == 1 ? cell[1] : cell[2]);
+    const int32_t ga = axis == 0 ? gs[0] : (axis == 1 ? gs[1] : gs[2]);
+    if (ca < 0 || ca >= ga) return WALK_MISS;
+#pragma unroll
+    for (int a = 0; a < 3; a++) next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
+    t = t_out;
+    return walk_next(w);
   }
-  return false;
+
+  const bool stored = entry != EMPTY && (entry & UNIFORM) == 0;
+  // The occupancy bit of the cell's 4^3 sub-block (1 outside stored bricks).
+  uint32_t sb_full = 1u;
+  if (stored) {
+    const uint32_t sb = ((((uint32_t)cell[1] & 15u) >> 2) * 4u + (((uint32_t)cell[2] & 15u) >> 2)) * 4u +
+                        (((uint32_t)cell[0] & 15u) >> 2);
+    sb_full = ((sb < 32u ? occ.x : occ.y) >> (sb & 31u)) & 1u;
+  }
+  uint32_t model;
+  const uint32_t local =
+      (uint32_t)((((cell[1] & 15) * 16 + (cell[2] & 15)) * 16) + (cell[0] & 15));
+  if (entry == EMPTY) {
+    model = 0;
+  } else if ((entry & UNIFORM) != 0) {
+    model = entry & ~UNIFORM;
+  } else if (sb_full) {
+    const uint32_t w = ld(s.grid, p.off[1] + entry * LW_BRICK_WORDS + (local >> 1));
+    model = (w >> ((local & 1u) * 16u)) & 0xFFFFu;
+  } else {
+    model = 0;
+  }
+  if (stored && ov == EMPTY) {
+    // A box of empty cells around the current one, each with no model (and the brick has no
+    // overlay): cross it in one move, landing where the cell-by-cell walk below would, with
+    // the same t, cells and step count. The walk is a merge of the three axes' crossing times
+    // (each a pure function of the cell, next_boundary), taking the smallest and, on a tie,
+    // the higher axis. It leaves the box at the first crossing of a far face, ts on axis x;
+    // by then each other axis has made the crossings that come before (ts, x) in that order.
+    // Taken only when that exit stays in the grid, within t_end and within MAX_STEPS;
+    // otherwise the cells are walked one by one. The box is the cell's empty 4^3 sub-block,
+    // or the empty 8^3 block holding it (aligned boxes inside the brick), or, for an empty
+    // cell of an occupied sub-block, the empty cube from the cell toward the ray's octant
+    // (cell_ext), clipped to the brick.
+    int32_t far[3];
+    bool boxed = false;
+    if (sb_full == 0u) {
+      // The 8^3 block's eight sub-blocks are the occupancy bits 0x330033 shifted by
+      // 2 x8 + 8 z8 + 32 y8.
+      const uint32_t x8 = ((uint32_t)cell[0] & 15u) >> 3, y8 = ((uint32_t)cell[1] & 15u) >> 3,
+                     z8 = ((uint32_t)cell[2] & 15u) >> 3;
+      const int32_t span = ((y8 ? occ.y : occ.x) & (0x330033u << (2u * x8 + 8u * z8))) == 0u ? 8 : 4;
+#pragma unroll
+      for (int a = 0; a < 3; a++) far[a] = (cell[a] & ~(span - 1)) + (step[a] > 0 ? span - 1 : 0);
+      boxed = true;
+    } else if (model == 0u && s.cell_ext != nullptr) {
+      const int32_t n = (int32_t)((ld(s.cell_ext, entry * 4096u + local) >> (4u * octant)) & 15u);
+      if (n >= 2) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const int32_t lo = cell[a] & ~15;
+          far[a] = step[a] > 0 ? min(cell[a] + n - 1, lo + 15) : (step[a] < 0 ? max(cell[a] - n + 1, lo) : cell[a]);
+        }
+        boxed = true;
+      }
+    }
+    if (boxed) {
+      float tf[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) tf[a] = next_boundary(get(o, a), get(inv, a), far[a], step[a]);
+      int x;
+      if (tf[0] < tf[1]) {
+        x = tf[0] < tf[2] ? 0 : 2;
+      } else if (tf[1] < tf[2]) {
+        x = 1;
+      } else {
+        x = 2;
+      }
+      const float ts = x == 0 ? tf[0] : (x == 1 ? tf[1] : tf[2]);
+      int32_t nc[3];
+      uint32_t moves = 1;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        int32_t c = cell[a];
+        if (a == x) {
+          moves += (uint32_t)((far[a] - c) * step[a]);
+          c = far[a] + step[a];
+        } else if (step[a] != 0) {
+          // The walk crosses axis a's boundaries from c toward far[a] while each comes
+          // before (ts, x). Their times never fall from one crossing to the next (f32
+          // rounding is monotone), so the crossings made are a prefix, and its length is
+          // found by binary lifting over the at most 7 of them, branch-free.
+          const int32_t avail = (far[a] - c) * step[a];
+          int32_t k = 0;
+#pragma unroll
+          for (int32_t st = 4; st >= 1; st >>= 1) {
+            const int32_t j = k + st;
+            const float tb = next_boundary(get(o, a), get(inv, a), c + (j - 1) * step[a], step[a]);
+            if (j <= avail && (tb < ts || (tb == ts && a > x))) k = j;
+          }
+          c += k * step[a];
+          moves += (uint32_t)k;
+        }
+        nc[a] = c;
+      }
+      const int32_t cx = x == 0 ? nc[0] : (x == 1 ? nc[1] : nc[2]);
+      const int32_t gx = x == 0 ? gs[0] : (x == 1 ? gs[1] : gs[2]);
+      if (ts <= t_end && cx >= 0 && cx < gx && it + moves - 1u < MAX_STEPS) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          cell[a] = nc[a];
+          next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
+        }
+        t = ts;
+        it += moves - 1u;
+        return walk_next(w);
+      }
+    }
+  }
+
+  const float t_cell_exit = f_min(next[0], f_min(next[1], next[2]));
+  const float t_lo = f_max(t - 1e-4f, tmin);
+  const float t_hi = t_cell_exit + 1e-4f;
+  Hit cand{f_min(t_hi, best_t), UINT32_MAX, 0u, 0.0f, 0.0f};
+  if (model != 0) {
+    const lw_model m = ld(s.models, model);
+    const uint32_t fw = ld(s.mfaces, model);
+    const V3 lo = o - v3((float)cell[0], (float)cell[1], (float)cell[2]);
+    // A face model's quads that can pass this ray's front-only test, one per axis with a
+    // non-zero step (the quad whose normal has the step's sign), tested in quad order. The
+    // others would be culled by test_quad without a candidate, so skipping them leaves the
+    // same best hit: a wave of cube visits runs 3 quad tests instead of 6.
+    uint32_t n = m.quad_count, order = 0u;
+    if (fw & MF_FACES) {
+      uint32_t c[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        c[a] = step[a] == 0 ? MF_NONE : (fw >> (3u * (2u * (uint32_t)a + (step[a] < 0 ? 1u : 0u)))) & 7u;
+      const uint32_t s0 = min(min(c[0], c[1]), c[2]), s2 = max(max(c[0], c[1]), c[2]);
+      const uint32_t s1 = c[0] ^ c[1] ^ c[2] ^ s0 ^ s2;
+      n = (c[0] != MF_NONE ? 1u : 0u) + (c[1] != MF_NONE ? 1u : 0u) + (c[2] != MF_NONE ? 1u : 0u);
+      order = s0 | (s1 << 3) | (s2 << 6);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t k = (fw & MF_FACES) ? (order >> (3u * i)) & 7u : i;
+      test_quad(s, p, m.quad_offset + k, lo, d, t_lo, cand);
+    }
+  }
+  if (ov != EMPTY) {
+    const uint32_t base = p.off[3] + ov * LW_OV_CELL_WORDS + local * 2u;
+    const uint32_t off = ld(s.grid, base);
+    const uint32_t count = ld(s.grid, base + 1);
+    for (uint32_t i = 0; i < count; i++) {
+      const uint32_t qi = ld(s.grid, p.off2[0] + off + i);
+      test_quad(s, p, qi, o, d, t_lo, cand);
+    }
+  }
+  if (cand.quad != UINT32_MAX) {
+    out = cand;
+    return WALK_HIT;
+  }
+
+  int axis;
+  if (next[0] < next[1]) {
+    axis = next[0] < next[2] ? 0 : 2;
+  } else if (next[1] < next[2]) {
+    axis = 1;
+  } else {
+    axis = 2;
+  }
+  t = axis == 0 ? next[0] : (axis == 1 ? next[1] : next[2]);
+  if (t > t_end) return WALK_MISS;
+  bool out_of_grid = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (a == axis) {
+      cell[a] += step[a];
+      out_of_grid = cell[a] < 0 || cell[a] >= gs[a];
+      next[a] = next_boundary(get(o, a), get(inv, a), cell[a], step[a]);
+    }
+  }
+  if (out_of_grid) return WALK_MISS;
+  return walk_next(w);
 }
+
+__device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 o, V3 d, float tmin,
+                                   float tmax, Hit &out, int32_t *cell_out) {
+  Walk w;
+  if (!walk_begin(p, o, d, tmin, tmax, w, cell_out)) return false;
+  for (;;) {
+    const int r = walk_step(s, p, w, out, cell_out);
+    if (r != WALK_ON) return r == WALK_HIT;
+  }
+}
+
 
 // ---- integrator.rs ----
 
