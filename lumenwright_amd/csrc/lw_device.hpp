@@ -218,11 +218,34 @@ __device__ __forceinline__ float next_boundary(float o, float inv, int32_t cell,
   return (bound - o) * inv;
 }
 
+// The first two arguments of every kernel that traces, (const lw_params p, const DevScene s),
+// as its kernel argument segment holds them: each argument at the next offset aligned for its
+// type, so s right after p. Every kernel that reaches trace, walk_begin, walk_step or
+// sample_event takes these two first (tests/test_library.py checks the sources for it).
+struct KernelArgs {
+  lw_params p;
+  DevScene s;
+};
+static_assert(offsetof(KernelArgs, s) == sizeof(lw_params), "DevScene follows lw_params unpadded");
+
+// The kernel's (p, s) read again from its argument segment through a pointer the compiler
+// cannot trace back to the kernel's entry, so that each field is loaded (a scalar load, from
+// the scalar cache) where it is used. Without it, every field the kernel uses anywhere is
+// loaded at its entry and held in scalar registers for the whole kernel; at this kernel's size
+// most of them are spilled to lanes of a vector register, and each use inside the walk loop
+// then costs a vector instruction (v_readlane) to read it back.
+__device__ __forceinline__ const KernelArgs &kernel_args() {
+  typedef const KernelArgs __attribute__((address_space(4))) *Seg;
+  Seg q = (Seg)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(q));
+  return *(const KernelArgs *)q;
+}
+
 // trace.rs:137-290 as a resumable walk: walk_begin sets up the ray (false: it misses the grid
 // or starts beyond tmax) and walk_step runs one iteration of the traversal loop, ending it with
 // a hit (WALK_HIT, the hit in `out`) or a miss (WALK_MISS), or leaving it to go on (WALK_ON).
 // trace() runs a walk to its end; a caller can also run two walks one after the other in one
-// loop, so that its lanes' walks of both share the loop's iterations. cell_out (when non-null)
+// loop. Both read the scene and its layout from kernel_args(). cell_out (when non-null)
 // receives the cell the traversal stopped in.
 constexpr int WALK_ON = 0, WALK_HIT = 1, WALK_MISS = 2;
 
@@ -235,8 +258,9 @@ struct Walk {
   uint2 occ;
 };
 
-__device__ __forceinline__ bool walk_begin(const lw_params &p, V3 o, V3 d, float tmin, float tmax,
-                                           Walk &w, int32_t *cell_out) {
+__device__ __forceinline__ bool walk_begin(V3 o, V3 d, float tmin, float tmax, Walk &w,
+                                           int32_t *cell_out) {
+  const lw_params &p = kernel_args().p;
   const uint32_t dx = p.grid_dims[0], dy = p.grid_dims[1], dz = p.grid_dims[2];
   const V3 size = V3{(float)(dx * 16u), (float)(dy * 16u), (float)(dz * 16u)};
 
@@ -282,8 +306,10 @@ __device__ __forceinline__ int walk_next(Walk &w) {
   return w.it < MAX_STEPS ? WALK_ON : WALK_MISS;
 }
 
-__device__ __forceinline__ int walk_step(const DevScene &s, const lw_params &p, Walk &w, Hit &out,
-                                         int32_t *cell_out) {
+__device__ __forceinline__ int walk_step(Walk &w, Hit &out, int32_t *cell_out) {
+  const KernelArgs &ka = kernel_args();
+  const DevScene &s = ka.s;
+  const lw_params &p = ka.p;
   const uint32_t dx = p.grid_dims[0], dz = p.grid_dims[2];
   const int32_t gs[3] = {(int32_t)dx * 16, (int32_t)p.grid_dims[1] * 16, (int32_t)dz * 16};
   const uint32_t *top = s.grid + p.off[0];
@@ -531,12 +557,13 @@ __device__ __forceinline__ int walk_step(const DevScene &s, const lw_params &p, 
   return walk_next(w);
 }
 
+// s and p are the kernel's own first two arguments (the walk reads them from kernel_args()).
 __device__ __forceinline__ bool trace(const DevScene &s, const lw_params &p, V3 o, V3 d, float tmin,
                                    float tmax, Hit &out, int32_t *cell_out) {
   Walk w;
-  if (!walk_begin(p, o, d, tmin, tmax, w, cell_out)) return false;
+  if (!walk_begin(o, d, tmin, tmax, w, cell_out)) return false;
   for (;;) {
-    const int r = walk_step(s, p, w, out, cell_out);
+    const int r = walk_step(w, out, cell_out);
     if (r != WALK_ON) return r == WALK_HIT;
   }
 }
@@ -889,10 +916,19 @@ __device__ __forceinline__ void sample_begin(SampleState &S, const Cam &c, uint3
 
 // One iteration of the event loop; false once the path has ended (a `break` of the loop or
 // MAX_EVENTS iterations), true where the loop would go on (`continue` or the loop's end).
-__device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s, const lw_params &p,
-                                             const float *lut, uint32_t *n_traces = nullptr) {
+// s_in and p_in are the kernel's own first two arguments.
+__device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s_in,
+                                             const lw_params &p_in, const float *lut,
+                                             uint32_t *n_traces = nullptr) {
   if (S.ev >= MAX_EVENTS) return false;
   S.ev++;
+  Hit hit;
+  if (n_traces) ++*n_traces;
+  bool has_hit = trace(s_in, p_in, S.o, S.d, S.tmin, INFINITY, hit, nullptr);
+  // The rest of the event reads the kernel's (p, s) afresh (kernel_args).
+  const KernelArgs &ka = kernel_args();
+  const DevScene &s = ka.s;
+  const lw_params &p = ka.p;
   const uint32_t max_bounces = p.frame[3];
   const float eps = p.misc_f[1];
   const uint32_t strategy = p.misc_u[3] > 0 ? p.emitter[0] : LW_EMITTER_SAMPLING_NONE;
@@ -902,9 +938,6 @@ __device__ __forceinline__ bool sample_event(SampleState &S, const DevScene &s, 
   const float sin_r = f_sqrt(f_max(1.0f - p.sun_dir[3] * p.sun_dir[3], 0.0f));
   const bool lit = p.sun_color[0] > 0.0f || p.sun_color[1] > 0.0f || p.sun_color[2] > 0.0f;
 
-  Hit hit;
-  if (n_traces) ++*n_traces;
-  bool has_hit = trace(s, p, S.o, S.d, S.tmin, INFINITY, hit, nullptr);
   bool forced = false;
   if (S.bounces == 0 && p.see_through[1] > 0 && has_hit && !is_see_through(p, hit.quad)) {
     Hit st;

@@ -87,3 +87,67 @@ def test_per_view_arrays_are_checked_before_the_call():
         HipBackend._per_view_images(views, [np.zeros(36), np.zeros(35)], "dL/dI")
     with pytest.raises(ValueError, match="2 views"):
         HipBackend._per_view_images(views, [np.zeros(36)], "target")
+
+
+def _functions(text):
+    """(name, params, body) of each function definition in HIP/C++ source text (comments and
+    strings blanked), found by its parameter list and matched braces."""
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = []
+    for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\(", text):
+        name = m.group(1)
+        if name in ("if", "for", "while", "switch", "return", "sizeof", "static_assert"):
+            continue
+        depth, i = 1, m.end()
+        while i < len(text) and depth:
+            depth += {"(": 1, ")": -1}.get(text[i], 0)
+            i += 1
+        params = text[m.end():i - 1]
+        j = i
+        while j < len(text) and text[j] in " \t\n":
+            j += 1
+        while text.startswith("const", j) or text.startswith("noexcept", j):
+            j = text.index(" ", j) if " " in text[j:j + 9] else j + 5
+            while j < len(text) and text[j] in " \t\n":
+                j += 1
+        if j >= len(text) or text[j] != "{":
+            continue
+        depth, k = 1, j + 1
+        while k < len(text) and depth:
+            depth += {"{": 1, "}": -1}.get(text[k], 0)
+            k += 1
+        head = text[max(0, m.start() - 200):m.start()]
+        out.append((name, params, text[j + 1:k - 1], "__global__" in head.split(";")[-1].split("}")[-1]))
+    return out
+
+
+def test_every_kernel_that_reads_its_arguments_again_takes_params_and_scene_first():
+    # The walk and the event loop read the kernel's (p, s) again from its kernel argument
+    # segment (kernel_args in lw_device.hpp), at offset 0 and right after lw_params. Every
+    # kernel that can reach that read must take (const lw_params p, const DevScene s) first.
+    csrc = ROOT / "lumenwright_amd" / "csrc"
+    funcs = []
+    for f in sorted(csrc.glob("*.hpp")) + sorted(csrc.glob("*.hip")):
+        funcs += _functions(f.read_text())
+    calls = {}
+    for name, _, body, _ in funcs:
+        calls.setdefault(name, set()).update(re.findall(r"\b([A-Za-z_]\w*)\s*[(<]", body))
+    reach = {"kernel_args"}
+    while True:
+        more = {n for n, c in calls.items() if n not in reach and c & reach}
+        if not more:
+            break
+        reach |= more
+    assert {"walk_step", "walk_begin", "trace", "sample_event", "render_sample"} <= reach
+    kernels = [(n, p, b) for n, p, b, g in funcs if g]
+    assert len(kernels) >= 30
+    bad = []
+    checked = 0
+    for name, params, body in kernels:
+        if set(re.findall(r"\b([A-Za-z_]\w*)\s*[(<]", body)) & reach:
+            checked += 1
+            first = [x.strip() for x in params.split(",")[:2]]
+            if first != ["const lw_params p", "const DevScene s"]:
+                bad.append((name, first))
+    assert checked >= 12 and not bad, bad
