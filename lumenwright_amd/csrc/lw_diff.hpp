@@ -171,16 +171,16 @@ __device__ __forceinline__ float rim_draw(uint32_t pixel, uint32_t sample, uint3
 // The radiance of (px, py, sample) under the parameters (ADJ false), or its replay, which
 // adds dl . dL_sample/dtheta into G (ADJ true; returns the same radiance). Emitter sampling
 // strategies NONE and ONE (the host checks).
+// s_in and p_in are the kernel's own first two arguments; after each event's trace the event
+// reads them again from the kernel argument segment (kernel_args, lw_device.hpp).
 template <bool ADJ, class Cam>
-__device__ __forceinline__ V3 render_sample_diff(const DevScene &s, const lw_params &p,
+__device__ __forceinline__ V3 render_sample_diff(const DevScene &s_in, const lw_params &p_in,
                                                  const Cam &c, const float *lut,
                                                  const DiffDev &D, uint32_t px, uint32_t py,
                                                  uint32_t sample, const GradSink &G,
                                                  const SuffixList &sfx, V3 dl) {
   const uint32_t width = cam_width(c);
   const uint32_t height = cam_height(c);
-  const uint32_t max_bounces = p.frame[3];
-  const float eps = p.misc_f[1];
   Rng rng = Rng::make(py * width + px, sample, cam_seed(c));
   uint32_t nsfx = 0;
 
@@ -200,17 +200,22 @@ __device__ __forceinline__ V3 render_sample_diff(const DevScene &s, const lw_par
   uint32_t bounces = 0;
   bool specular = true;
   bool in_water = false;
-  const uint32_t strategy = p.misc_u[3] > 0 ? p.emitter[0] : LW_EMITTER_SAMPLING_NONE;
   float tmin = 0.0f;
-  const V3 sigma = V3{p.water_color[0], p.water_color[1], p.water_color[2]};
-  const V3 sun = V3{p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]};
-  const V3 sun_color = V3{p.sun_color[0], p.sun_color[1], p.sun_color[2]};
-  const float sin_r = f_sqrt(f_max(1.0f - p.sun_dir[3] * p.sun_dir[3], 0.0f));
-  const bool lit = p.sun_color[0] > 0.0f || p.sun_color[1] > 0.0f || p.sun_color[2] > 0.0f;
 
   for (uint32_t ev = 0; ev < MAX_EVENTS; ev++) {
     Hit hit;
-    bool has_hit = trace(s, p, o, d, tmin, INFINITY, hit, nullptr);
+    bool has_hit = trace(s_in, p_in, o, d, tmin, INFINITY, hit, nullptr);
+    const KernelArgs &ka = kernel_args();
+    const DevScene &s = ka.s;
+    const lw_params &p = ka.p;
+    const uint32_t max_bounces = p.frame[3];
+    const float eps = p.misc_f[1];
+    const uint32_t strategy = p.misc_u[3] > 0 ? p.emitter[0] : LW_EMITTER_SAMPLING_NONE;
+    const V3 sigma = V3{p.water_color[0], p.water_color[1], p.water_color[2]};
+    const V3 sun = V3{p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]};
+    const V3 sun_color = V3{p.sun_color[0], p.sun_color[1], p.sun_color[2]};
+    const float sin_r = f_sqrt(f_max(1.0f - p.sun_dir[3] * p.sun_dir[3], 0.0f));
+    const bool lit = p.sun_color[0] > 0.0f || p.sun_color[1] > 0.0f || p.sun_color[2] > 0.0f;
     bool forced = false;
     if (bounces == 0 && p.see_through[1] > 0 && has_hit && !is_see_through(p, hit.quad)) {
       Hit st;
