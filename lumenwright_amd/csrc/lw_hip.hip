@@ -58,9 +58,14 @@ void *g_scratch_ptr[kMaxScratchDevices] = {};
 namespace {
 
 constexpr int kBlock = 256;  // 4 waves of 64: a 16 x 16 pixel tile
-// The persistent tile kernel's work item: a kTileW x kTileH pixel tile with all its samples,
-// 64 / (kTileW kTileH) sample slots per pixel in a wave.
-constexpr uint32_t kTileW = 2, kTileH = 2;
+// The persistent tile kernel's work item: a tile of one view's pixels with all their samples,
+// rendered by one wave as 2^sh pixels x (64 >> sh) sample slots. The tile shape sh is per
+// view: 0 is one pixel (1 x 1), 1 two pixels side by side (2 x 1), 2 a 2 x 2 square.
+constexpr __host__ __device__ uint32_t tile_w(uint32_t sh) { return sh ? 2u : 1u; }
+constexpr __host__ __device__ uint32_t tile_h(uint32_t sh) { return sh == 2u ? 2u : 1u; }
+__host__ __device__ __forceinline__ uint64_t tile_count(uint32_t w, uint32_t h, uint32_t sh) {
+  return (uint64_t)((w + tile_w(sh) - 1u) / tile_w(sh)) * ((h + tile_h(sh) - 1u) / tile_h(sh));
+}
 
 __device__ __forceinline__ void load_lut(float *lut_s, const float *lut) {
   for (int i = threadIdx.x; i < 256; i += blockDim.x) lut_s[i] = lut[i];
@@ -472,30 +477,91 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
 }
 
 // The direct form in one persistent launch, for a frame or a batch of views. A work item is one
-// kTileW x kTileH pixel tile of one view with all of its samples (view << 24 | tile, tiles
-// numbered row by row; without `items`, item k is tile k of view 0). Whole waves take items
-// from a device counter. A wave's lanes are the tile's 4 pixels x 16 sample slots: in round r,
-// slot j renders sample 16 r + j of its pixel, so a round's paths are 16 jittered samples of
-// each of 4 neighbouring pixels, whose first walks and first bounces stay close together. The
-// lanes leave their results in the wave's row of LDS, and the pixel's slot-0 lane adds the
+// tile of one view with all of its samples (view << 24 | tile, tiles numbered row by row;
+// without `items`, item k is tile k of view 0), its shape sh the view's (tile_w, tile_h).
+// Whole waves take items from a device counter. A wave's lanes are the tile's 2^sh pixels x
+// (64 >> sh) sample slots: in round r, slot j renders sample (64 >> sh) r + j of its pixel, so a
+// round's paths are jittered samples of one pixel (or of 2 or 4 neighbouring pixels), whose
+// first walks and first bounces stay close together: at 256 spp one pixel's 64 samples per
+// round run 3.4% faster than 2 x 2 pixels' 16. The lanes leave their results in the wave's
+// row of LDS, and the pixel's slot-0 lane adds the
 // finite ones to the pixel's f64 sums in sample order (the CPU backend's order, so the image is
 // render_hq's bit for bit, render/cpu/mod.rs:66-79), kept in LDS; it writes the pixel's mean at
 // the end. No staging and no accumulation pass. A wave leaves once the counter has passed the
 // last item, so every wave ends. With `progress_host`, the index of each item taken is stored
 // to host memory for the caller's progress reports.
-template <int W>
+// One item of k_render_tiles: the tile `tile` of view v in shape SH (tile_w, tile_h), all of
+// its samples, its means written to `out`.
+template <int SH>
+__device__ __forceinline__ void tiles_item(const lw_params &p, const DevScene &s,
+                                           const BatchDev *__restrict__ bd, const float *lut_s,
+                                           double *sums, float4 *row, uint32_t v, uint32_t tile,
+                                           float *__restrict__ out) {
+  constexpr uint32_t tw = tile_w(SH), th = tile_h(SH);
+  constexpr uint32_t np = 1u << SH, ns = 64u >> SH;  // pixels, and sample slots per pixel
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t pix = lane & (np - 1u), slot = lane >> SH;
+  double *acc = sums + (size_t)((threadIdx.x >> 6) * 4u + pix) * 3;
+  const ViewCam &cam = bd->cam[v];
+  const uint32_t tiles_x = (cam.width + tw - 1u) / tw;
+  const uint32_t px = (tile % tiles_x) * tw + pix % tw;
+  const uint32_t py = (tile / tiles_x) * th + pix / tw;
+  const bool inside = px < cam.width && py < cam.height;
+  if (slot == 0u) {
+    acc[0] = 0.0;
+    acc[1] = 0.0;
+    acc[2] = 0.0;
+  }
+  for (uint32_t base = 0; base < cam.spp; base += ns) {
+    const uint32_t smp = base + slot;
+    float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // w: 1 for a finite sample
+    if (inside && smp < cam.spp) {
+      const V3 c = render_sample(s, p, cam, lut_s, px, py, smp);
+      if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) r = make_float4(c.x, c.y, c.z, 1.0f);
+    }
+    row[lane] = r;
+    __builtin_amdgcn_wave_barrier();
+    if (slot == 0u && inside) {
+      double a0 = acc[0], a1 = acc[1], a2 = acc[2];
+      for (uint32_t j = 0; j < ns; j++) {
+        const float4 q = row[pix + np * j];
+        if (q.w != 0.0f) {
+          a0 += (double)q.x;
+          a1 += (double)q.y;
+          a2 += (double)q.z;
+        }
+      }
+      acc[0] = a0;
+      acc[1] = a1;
+      acc[2] = a2;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (slot == 0u && inside) {
+    float *o = out + ((size_t)bd->pix_prefix[v] + (size_t)py * cam.width + px) * 3;
+    o[0] = (float)(acc[0] / (double)cam.spp);
+    o[1] = (float)(acc[1] / (double)cam.spp);
+    o[2] = (float)(acc[2] / (double)cam.spp);
+  }
+}
+
+// SH is the tile shape of every item, the same for all views of a launch. A fixed shape makes a
+// one-pixel tile's pixel the same for the whole wave, which frees vector registers (168 B of
+// spill per lane for one pixel against 196 for 2 x 2): 1147 against 1107 Msamples/s at 720p
+// and 256 spp. The shape read per item (one copy of the code for every shape) ran as fast as
+// 2 x 2 everywhere (1104), and one copy of the item's code per shape, picked per item, spilled
+// more (232 B).
+template <int W, int SH>
 __global__ void __launch_bounds__(kBlock, W) k_render_tiles(const lw_params p, const DevScene s,
                                                   const BatchDev *__restrict__ bd,
                                                   const uint32_t *__restrict__ items, uint32_t n_items,
                                                   uint32_t *__restrict__ counter, float *__restrict__ out,
                                                   uint32_t *progress_host) {
-  constexpr uint32_t NP = kTileW * kTileH, NS = 64u / NP;
   __shared__ float lut_s[256];
-  __shared__ double sums[(kBlock / 64) * NP * 3];
+  __shared__ double sums[(kBlock / 64) * 4 * 3];  // up to 4 pixels per wave
   __shared__ float4 res[kBlock];
   load_lut(lut_s, s.gamma_lut);
-  const uint32_t lane = threadIdx.x & 63u, pix = lane % NP, slot = lane / NP;
-  double *acc = sums + (size_t)((threadIdx.x >> 6) * NP + pix) * 3;
+  const uint32_t lane = threadIdx.x & 63u;
   float4 *row = res + (threadIdx.x & ~63u);  // this wave's results, one per lane
   for (;;) {
     uint32_t k = 0;
@@ -510,47 +576,7 @@ __global__ void __launch_bounds__(kBlock, W) k_render_tiles(const lw_params p, c
     if (k >= n_items) break;
     const uint32_t item = __builtin_amdgcn_readfirstlane(items ? items[k] : k);
     const uint32_t v = item >> 24, tile = item & 0xFFFFFFu;
-    const ViewCam &cam = bd->cam[v];
-    const uint32_t tiles_x = (cam.width + kTileW - 1u) / kTileW;
-    const uint32_t px = (tile % tiles_x) * kTileW + pix % kTileW;
-    const uint32_t py = (tile / tiles_x) * kTileH + pix / kTileW;
-    const bool inside = px < cam.width && py < cam.height;
-    if (slot == 0u) {
-      acc[0] = 0.0;
-      acc[1] = 0.0;
-      acc[2] = 0.0;
-    }
-    for (uint32_t base = 0; base < cam.spp; base += NS) {
-      const uint32_t smp = base + slot;
-      float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // w: 1 for a finite sample
-      if (inside && smp < cam.spp) {
-        const V3 c = render_sample(s, p, cam, lut_s, px, py, smp);
-        if (isfinite(c.x) && isfinite(c.y) && isfinite(c.z)) r = make_float4(c.x, c.y, c.z, 1.0f);
-      }
-      row[lane] = r;
-      __builtin_amdgcn_wave_barrier();
-      if (slot == 0u && inside) {
-        double a0 = acc[0], a1 = acc[1], a2 = acc[2];
-        for (uint32_t j = 0; j < NS; j++) {
-          const float4 q = row[pix + NP * j];
-          if (q.w != 0.0f) {
-            a0 += (double)q.x;
-            a1 += (double)q.y;
-            a2 += (double)q.z;
-          }
-        }
-        acc[0] = a0;
-        acc[1] = a1;
-        acc[2] = a2;
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
-    if (slot == 0u && inside) {
-      float *o = out + ((size_t)bd->pix_prefix[v] + (size_t)py * cam.width + px) * 3;
-      o[0] = (float)(acc[0] / (double)cam.spp);
-      o[1] = (float)(acc[1] / (double)cam.spp);
-      o[2] = (float)(acc[2] / (double)cam.spp);
-    }
+    tiles_item<SH>(p, s, bd, lut_s, sums, row, v, tile, out);
   }
 }
 
@@ -1904,6 +1930,23 @@ void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, l
     }                                          \
   } while (0)
 
+// k_render_tiles<W, SH>: 4, 5 or 6 waves per SIMD (set_tiles_waves keeps the budget in that
+// range) and the tile shape SH (0, 1 or 2).
+#define LW_TILES_DISPATCH(waves, shape, KERNEL_EXPR)                           \
+  do {                                                                        \
+    switch ((waves) * 3 + (shape)) {                                          \
+      case 12: { constexpr int W = 4, SH = 0; KERNEL_EXPR; } break;           \
+      case 13: { constexpr int W = 4, SH = 1; KERNEL_EXPR; } break;           \
+      case 14: { constexpr int W = 4, SH = 2; KERNEL_EXPR; } break;           \
+      case 15: { constexpr int W = 5, SH = 0; KERNEL_EXPR; } break;           \
+      case 16: { constexpr int W = 5, SH = 1; KERNEL_EXPR; } break;           \
+      case 17: { constexpr int W = 5, SH = 2; KERNEL_EXPR; } break;           \
+      case 18: { constexpr int W = 6, SH = 0; KERNEL_EXPR; } break;           \
+      case 19: { constexpr int W = 6, SH = 1; KERNEL_EXPR; } break;           \
+      default: { constexpr int W = 6, SH = 2; KERNEL_EXPR; } break;           \
+    }                                                                         \
+  } while (0)
+
 // The differentiable kernels: 4 waves per SIMD, or free (LW_WAVES=1).
 #define LW_DIFF_DISPATCH(waves, KERNEL_EXPR)   \
   do {                                         \
@@ -1962,39 +2005,92 @@ void run_wavefront(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
 
 constexpr double kTargetLaunchMs = 150.0;  // render/cuda.rs:19
 // Frames with at least this many samples per pixel render in one persistent k_render_tiles
-// launch (a wave's 16 sample slots are then busy in every round).
+// launch (a wave's sample slots, 16 or more per pixel, are then busy in every round).
 constexpr uint32_t kTilesMinSpp = 16;
-// ... when it has an item for every resident wave and at least kTilesPerWave rounds (an item's
-// 16 samples per pixel) for each: then the last rounds to start are a small part of the
-// launch (measured: 256 x 144 at 64 spp, 7.2 rounds a wave, 712 Msamples/s persistent against
-// 664 in launches; 320 x 180 at 16 spp, 2.8 rounds a wave, 940 against 986).
+// ... when it has an item for every resident wave and at least kTilesPerWave rounds (a round
+// being one wave's 64 samples of its tile) for each: then the last rounds to start are a small
+// part of the launch (measured with 2 x 2-pixel tiles: 256 x 144 at 64 spp, 7.2 rounds a wave,
+// 712 Msamples/s persistent against 664 in launches; 320 x 180 at 16 spp, 2.8 rounds a wave,
+// 940 against 986).
 constexpr uint32_t kTilesPerWave = 4;
 
 // The tile kernel's budget and the resident workgroups it allows (its launch's grid).
 void set_tiles_waves(lw_hip_backend *b, int waves) {
-  b->tiles_waves = waves;
+  b->tiles_waves = std::min(std::max(waves, 4), 6);
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, b->device));
   int per_cu = 0;
-  LW_DIRECT_DISPATCH(b->tiles_waves, {
+  LW_TILES_DISPATCH(b->tiles_waves, 2, {
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void *>(&k_render_tiles<W>), kBlock, 0));
+        &per_cu, reinterpret_cast<const void *>(&k_render_tiles<W, SH>), kBlock, 0));
   });
   b->blocks_tiles = (uint32_t)std::max(1, prop.multiProcessorCount) * (uint32_t)std::max(1, per_cu);
 }
 
+// A view's tile shape in k_render_tiles: the fewest pixels per wave (one pixel's 64 samples a
+// round, else two pixels' 32) whose rounds leave at most an eighth of the wave's slots idle,
+// else 2 x 2 pixels' 16; and a larger tile where the view would have 2^24 tiles or more (an
+// item holds the tile in 24 bits). 256 spp: one pixel; 84: two; 100 or 16: 2 x 2.
+uint32_t tile_shape(uint32_t w, uint32_t h, uint32_t spp) {
+  uint32_t sh = 2;
+  for (uint32_t c = 0; c < 2; c++) {
+    const uint64_t ns = 64u >> c, slots = (spp + ns - 1) / ns * ns;
+    if ((slots - spp) * 8 <= slots) {
+      sh = c;
+      break;
+    }
+  }
+  while (sh < 2 && tile_count(w, h, sh) >= (1u << 24)) sh++;
+  return sh;
+}
+
+// The tile shape of a batch's one k_render_tiles launch: every view's shape when they agree,
+// else the shape of the fewest wave slots per sample over all views, each shape's slots
+// weighted by its measured cost per slot (1, 1.015, 1.035 for one, two and four pixels per
+// wave); and then a larger tile while a view would have 2^24 tiles or more.
+uint32_t batch_tile_shape(const BatchDev &bd) {
+  const uint32_t nv = bd.nviews;
+  uint32_t common = tile_shape(bd.cam[0].width, bd.cam[0].height, bd.cam[0].spp);
+  for (uint32_t v = 1; v < nv && common < 3; v++)
+    if (tile_shape(bd.cam[v].width, bd.cam[v].height, bd.cam[v].spp) != common) common = 3;
+  uint32_t sh = common;
+  if (common == 3) {
+    const double weight[3] = {1.0, 1.015, 1.035};
+    double best = 0.0;
+    for (uint32_t c = 0; c < 3; c++) {
+      const uint64_t ns = 64u >> c;
+      double slots = 0.0;
+      for (uint32_t v = 0; v < nv; v++) {
+        const ViewCam &cam = bd.cam[v];
+        slots += (double)cam.width * cam.height * (double)((cam.spp + ns - 1) / ns * ns);
+      }
+      if (c == 0 || slots * weight[c] < best) {
+        best = slots * weight[c];
+        sh = c;
+      }
+    }
+  }
+  for (bool big = true; big && sh < 2;) {
+    big = false;
+    for (uint32_t v = 0; v < nv; v++)
+      if (tile_count(bd.cam[v].width, bd.cam[v].height, sh) >= (1u << 24)) big = true;
+    if (big) sh++;
+  }
+  return sh;
+}
+
 // Whether a frame renders in one persistent tile launch (render_frame's rule).
 bool uses_tiles(const lw_hip_backend *b, uint32_t w, uint32_t h, uint32_t spp) {
-  const uint64_t tile_items = (uint64_t)((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH);
+  const uint32_t sh = tile_shape(w, h, spp);
+  const uint64_t tile_items = tile_count(w, h, sh);
   const uint64_t waves = (uint64_t)b->blocks_tiles * (kBlock / 64);
-  const uint64_t rounds = tile_items * ((spp + 64u / (kTileW * kTileH) - 1) / (64u / (kTileW * kTileH)));
+  const uint64_t rounds = tile_items * ((spp + (64u >> sh) - 1) / (64u >> sh));
   return b->mode == MODE_DIRECT && b->per_lane == 1 && !b->frame_launches && spp >= kTilesMinSpp &&
          tile_items >= waves && rounds >= (uint64_t)kTilesPerWave * waves;
 }
 
-// A frame in one persistent launch (k_render_tiles over its 2 x 2-pixel items, all samples of an
-// item in one wave, summed in sample order in f64): the image equals the launch ramp's bit for
-// bit. With `progress`, the host reports (samples done, spp) while the launch runs, from the item
+// A frame in one persistent launch (k_render_tiles over its tiles, all samples of a tile in one
+// wave, summed in sample order in f64): the image equals the launch ramp's bit for bit. With `progress`, the host reports (samples done, spp) while the launch runs, from the item
 // index the kernel last stored to host memory, and (spp, spp) at the end.
 void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p, uint32_t spp,
                         lw_progress_fn progress, void *user, float *out_rgb,
@@ -2002,7 +2098,10 @@ void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_para
   const uint32_t w = p.frame[0], h = p.frame[1];
   const uint64_t npix = (uint64_t)w * h;
   if (npix > (1ull << 26)) throw InvalidScene{"frame too large"};
-  const uint32_t n_items = ((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH);
+  const uint32_t sh = tile_shape(w, h, spp);
+  const uint64_t n_tiles = tile_count(w, h, sh);
+  if (n_tiles >= (1u << 24)) throw InvalidScene{"frame too large"};
+  const uint32_t n_items = (uint32_t)n_tiles;
   BatchPlan plan;
   BatchDev &bd = plan.bd();
   std::memset(&bd, 0, sizeof(bd));
@@ -2027,7 +2126,7 @@ void render_frame_tiles(lw_hip_backend *b, const lw_hip_scene &sc, const lw_para
   const DevScene ds = sc.dev();
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (n_items + 3) / 4));
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-  LW_DIRECT_DISPATCH(b->tiles_waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+  LW_TILES_DISPATCH(b->tiles_waves, sh, (k_render_tiles<W, SH><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
                                    p, ds, b->batch_dev.ptr, nullptr, n_items, b->counter.ptr, b->out.ptr,
                                    progress ? b->progress_dev : nullptr)));
   HIP_TRY(hipGetLastError());
@@ -2360,22 +2459,24 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
   lw_timing tm{};
   HIP_TRY(hipEventRecord(b->ev[0], b->stream));
   if (tiles_form) {
-    // One persistent launch over (view, kTileW x kTileH tile) items, the views with the most
-    // samples per pixel first so that the last items to start are the shortest.
+    // One persistent launch over (view, tile) items, each view with its own tile shape, the
+    // views with the most samples per pixel first so that the last items to start are the
+    // shortest.
     std::vector<uint32_t> order(nv);
     for (size_t v = 0; v < nv; v++) order[v] = (uint32_t)v;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t c) { return bd.cam[a].spp > bd.cam[c].spp; });
+    const uint32_t shape = batch_tile_shape(bd);
     std::vector<uint32_t> list;
     for (uint32_t v : order) {
-      const uint64_t n = (uint64_t)((bd.cam[v].width + kTileW - 1) / kTileW) * ((bd.cam[v].height + kTileH - 1) / kTileH);
+      const uint64_t n = tile_count(bd.cam[v].width, bd.cam[v].height, shape);
       if (n >= (1u << 24)) throw InvalidScene{"a view of the batch has 2^24 tiles or more"};
       for (uint32_t t = 0; t < (uint32_t)n; t++) list.push_back(v << 24 | t);
     }
     b->batch_items.upload(list.data(), list.size(), b->stream);
     k_fill_u32<<<dim3(1), dim3(64), 0, b->stream>>>(b->counter.ptr, 0u);
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(b->blocks_tiles, (uint32_t)((list.size() + 3) / 4)));
-    LW_DIRECT_DISPATCH(b->tiles_waves, (k_render_tiles<W><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
+    LW_TILES_DISPATCH(b->tiles_waves, shape, (k_render_tiles<W, SH><<<dim3(grid), dim3(kBlock), 0, b->stream>>>(
                                      p0, ds, b->batch_dev.ptr, b->batch_items.ptr, (uint32_t)list.size(),
                                      b->counter.ptr, b->out.ptr, nullptr)));
     HIP_TRY(hipGetLastError());

@@ -168,11 +168,14 @@ def test_autotune_picks_the_fastest_shape_and_keeps_the_images(batch_setup):
         b.close()
 
 
-def test_tile_kernel_frame_with_ragged_edges_and_a_partial_sample_group():
-    # 641 x 201 pixels at 84 spp: the 2 x 2-pixel items along the right and bottom edges hold
-    # one or two pixels, and each item's last round of 16 sample slots holds 4 samples. The
-    # persistent tile kernel (one launch) gives the launch ramp's image bit for bit.
-    cfg = frames.config(1, width=641, height=201, spp=84, world_chunks_x=6, world_chunks_z=6)
+@pytest.mark.parametrize("spp", [128, 84, 100, 20], ids=["1x1", "2x1", "2x2-7-rounds", "2x2"])
+def test_tile_kernel_frame_with_ragged_edges_and_a_partial_sample_group(spp):
+    # 641 x 201 pixels. The tile shape follows the spp: at 128 a wave is one pixel's 64 sample
+    # slots (2 rounds), at 84 two pixels' 32 (3 rounds, the last with 20 samples), at 100 and
+    # 20 four pixels' 16 (the last round with 4 samples); the tiles along the right and bottom
+    # edges hold fewer pixels. The persistent tile kernel (one launch) gives the launch ramp's
+    # image bit for bit.
+    cfg = frames.config(1, width=641, height=201, spp=spp, world_chunks_x=6, world_chunks_z=6)
     frame = frames.make_frame(cfg)
     tiles = open_backend({})
     ramp = open_backend({"LW_FRAME_FORM": "launches"})
@@ -195,12 +198,14 @@ def test_tile_kernel_frame_with_ragged_edges_and_a_partial_sample_group():
                                  {"LW_BATCH_FORM": "passes", "LW_BATCH_PASSES": "3"}],
                          ids=["tiles", "2-passes", "3-passes"])
 def test_batch_passes_with_partial_sample_groups(batch_setup, env):
-    # Views of 20, 40 and 50 spp: in passes, a view's samples per pass round up to a multiple
-    # of 16 (32 + 8, 32 + 18 for two passes), so passes mix whole 16-sample waves with a
-    # remainder; in the one persistent launch an item's last round is partial. Either way
-    # every view equals its single-view render bit for bit.
+    # Views of 20, 40, 50, 17, 128 and 84 spp: in passes, a view's samples per pass round up to
+    # a multiple of 16 (32 + 8, 32 + 18 for two passes), so passes mix whole 16-sample waves
+    # with a remainder; in the one persistent launch the views share one tile shape and an
+    # item's last round may be partial. Either way every view equals its single-view render
+    # bit for bit.
     base, views = batch_setup
-    picked = [(views[0][0], 20), (views[1][0], 40), (views[3][0], 50), (views[2][0], 17)]
+    picked = [(views[0][0], 20), (views[1][0], 40), (views[3][0], 50), (views[2][0], 17),
+              (views[4][0], 128), (views[5][0], 84)]
     b = open_backend(env)
     try:
         scene = b.upload(base)
@@ -210,6 +215,29 @@ def test_batch_passes_with_partial_sample_groups(batch_setup, env):
         for (p, spp), img in zip(picked, outs):
             single = b.render_hq_scene(scene, p, spp)
             assert np.array_equal(img.view("<u4"), single.view("<u4"))
+        scene.free()
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("spps", [(64, 128, 256, 192), (32, 96, 32, 96), (16, 256, 64, 1024)],
+                         ids=["one-pixel-tiles", "two-pixel-tiles", "mixed"])
+def test_batch_tile_shapes(batch_setup, spps):
+    # The batch's one launch takes every view's tile shape when they agree (one pixel's 64
+    # samples a round for multiples of 64 spp, two pixels' 32 for 32 and 96) and otherwise the
+    # cheapest shape for all; the single-view renders take each view's own. The images agree
+    # bit for bit all the same.
+    base, views = batch_setup
+    picked = [(views[i][0], s) for i, s in zip((3, 1, 0, 2), spps)]
+    b = open_backend({})
+    try:
+        scene = b.upload(base)
+        outs = b.render_hq_batch(scene, picked)
+        assert b.last_timing()["launches"] == 1
+        for (p, spp), img in zip(picked, outs):
+            single = b.render_hq_scene(scene, p, spp)
+            assert np.array_equal(img.view("<u4"), single.view("<u4"))
+            assert np.isfinite(img).all()
         scene.free()
     finally:
         b.close()

@@ -90,8 +90,9 @@ def test_per_view_arrays_are_checked_before_the_call():
 
 
 def _functions(text):
-    """(name, params, body) of each function definition in HIP/C++ source text (comments and
-    strings blanked), found by its parameter list and matched braces."""
+    """(name, params, body, is_kernel, head) of each function definition in HIP/C++ source
+    text (comments blanked), found by its parameter list and matched braces; head is the text
+    before the name back to the previous statement or block."""
     text = re.sub(r"//[^\n]*", "", text)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     out = []
@@ -117,8 +118,8 @@ def _functions(text):
         while k < len(text) and depth:
             depth += {"{": 1, "}": -1}.get(text[k], 0)
             k += 1
-        head = text[max(0, m.start() - 200):m.start()]
-        out.append((name, params, text[j + 1:k - 1], "__global__" in head.split(";")[-1].split("}")[-1]))
+        head = text[max(0, m.start() - 300):m.start()].split(";")[-1].split("}")[-1]
+        out.append((name, params, text[j + 1:k - 1], "__global__" in head, head))
     return out
 
 
@@ -131,7 +132,7 @@ def test_every_kernel_that_reads_its_arguments_again_takes_params_and_scene_firs
     for f in sorted(csrc.glob("*.hpp")) + sorted(csrc.glob("*.hip")):
         funcs += _functions(f.read_text())
     calls = {}
-    for name, _, body, _ in funcs:
+    for name, _, body, _, _ in funcs:
         calls.setdefault(name, set()).update(re.findall(r"\b([A-Za-z_]\w*)\s*[(<]", body))
     reach = {"kernel_args"}
     while True:
@@ -140,7 +141,7 @@ def test_every_kernel_that_reads_its_arguments_again_takes_params_and_scene_firs
             break
         reach |= more
     assert {"walk_step", "walk_begin", "trace", "sample_event", "render_sample"} <= reach
-    kernels = [(n, p, b) for n, p, b, g in funcs if g]
+    kernels = [(n, p, b) for n, p, b, g, _ in funcs if g]
     assert len(kernels) >= 30
     bad = []
     checked = 0
@@ -151,3 +152,8 @@ def test_every_kernel_that_reads_its_arguments_again_takes_params_and_scene_firs
             if first != ["const lw_params p", "const DevScene s"]:
                 bad.append((name, first))
     assert checked >= 12 and not bad, bad
+    # A called (not inlined) function has no kernel argument segment of its own to read, so
+    # every device function on the way to kernel_args is force-inlined into its kernel.
+    not_inlined = sorted({n for n, _, _, g, head in funcs
+                          if n in reach and not g and "__device__" in head and "__forceinline__" not in head})
+    assert not not_inlined, not_inlined
