@@ -154,8 +154,8 @@ def roofline(config, samples, trace_ms, launches, steps=1):
             "kernel": ("k_render_tiles (one persistent launch per frame)" if persistent else
                        "path-tracing kernels of a launch (k_render | k_paths | k_wf_*)"),
             "mean_launch_ms": trace_ms / launches, "bytes_per_sample": b_alg["bytes_per_sample"],
-            "note": "VALU-issue bound: the VALU issues on every cycle at 16.3% lane "
-                    "utilisation, L2 hit 96.6%; HBM traffic 326 B/sample, mostly register spill "
+            "note": "VALU-issue bound: the VALU issues on every cycle at 16.4% lane "
+                    "utilisation, L2 hit 97.2%; HBM traffic 298 B/sample, mostly register spill "
                     "(profiles/k_render_tiles_cfg1_720p_*.md)" if config == 1 else None}
 
 
