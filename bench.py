@@ -9,7 +9,9 @@ resident in HBM. W untimed warmup steps, then K timed steps; one JSON line.
 {160x90 .. 1280x720} and spp from {16 .. 1024} -- rendered once as one batch
 (lw_hip_render_hq_batch) and once as 64 plain render_hq calls; a step renders all 64 views
 both ways. The images are identical (same samples, same summation order), so the two are
-compared at equal image error.
+compared at equal image error. The views are also rendered once through the reference's
+per-view launch loop (render/cuda.rs:132-169) on the device (LW_FRAME_FORM=launches), again
+with identical images.
 
 --config 3: the differentiable render of the 8 x 8-chunk world with 16 emitter blocks
 (256 x 144, 64 spp, 4 bounces): a step is one forward and one adjoint (path replay) of an L2
@@ -43,7 +45,8 @@ Besides the throughput the config-1 line carries:
                 the f64 render on a subset of rows;
 and the metric's other two parts (leave them out with --no-metric-parts):
   batched       config 2 in full: 64 views as one batch and as 64 plain calls, the ratio
-                batched_over_plain and whether the images are identical;
+                batched_over_plain and whether the images are identical; and against the
+                reference's per-view launch loop, batched_over_launch_loop;
   adjoint       config 3's adjoint / forward kernel-time ratio, and one config-4 fit step's
                 forward and adjoint times.
 """
@@ -392,6 +395,7 @@ def bench_plain(args, lw, frames):
         line["speedup_vs_cpu_baseline"] = value / cpu["value"]
     if "batched" in extra:
         line["batched_over_plain"] = extra["batched"]["batched_over_plain"]
+        line["batched_over_launch_loop"] = extra["batched"]["batched_over_launch_loop"]
         line["adjoint_over_forward"] = extra["adjoint"]["cfg3"]["adjoint_over_forward"]
     return line
 
@@ -456,7 +460,30 @@ def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0, bound_subset
     finally:
         scene.free()
         b.close()
+    # The reference's per-view loop (render/cuda.rs:132-169: launches of 1, 2, 4, ... samples
+    # per pixel, a host synchronisation and an accumulation after each) on the device: the
+    # drop-in with LW_FRAME_FORM=launches, the same samples and the same images.
+    saved = os.environ.get("LW_FRAME_FORM")
+    os.environ["LW_FRAME_FORM"] = "launches"
+    try:
+        bl = lw.HipBackend.open(0)
+    finally:
+        if saved is None:
+            os.environ.pop("LW_FRAME_FORM", None)
+        else:
+            os.environ["LW_FRAME_FORM"] = saved
+    scene_l = bl.upload(base)
+    try:
+        bl.render_hq_scene(scene_l, *views[0])  # warmup
+        t = time.perf_counter()
+        loop_imgs = [bl.render_hq_scene(scene_l, p, s) for p, s in views]
+        t_loop = time.perf_counter() - t
+        log(f"per-view launch loop: {t_loop:.2f} s")
+    finally:
+        scene_l.free()
+        bl.close()
     same = all(np.array_equal(x.view("<u4"), y.view("<u4")) for x, y in zip(batch_imgs, plain_imgs))
+    same_loop = all(np.array_equal(x.view("<u4"), y.view("<u4")) for x, y in zip(batch_imgs, loop_imgs))
     return {"workload": "cfg2: 64 seeded views of the cfg1 world, sizes 160x90..1280x720, "
                         "spp 16..1024, 8 bounces, one batch vs 64 plain calls",
             "views": 64, "samples": samples, "steps": steps,
@@ -464,6 +491,8 @@ def batch_block(lw, frames, steps: int = 1, spp_scale: float = 1.0, bound_subset
             "plain_msamples_per_s": samples / t_plain / 1e6,
             "batched_s": t_batch, "plain_s": t_plain,
             "batched_over_plain": t_plain / t_batch, "images_identical": same,
+            "launch_loop_s": t_loop, "launch_loop_msamples_per_s": samples / t_loop / 1e6,
+            "batched_over_launch_loop": t_loop / t_batch, "launch_loop_images_identical": same_loop,
             "batch_passes": tb["launches"], "batch_trace_ms": tb["trace_ms"],
             "full_gpu_views": bound}
 
@@ -498,6 +527,9 @@ def bench_batch(args, lw, frames):
         "plain_msamples_per_s": blk["plain_msamples_per_s"],
         "batched_over_plain": blk["batched_over_plain"],
         "images_identical": blk["images_identical"],
+        "launch_loop_msamples_per_s": blk["launch_loop_msamples_per_s"],
+        "batched_over_launch_loop": blk["batched_over_launch_loop"],
+        "launch_loop_images_identical": blk["launch_loop_images_identical"],
         "batch_passes": blk["batch_passes"],
         "full_gpu_views": blk["full_gpu_views"],
         "roofline": batch_roofline(samples, blk),
