@@ -101,23 +101,41 @@ __global__ void __launch_bounds__(kBlock, W) k_render(const lw_params p, const D
   o[2] = c.z;
 }
 
-// A launch lane's pixel and sample within the launch. Without `slots`, a workgroup is a
-// 16 x 16 pixel tile (2 x 2 waves of 8 x 8) at sample blockIdx.y; with it, a workgroup is a
-// 4 x 4 pixel tile (2 x 2 waves of 2 x 2 pixels x 16 samples) at samples 16 blockIdx.y .. + 15,
-// for launches of a multiple of 16 samples per pixel (see k_render_slots).
+// A launch lane's pixel and sample within the launch, in layout L. L 0: a workgroup is a
+// 16 x 16 pixel tile (2 x 2 waves of 8 x 8) at sample blockIdx.y. L 1: a 4 x 4 pixel tile
+// (2 x 2 waves of 2 x 2 pixels x 16 samples) at samples 16 blockIdx.y .. + 15, for launches of
+// a multiple of 16 samples per pixel. L 2: a 2 x 2 pixel tile (a wave per pixel, its 64
+// lanes 64 samples) at samples 64 blockIdx.y .. + 63, for launches of a multiple of 64; the
+// wave's pixel is then the same for all its lanes (scalar registers). The layouts differ only
+// in which lane renders a (pixel, sample), never in its value.
 struct LaunchLane {
   uint32_t px, py, sl;
 };
-__device__ __forceinline__ LaunchLane launch_lane(uint32_t w, uint32_t slots) {
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  if (slots) {
+template <int L>
+__device__ __forceinline__ LaunchLane launch_lane(uint32_t w) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if constexpr (L == 2) {
+    const uint32_t tx = (w + 1u) / 2u;
+    return LaunchLane{(blockIdx.x % tx) * 2u + (wave & 1u), (blockIdx.x / tx) * 2u + (wave >> 1),
+                      blockIdx.y * 64u + lane};
+  } else if constexpr (L == 1) {
     const uint32_t tx = (w + 3u) / 4u, pix = lane & 3u;
     return LaunchLane{(blockIdx.x % tx) * 4u + (wave & 1u) * 2u + (pix & 1u),
                       (blockIdx.x / tx) * 4u + (wave >> 1) * 2u + (pix >> 1), blockIdx.y * 16u + (lane >> 2)};
+  } else {
+    const uint32_t tx = (w + 15u) / 16u;
+    return LaunchLane{(blockIdx.x % tx) * 16u + (wave & 1u) * 8u + (lane & 7u),
+                      (blockIdx.x / tx) * 16u + (wave >> 1) * 8u + (lane >> 3), blockIdx.y};
   }
-  const uint32_t tx = (w + 15u) / 16u;
-  return LaunchLane{(blockIdx.x % tx) * 16u + (wave & 1u) * 8u + (lane & 7u),
-                    (blockIdx.x / tx) * 16u + (wave >> 1) * 8u + (lane >> 3), blockIdx.y};
+}
+// The layout for a launch of nb samples per pixel, and its grid.
+__host__ __forceinline__ uint32_t launch_layout(uint32_t nb) {
+  return nb % 64u == 0 ? 2u : (nb % 16u == 0 ? 1u : 0u);
+}
+__host__ __forceinline__ dim3 launch_grid(uint32_t lay, uint32_t w, uint32_t h, uint32_t nb) {
+  if (lay == 2u) return dim3(((w + 1) / 2) * ((h + 1) / 2), nb / 64u);
+  if (lay == 1u) return dim3(((w + 3) / 4) * ((h + 3) / 4), nb / 16u);
+  return dim3(((w + 15) / 16) * ((h + 15) / 16), nb);
 }
 
 // k_render with 64-lane waves of 2 x 2 pixels x 16 samples, for launches of a multiple of 16
@@ -126,18 +144,15 @@ __device__ __forceinline__ LaunchLane launch_lane(uint32_t w, uint32_t slots) {
 // of 4 neighbouring pixels, whose first walks and bounces stay close together (as in
 // k_render_tiles). The staging is indexed as k_render's, so the launch's accumulation is
 // unchanged.
-template <int W>
+template <int W, int L>
 __global__ void __launch_bounds__(kBlock, W) k_render_slots(const lw_params p, const DevScene s,
                                                          uint32_t sample_start, float *__restrict__ rad) {
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
   const uint32_t w = p.frame[0], h = p.frame[1];
-  const uint32_t tiles_x = (w + 3u) / 4u;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, pix = lane & 3u, slot = lane >> 2;
-  const uint32_t px = (blockIdx.x % tiles_x) * 4u + (wave & 1u) * 2u + (pix & 1u);
-  const uint32_t py = (blockIdx.x / tiles_x) * 4u + (wave >> 1) * 2u + (pix >> 1);
+  const LaunchLane Ln = launch_lane<L>(w);
+  const uint32_t px = Ln.px, py = Ln.py, sl = Ln.sl;
   if (px >= w || py >= h) return;
-  const uint32_t sl = blockIdx.y * 16u + slot;
   const V3 c = render_sample(s, p, lut_s, px, py, sample_start + sl);
   float *o = rad + ((size_t)sl * h * w + (size_t)py * w + px) * 3;
   o[0] = c.x;
@@ -1057,15 +1072,15 @@ __global__ void __launch_bounds__(kBlock, W) k_wf_trace(const lw_params p, const
 // ---- differentiable rendering (lw_diff.hpp) ----
 
 // The forward under the parameters: k_render with render_sample_diff.
-template <int W>
+template <int W, int L>
 __global__ void __launch_bounds__(kBlock, W) k_render_diff(const lw_params p, const DevScene s,
                                                         const DiffDev D, uint32_t sample_start,
-                                                        uint32_t slots, float *__restrict__ rad) {
+                                                        float *__restrict__ rad) {
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
   const uint32_t w = p.frame[0], h = p.frame[1];
-  const LaunchLane L = launch_lane(w, slots);
-  const uint32_t px = L.px, py = L.py, sl = L.sl;
+  const LaunchLane Ln = launch_lane<L>(w);
+  const uint32_t px = Ln.px, py = Ln.py, sl = Ln.sl;
   if (px >= w || py >= h) return;
   const GradSink G{nullptr, 0u, 0u};
   const SuffixList sfx{nullptr, 0u};
@@ -1080,10 +1095,10 @@ __global__ void __launch_bounds__(kBlock, W) k_render_diff(const lw_params p, co
 // The adjoint: each lane replays one sample with dl = dL/dI(pixel) / spp; the workgroup
 // sums its gradients in LDS (f64) and adds them to D.grads once per slot. Dynamic LDS:
 // diff_slots(D) doubles.
-template <int W>
+template <int W, int L>
 __global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const DevScene s,
                                                     const DiffDev D, uint32_t sample_start,
-                                                    uint32_t slots, float inv_spp) {
+                                                    float inv_spp) {
   __shared__ float lut_s[256];
   __shared__ uint32_t sfx_s[kMaxSuffix * kBlock];
   extern __shared__ double grad_s[];
@@ -1091,15 +1106,15 @@ __global__ void __launch_bounds__(kBlock, W) k_adjoint(const lw_params p, const 
   for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) grad_s[i] = 0.0;
   load_lut(lut_s, s.gamma_lut);  // ends with a barrier
   const uint32_t w = p.frame[0], h = p.frame[1];
-  const LaunchLane L = launch_lane(w, slots);
-  const uint32_t px = L.px, py = L.py;
+  const LaunchLane Ln = launch_lane<L>(w);
+  const uint32_t px = Ln.px, py = Ln.py;
   if (px < w && py < h) {
     const float *g = D.dldi + ((size_t)py * w + px) * 3;
     const V3 dl = V3{g[0], g[1], g[2]} * inv_spp;
     if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
       const GradSink G{grad_s, D.n_pal, D.n_int};
       const SuffixList sfx{sfx_s + threadIdx.x, (uint32_t)kBlock};
-      render_sample_diff<true>(s, p, p, lut_s, D, px, py, sample_start + L.sl, G, sfx, dl);
+      render_sample_diff<true>(s, p, p, lut_s, D, px, py, sample_start + Ln.sl, G, sfx, dl);
     }
   }
   __syncthreads();
@@ -1152,20 +1167,31 @@ __global__ void __launch_bounds__(kBlock, W) k_render_multi(const lw_params p, c
 // tiles x n workgroups (n = the view's samples this pass), workgroup q tiles + t standing for
 // 16 x 16 tile t at sample q (k_render_batch's layout). When n is a multiple of 16, the 16
 // workgroups 16 g .. 16 g + 15 of a tile are instead its sixteen 4 x 4 sub-tiles at samples
-// 16 g .. 16 g + 15, with waves of 2 x 2 pixels x 16 samples (see k_render_slots): the same
-// (pixel, sample) pairs, so the staging and its accumulation are unchanged.
+// 16 g .. 16 g + 15, with waves of 2 x 2 pixels x 16 samples (see launch_lane); with L 2, when
+// every view's n is a multiple of 64, the 64 workgroups 64 g .. 64 g + 63 are its 2 x 2
+// sub-tiles at samples 64 g .. 64 g + 63, a wave per pixel. The same (pixel, sample) pairs
+// either way, so the staging and its accumulation are unchanged.
 struct BatchItem {
   uint32_t v, sl, px, py;
   bool in_frame;
 };
+template <int L>
 __device__ __forceinline__ BatchItem batch_item(const BatchDev *__restrict__ bd, uint32_t base) {
   BatchItem it;
   it.v = find_view(bd->item_prefix, bd->nviews, base);
   const uint32_t rest = (base - bd->item_prefix[it.v]) >> 8;
   const uint32_t tiles = bd->tiles[it.v], tx = bd->tiles_x[it.v];
   const uint32_t tile = rest % tiles, q = rest / tiles;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  if (bd->n[it.v] % 16u == 0) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if constexpr (L == 2) {
+    // every view's samples this pass a multiple of 64 (the host checks): workgroups
+    // 64 g .. 64 g + 63 of a tile are its sixty-four 2 x 2 sub-tiles at samples 64 g .. + 63,
+    // a wave per pixel
+    const uint32_t sub = q & 63u;
+    it.sl = (q & ~63u) + lane;
+    it.px = (tile % tx) * 16u + (sub & 7u) * 2u + (wave & 1u);
+    it.py = (tile / tx) * 16u + (sub >> 3) * 2u + (wave >> 1);
+  } else if (bd->n[it.v] % 16u == 0) {
     const uint32_t sub = q & 15u, pix = lane & 3u;
     it.sl = (q & ~15u) + (lane >> 2);
     it.px = (tile % tx) * 16u + (sub & 3u) * 4u + (wave & 1u) * 2u + (pix & 1u);
@@ -1180,7 +1206,7 @@ __device__ __forceinline__ BatchItem batch_item(const BatchDev *__restrict__ bd,
 }
 
 // The differentiable forward of a batch pass (k_render_batch with render_sample_diff).
-template <int W>
+template <int W, int L>
 __global__ void __launch_bounds__(kBlock, W) k_render_diff_batch(const lw_params p, const DevScene s,
                                                               const DiffDev D,
                                                               const BatchDev *__restrict__ bd,
@@ -1189,7 +1215,7 @@ __global__ void __launch_bounds__(kBlock, W) k_render_diff_batch(const lw_params
   if (base >= bd->total) return;
   __shared__ float lut_s[256];
   load_lut(lut_s, s.gamma_lut);
-  const BatchItem it = batch_item(bd, base);
+  const BatchItem it = batch_item<L>(bd, base);
   if (!it.in_frame) return;
   const ViewCam cam = bd->cam[it.v];
   const GradSink G{nullptr, 0u, 0u};
@@ -1205,7 +1231,7 @@ __global__ void __launch_bounds__(kBlock, W) k_render_diff_batch(const lw_params
 
 // The adjoint of a batch pass: D.dldi holds every view's dL/dI, view v's pixels from
 // pix_prefix[v]; each sample's dl is its pixel's dL/dI / the view's spp.
-template <int W>
+template <int W, int L>
 __global__ void __launch_bounds__(kBlock, W) k_adjoint_batch(const lw_params p, const DevScene s,
                                                           const DiffDev D,
                                                           const BatchDev *__restrict__ bd) {
@@ -1217,7 +1243,7 @@ __global__ void __launch_bounds__(kBlock, W) k_adjoint_batch(const lw_params p, 
   const uint32_t ns = diff_slots(D);
   for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) grad_s[i] = 0.0;
   load_lut(lut_s, s.gamma_lut);  // ends with a barrier
-  const BatchItem it = batch_item(bd, base);
+  const BatchItem it = batch_item<L>(bd, base);
   if (it.in_frame) {
     const ViewCam cam = bd->cam[it.v];
     const float *g = D.dldi + ((size_t)bd->pix_prefix[it.v] + (size_t)it.py * cam.width + it.px) * 3;
@@ -1947,6 +1973,21 @@ void build_scene(lw_hip_backend *b, const lw_voxel_scene &v, lw_hip_scene &sc, l
     }                                                                         \
   } while (0)
 
+// A launch layout L (launch_lane) from its runtime value.
+#define LW_LAYOUT_DISPATCH(lay, KERNEL_EXPR)                                  \
+  do {                                                                        \
+    if ((lay) == 2u) {                                                        \
+      constexpr int L = 2;                                                    \
+      KERNEL_EXPR;                                                            \
+    } else if ((lay) == 1u) {                                                 \
+      constexpr int L = 1;                                                    \
+      KERNEL_EXPR;                                                            \
+    } else {                                                                  \
+      constexpr int L = 0;                                                    \
+      KERNEL_EXPR;                                                            \
+    }                                                                         \
+  } while (0)
+
 // The differentiable kernels: 4 waves per SIMD, or free (LW_WAVES=1).
 #define LW_DIFF_DISPATCH(waves, KERNEL_EXPR)   \
   do {                                         \
@@ -2255,19 +2296,26 @@ void render_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &pa
     const auto start = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (diff) {
-      const uint32_t by16 = nb % 16u == 0 ? 1u : 0u;  // 2 x 2 pixels x 16 samples a wave
-      const dim3 grid = by16 ? dim3(((w + 3) / 4) * ((h + 3) / 4), nb / 16u) : dim3(tiles, nb);
-      LW_DIFF_DISPATCH(b->waves, (k_render_diff<W><<<grid, dim3(kBlock), 0, b->stream>>>(
-                                     p, ds, *diff, done, by16, rad)));
+      const uint32_t lay = launch_layout(nb);
+      const dim3 grid = launch_grid(lay, w, h, nb);
+      LW_LAYOUT_DISPATCH(lay, LW_DIFF_DISPATCH(b->waves, (k_render_diff<W, L><<<grid, dim3(kBlock), 0, b->stream>>>(
+                                                              p, ds, *diff, done, rad))));
     } else if (mode == MODE_DIRECT && b->per_lane > 1) {
       const uint32_t per = std::min(b->per_lane, nb);
       LW_DIRECT_DISPATCH(b->waves, (k_render_multi<W><<<dim3(tiles, (nb + per - 1) / per),
                                                          dim3(kBlock), 0, b->stream>>>(
                                        p, ds, done, nb, per, rad)));
     } else if (mode == MODE_DIRECT && nb % 16u == 0 && !b->xcd_remap) {
-      const uint32_t tiles4 = ((w + 3) / 4) * ((h + 3) / 4);
-      LW_DIRECT_DISPATCH(b->waves, (k_render_slots<W><<<dim3(tiles4, nb / 16u), dim3(kBlock), 0, b->stream>>>(
-                                       p, ds, done, rad)));
+      // a wave per pixel's 64 samples, or 2 x 2 pixels' 16
+      const uint32_t lay = launch_layout(nb);
+      const dim3 grid = launch_grid(lay, w, h, nb);
+      if (lay == 2u) {
+        LW_DIRECT_DISPATCH(b->waves, (k_render_slots<W, 2><<<grid, dim3(kBlock), 0, b->stream>>>(
+                                         p, ds, done, rad)));
+      } else {
+        LW_DIRECT_DISPATCH(b->waves, (k_render_slots<W, 1><<<grid, dim3(kBlock), 0, b->stream>>>(
+                                         p, ds, done, rad)));
+      }
     } else if (mode == MODE_DIRECT) {
       LW_DIRECT_DISPATCH(b->waves, (k_render<W><<<dim3(tiles, nb), dim3(kBlock), 0, b->stream>>>(
                                        p, ds, done, b->xcd_remap, rad)));
@@ -2406,6 +2454,15 @@ BatchPlan plan_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *v
   return plan;
 }
 
+// Whether every pass of a planned batch renders a multiple of 64 samples of each view (its
+// samples per pass and its spp both multiples of 64), so that its passes can run a wave per
+// pixel (batch_item<2>).
+bool batch_by64(const BatchDev &bd) {
+  for (uint32_t v = 0; v < bd.nviews; v++)
+    if (bd.per_pass[v] % 64u != 0 || bd.cam[v].spp % 64u != 0) return false;
+  return bd.nviews > 0;
+}
+
 // With `diff`, the samples are render_sample_diff's (direct form). Without `outs`, the views'
 // f64 sums stay in b->acc (laid out by pix_prefix) for the caller.
 void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *views, size_t nv,
@@ -2445,8 +2502,13 @@ void render_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_view *view
     k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, st>>>(b->batch_dev.ptr);
     k_fill_u32<<<dim3(1), dim3(64), 0, st>>>(b->counter.ptr, 0u);
     if (diff) {
-      LW_DIFF_DISPATCH(b->waves, (k_render_diff_batch<W><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
-                                     p0, ds, *diff, b->batch_dev.ptr, b->rad.ptr)));
+      if (batch_by64(bd)) {
+        LW_DIFF_DISPATCH(b->waves, (k_render_diff_batch<W, 2><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
+                                       p0, ds, *diff, b->batch_dev.ptr, b->rad.ptr)));
+      } else {
+        LW_DIFF_DISPATCH(b->waves, (k_render_diff_batch<W, 0><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
+                                       p0, ds, *diff, b->batch_dev.ptr, b->rad.ptr)));
+      }
     } else if (b->mode == MODE_DIRECT) {
       LW_DIRECT_DISPATCH(b->waves, (k_render_batch<W><<<dim3(direct_blocks), dim3(kBlock), 0, st>>>(
                                        p0, ds, b->batch_dev.ptr, b->rad.ptr)));
@@ -2648,10 +2710,10 @@ void adjoint_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   for (uint32_t done = 0; done < spp;) {
     const uint32_t nb = std::min(spp - done, kMaxBatch);
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    const uint32_t by16 = nb % 16u == 0 ? 1u : 0u;  // 2 x 2 pixels x 16 samples a wave
-    const dim3 grid = by16 ? dim3(((w + 3) / 4) * ((h + 3) / 4), nb / 16u) : dim3(tiles, nb);
-    LW_DIFF_DISPATCH(b->waves, (k_adjoint<W><<<grid, dim3(kBlock), lds, b->stream>>>(
-                                   p, ds, D, done, by16, inv_spp)));
+    const uint32_t lay = launch_layout(nb);
+    const dim3 grid = launch_grid(lay, w, h, nb);
+    LW_LAYOUT_DISPATCH(lay, LW_DIFF_DISPATCH(b->waves, (k_adjoint<W, L><<<grid, dim3(kBlock), lds, b->stream>>>(
+                                                            p, ds, D, done, inv_spp))));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(hipEventSynchronize(b->ev[1]));
@@ -2687,8 +2749,13 @@ void adjoint_batch(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   HIP_TRY(hipEventRecord(b->ev[2], b->stream));
   for (uint32_t k = 0; k < plan.npass; k++) {
     k_batch_plan<<<dim3(1), dim3(kMaxViews), 0, b->stream>>>(b->batch_dev.ptr);
-    LW_DIFF_DISPATCH(b->waves, (k_adjoint_batch<W><<<dim3(blocks), dim3(kBlock), lds, b->stream>>>(
-                                   p0, ds, D, b->batch_dev.ptr)));
+    if (batch_by64(plan.bd())) {
+      LW_DIFF_DISPATCH(b->waves, (k_adjoint_batch<W, 2><<<dim3(blocks), dim3(kBlock), lds, b->stream>>>(
+                                     p0, ds, D, b->batch_dev.ptr)));
+    } else {
+      LW_DIFF_DISPATCH(b->waves, (k_adjoint_batch<W, 0><<<dim3(blocks), dim3(kBlock), lds, b->stream>>>(
+                                     p0, ds, D, b->batch_dev.ptr)));
+    }
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(b->ev[3], b->stream));
