@@ -236,8 +236,22 @@ def batch_problem():
                                                        cutout_fraction=0.05))
 
 
-def test_batched_forward_equals_single_views(hip_backend, batch_problem):
-    bp = batch_problem
+@pytest.fixture(scope="module")
+def batch_problem_64():
+    # 64 spp per view in one pass: the passes run a wave per pixel's 64 samples
+    return diff.config4_problem(n_views=3, size=(72, 40), spp=64, n_groups=16,
+                                world=frames.WorldSpec(8, 8, 24, cube_types=24, cutout_types=4,
+                                                       water_level=62, emitter_fraction=0.01,
+                                                       cutout_fraction=0.05))
+
+
+@pytest.fixture(params=["8spp", "64spp"])
+def any_batch_problem(request, batch_problem, batch_problem_64):
+    return batch_problem if request.param == "8spp" else batch_problem_64
+
+
+def test_batched_forward_equals_single_views(hip_backend, any_batch_problem):
+    bp = any_batch_problem
     sc = hip_backend.upload(bp.base)
     outs = hip_backend.render_diff_batch(sc, bp.views, bp.truth.desc())
     for (p, spp), img in zip(bp.views, outs):
@@ -246,8 +260,8 @@ def test_batched_forward_equals_single_views(hip_backend, batch_problem):
     sc.free()
 
 
-def test_batched_adjoint_is_the_sum_of_the_views(hip_backend, batch_problem):
-    bp = batch_problem
+def test_batched_adjoint_is_the_sum_of_the_views(hip_backend, any_batch_problem):
+    bp = any_batch_problem
     sc = hip_backend.upload(bp.base)
     rng = np.random.default_rng(2)
     Gs = [rng.standard_normal(int(p["frame"][0]) * int(p["frame"][1]) * 3).astype("<f4")
