@@ -1475,8 +1475,10 @@ struct lw_hip_backend {
   void *scratch = nullptr;  // this device's scratch array (hipGetSymbolAddress), or null
   uint64_t last_iterations = 0;
   // Batches: the staging cap, a minimum pass count (0: as few as fit) and whether a pass
-  // replays as a captured hipGraph (LW_BATCH_GRAPHS=0 launches it kernel by kernel).
-  uint64_t batch_staging = 2ull << 30;
+  // replays as a captured hipGraph (LW_BATCH_GRAPHS=0 launches it kernel by kernel). 8 GiB of
+  // the 288 GB HBM lets the cfg4 fit (32 views of 640 x 360 at 128 spp, 88.5 MB of staging
+  // per sample per pixel) run in 2 passes of 64 samples, a wave per pixel, instead of 4 of 32.
+  uint64_t batch_staging = 8ull << 30;
   uint32_t batch_passes = 0;
   bool use_graphs = true;
   // LW_BATCH_FORM=passes: the direct form's batches in passes (k_render_batch) rather than in
