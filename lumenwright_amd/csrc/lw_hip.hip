@@ -2128,8 +2128,9 @@ bool uses_tiles(const lw_hip_backend *b, uint32_t w, uint32_t h, uint32_t spp) {
   const uint64_t tile_items = tile_count(w, h, sh);
   const uint64_t waves = (uint64_t)b->blocks_tiles * (kBlock / 64);
   const uint64_t rounds = tile_items * ((spp + (64u >> sh) - 1) / (64u >> sh));
+  // (a frame of 2^24 tiles or more, which an item cannot name, takes the launch ramp)
   return b->mode == MODE_DIRECT && b->per_lane == 1 && !b->frame_launches && spp >= kTilesMinSpp &&
-         tile_items >= waves && rounds >= (uint64_t)kTilesPerWave * waves;
+         tile_items >= waves && rounds >= (uint64_t)kTilesPerWave * waves && tile_items < (1u << 24);
 }
 
 // A frame in one persistent launch (k_render_tiles over its tiles, all samples of a tile in one

@@ -194,6 +194,28 @@ def test_tile_kernel_frame_with_ragged_edges_and_a_partial_sample_group(spp):
         ramp.close()
 
 
+def test_a_frame_of_2_24_pixels_takes_two_pixel_tiles():
+    # 4096 x 4096 at 64 spp would be 2^24 one-pixel tiles, more than an item can name: the frame
+    # takes tiles of two pixels (still one launch) and equals the launch ramp's image.
+    cfg = frames.config(1, width=4096, height=4096, spp=64, max_bounces=1, world_chunks_x=6,
+                        world_chunks_z=6)
+    frame = frames.make_frame(cfg)
+    tiles = open_backend({})
+    ramp = open_backend({"LW_FRAME_FORM": "launches"})
+    try:
+        s1, s2 = tiles.upload(frame), ramp.upload(frame)
+        a = tiles.render_hq_scene(s1, frame.params, cfg.spp)
+        assert tiles.last_timing()["launches"] == 1
+        b = ramp.render_hq_scene(s2, frame.params, cfg.spp)
+        assert ramp.last_timing()["launches"] > 1
+        assert np.array_equal(a.view("<u4"), b.view("<u4"))
+        s1.free()
+        s2.free()
+    finally:
+        tiles.close()
+        ramp.close()
+
+
 @pytest.mark.parametrize("env", [{}, {"LW_BATCH_FORM": "passes", "LW_BATCH_PASSES": "2"},
                                  {"LW_BATCH_FORM": "passes", "LW_BATCH_PASSES": "3"}],
                          ids=["tiles", "2-passes", "3-passes"])
