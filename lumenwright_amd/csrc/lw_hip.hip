@@ -59,8 +59,8 @@ namespace {
 
 constexpr int kBlock = 256;  // 4 waves of 64: a 16 x 16 pixel tile
 // The persistent tile kernel's work item: a tile of one view's pixels with all their samples,
-// rendered by one wave as 2^sh pixels x (64 >> sh) sample slots. The tile shape sh is per
-// view: 0 is one pixel (1 x 1), 1 two pixels side by side (2 x 1), 2 a 2 x 2 square.
+// rendered by one wave as 2^sh pixels x (64 >> sh) sample slots. The tile shape sh, one for a
+// whole launch: 0 is one pixel (1 x 1), 1 two pixels side by side (2 x 1), 2 a 2 x 2 square.
 constexpr __host__ __device__ uint32_t tile_w(uint32_t sh) { return sh ? 2u : 1u; }
 constexpr __host__ __device__ uint32_t tile_h(uint32_t sh) { return sh == 2u ? 2u : 1u; }
 __host__ __device__ __forceinline__ uint64_t tile_count(uint32_t w, uint32_t h, uint32_t sh) {
@@ -493,18 +493,17 @@ __global__ void __launch_bounds__(kBlock, W) k_render_batch(const lw_params p, c
 
 // The direct form in one persistent launch, for a frame or a batch of views. A work item is one
 // tile of one view with all of its samples (view << 24 | tile, tiles numbered row by row;
-// without `items`, item k is tile k of view 0), its shape sh the view's (tile_w, tile_h).
-// Whole waves take items from a device counter. A wave's lanes are the tile's 2^sh pixels x
-// (64 >> sh) sample slots: in round r, slot j renders sample (64 >> sh) r + j of its pixel, so a
-// round's paths are jittered samples of one pixel (or of 2 or 4 neighbouring pixels), whose
-// first walks and first bounces stay close together: at 256 spp one pixel's 64 samples per
-// round run 3.4% faster than 2 x 2 pixels' 16. The lanes leave their results in the wave's
-// row of LDS, and the pixel's slot-0 lane adds the
-// finite ones to the pixel's f64 sums in sample order (the CPU backend's order, so the image is
-// render_hq's bit for bit, render/cpu/mod.rs:66-79), kept in LDS; it writes the pixel's mean at
-// the end. No staging and no accumulation pass. A wave leaves once the counter has passed the
-// last item, so every wave ends. With `progress_host`, the index of each item taken is stored
-// to host memory for the caller's progress reports.
+// without `items`, item k is tile k of view 0), in the launch's tile shape SH (tile_w, tile_h).
+// Whole waves take items from a device counter. A wave's lanes are the tile's 2^SH pixels x
+// (64 >> SH) sample slots: in round r, slot j renders sample (64 >> SH) r + j of its pixel, so
+// a round's paths are jittered samples of one pixel (or of 2 or 4 neighbouring pixels), whose
+// first walks and first bounces stay close together. The lanes leave their results in the
+// wave's row of LDS, and the pixel's slot-0 lane adds the finite ones to the pixel's f64 sums
+// in sample order (the CPU backend's order, so the image is render_hq's bit for bit,
+// render/cpu/mod.rs:66-79), kept in LDS; it writes the pixel's mean at the end. No staging and
+// no accumulation pass. A wave leaves once the counter has passed the last item, so every wave
+// ends. With `progress_host`, the index of each item taken is stored to host memory for the
+// caller's progress reports.
 // One item of k_render_tiles: the tile `tile` of view v in shape SH (tile_w, tile_h), all of
 // its samples, its means written to `out`.
 template <int SH>
