@@ -1454,10 +1454,10 @@ struct lw_hip_backend {
   // against 528 at 5, 516 at 7 and 412 at 8. The persistent form, which has no 6-wave variant, then
   // runs at its own best, 4.
   int waves = 6, trace_waves = 1;
-  // LW_TILES_WAVES (or LW_WAVES): the persistent tile kernel's budget. With the model-free cube
-  // jump it measured fastest at 5 waves per SIMD: 922 Msamples/s at the 720p 256-spp frame
-  // against 916 at 4, 882 at 6 and 650 at 8 (the launch ramp's k_render stays fastest at 6: 851
-  // against 815 at 5).
+  // LW_TILES_WAVES (or LW_WAVES): the persistent tile kernel's budget, 4 to 6 waves per SIMD
+  // (other values are brought into that range). 5 measured fastest: 1146 Msamples/s at the
+  // 720p 256-spp frame with one-pixel tiles against 1094 at 4 and 1088 at 6 (the launch ramp's
+  // k_render stays fastest at 6).
   int tiles_waves = 5;
   // LW_XCD_REMAP=1: the direct kernel gives each XCD a contiguous run of tiles.
   uint32_t xcd_remap = 0;
