@@ -2705,7 +2705,6 @@ void adjoint_frame(lw_hip_backend *b, const lw_hip_scene &sc, const lw_params &p
   k_zero<<<dim3((slots + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream>>>(D.grads, slots);
   HIP_TRY(hipGetLastError());
   const DevScene ds = sc.dev();
-  const uint32_t tiles = ((w + 15) / 16) * ((h + 15) / 16);
   const float inv_spp = 1.0f / (float)spp;
   const size_t lds = (size_t)slots * sizeof(double);
   lw_timing tm{};
