@@ -194,6 +194,25 @@ def test_tile_kernel_frame_with_ragged_edges_and_a_partial_sample_group(spp):
         ramp.close()
 
 
+@pytest.mark.parametrize("spp", [64, 96, 80], ids=["one-pixel", "two-pixel", "2x2"])
+def test_tile_kernel_frame_matches_the_cpu_backend(oracle, spp):
+    # 160 x 128 pixels of the config-1 world with water and emitters: enough tiles and rounds
+    # for the persistent tile kernel in each shape (one pixel at 64 spp, two at 96, 2 x 2 at
+    # 80). Its image is the CPU backend's (render/cpu/mod.rs) bit for bit.
+    cfg = frames.config(1, width=160, height=128, spp=spp, world_chunks_x=6, world_chunks_z=6)
+    frame = frames.make_frame(cfg)
+    b = open_backend({})
+    try:
+        sc = b.upload(frame)
+        img = b.render_hq_scene(sc, frame.params, spp)
+        assert b.last_timing()["launches"] == 1
+        want = oracle.render_hq(frame.view(), spp).reshape(-1, 3)
+        assert np.array_equal(img.view("<u4"), want.view("<u4"))
+        sc.free()
+    finally:
+        b.close()
+
+
 def test_a_frame_of_2_24_pixels_takes_two_pixel_tiles():
     # 4096 x 4096 at 64 spp would be 2^24 one-pixel tiles, more than an item can name: the frame
     # takes tiles of two pixels (still one launch) and equals the launch ramp's image.
