@@ -3250,7 +3250,8 @@ int lw_hip_render_diff(lw_hip_backend *b, const lw_hip_scene *scene, const lw_pa
   if (!b || !scene || !params || !diff || !out_rgb) return LW_EINVAL;
   return guarded(b, [&] {
     const DiffDev D = prepare_diff(b, *scene, *params, *diff);
-    render_frame(b, *scene, *params, spp, nullptr, nullptr, out_rgb, &D, out_f64);
+    // No progress to report: launches of up to kMaxBatch samples per pixel, as the adjoint's.
+    render_frame(b, *scene, *params, spp, nullptr, nullptr, out_rgb, &D, out_f64, true);
   });
 }
 
